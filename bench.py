@@ -1,0 +1,227 @@
+"""Walker-evaluation throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload medium] [--walkers N]
+                    [--mode bf16x3|fp32|fp64] [--no-cpu-baseline]
+
+One step is one walker evaluation of every walker of the batch through the C-ABI, with
+the inputs already in device memory:
+    qmckl_amd_set_electron_coord_device    (qmckl_set_electron_coord)
+    qmckl_amd_get_mo_basis_mo_vgl_device   (AO VGL -> MO VGL)
+    qmckl_amd_get_jastrow_champ_value_device
+    qmckl_amd_get_jastrow_champ_gl_device
+The step is timed for the FP64 product (the reference's arithmetic) and for the
+split-precision AO->MO product (--mode, default bf16x3); `value` is the split-precision
+throughput.  The MO error of the split product against the FP64 one is reported per VGL
+component, together with per-kernel times measured with HIP events on the library's
+stream, the roofline position of the dominant kernel and the CPU oracle's throughput on a
+bounded sample of the same workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SLOTS = ["ao_vgl", "mo_gemm", "jastrow_pair", "jastrow_een", "jastrow_total"]
+PEAK = {"hbm_GBs": 8000.0, "fp64_TFs": 78.6, "fp32_TFs": 157.3, "bf16_TFs": 2500.0}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def per_walker_costs(s):
+    """Algorithmic work per walker (SURVEY-independent restatement of the formulas):
+    AO bytes = 8 (5 Ne Nao + 3 Ne); MO flops = 2*5*Ne*Nao*Nmo;
+    e-e-n flops = 2 Ne^2 Nn * 5 T with T = c(c-1)/2 (value + 4 derivative components)."""
+    Ne, Nao, Nmo, Nn, c = s.elec_num, s.ao_num, s.mo_num_kept, s.nucl_num, s.cord
+    T = c * (c - 1) // 2
+    return {
+        "ao_bytes": 8.0 * (5 * Ne * Nao + 3 * Ne),
+        "mo_flops": 2.0 * 5 * Ne * Nao * Nmo,
+        "een_flops": 2.0 * Ne * Ne * Nn * 5 * T,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="medium")
+    ap.add_argument("--walkers", type=int, default=None)
+    ap.add_argument("--mode", default="bf16x3", choices=["bf16x3", "fp32", "fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: walkers are independent and this build measures one GPU's "
+              f"share (--gpus {args.gpus} is not supported).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import qmckl_amd as q
+    from qmckl_amd import system as S
+
+    L = q.lib()
+    log(f"building workload {args.workload}")
+    s, x, W = S.workload(args.workload, walk_num=args.walkers)
+    Ne, Nmo = s.elec_num, s.mo_num_kept
+    P = W * Ne
+    log(f"Nn={s.nucl_num} Ne={Ne} Nao={s.ao_num} Nmo={Nmo} W={W} precision={s.precision}")
+    ctx = q.context_create()
+    S.load(ctx, s)
+    S.set_coords(ctx, s, x, W)
+    xd = torch.tensor(x, dtype=torch.float64, device="cuda")
+    mo = torch.empty(P * 5 * Nmo, dtype=torch.float64, device="cuda")
+    mo64 = torch.empty_like(mo)
+    val = torch.empty(W, dtype=torch.float64, device="cuda")
+    gl = torch.empty(W * 4 * Ne, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        q.check(ctx, L.qmckl_amd_set_electron_coord_device(ctx, b"T", W, xd.data_ptr(), xd.numel()))
+        q.check(ctx, L.qmckl_amd_get_mo_basis_mo_vgl_device(ctx, mo.data_ptr(), mo.numel()))
+        q.check(ctx, L.qmckl_amd_get_jastrow_champ_value_device(ctx, val.data_ptr(), val.numel()))
+        q.check(ctx, L.qmckl_amd_get_jastrow_champ_gl_device(ctx, gl.data_ptr(), gl.numel()))
+
+    def timed(mode):
+        q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, mode))
+        for _ in range(args.warmup):
+            step()
+        q.synchronize(ctx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        q.synchronize(ctx)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    def kernel_times(mode, n=3):
+        q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, mode))
+        step()
+        q.synchronize(ctx)
+        ms = np.zeros(5)
+        cnt = np.zeros(5, np.int64)
+        q.check(ctx, L.qmckl_amd_set_profiling(ctx, 1))
+        q.check(ctx, L.qmckl_amd_get_kernel_times(ctx, q.ptr(ms), q.ptr(cnt), 5))
+        for _ in range(n):
+            step()
+        q.check(ctx, L.qmckl_amd_get_kernel_times(ctx, q.ptr(ms), q.ptr(cnt), 5))
+        q.check(ctx, L.qmckl_amd_set_profiling(ctx, 0))
+        return {SLOTS[i]: {"ms_per_step": ms[i] / n, "launches_per_step": int(cnt[i]) // n} for i in range(5)}
+
+    modes = {"fp64": q.MO_FP64, "fp32": q.MO_FP32, "bf16x3": q.MO_BF16X3}
+    log("timing FP64 path")
+    t64 = timed(q.MO_FP64)
+    mo64.copy_(mo)
+    v64 = val.clone()
+    gl64 = gl.clone()
+    log(f"FP64: {t64 * 1e3:.3f} ms/step")
+    k64 = kernel_times(q.MO_FP64)
+    log(f"timing {args.mode} path")
+    tsp = timed(modes[args.mode])
+    log(f"{args.mode}: {tsp * 1e3:.3f} ms/step")
+    ksp = kernel_times(modes[args.mode])
+    torch.cuda.synchronize()
+    q.synchronize(ctx)
+    # MO error of the split product per VGL component, relative to max |component|
+    a = mo.view(P, 5, Nmo)
+    b = mo64.view(P, 5, Nmo)
+    mo_err = [float(((a[:, c] - b[:, c]).abs().max() / b[:, c].abs().max()).item()) for c in range(5)]
+    dJ = float((torch.log(val) - torch.log(v64)).abs().max().item())
+    dgl = float((gl - gl64).abs().max().item())
+
+    costs = per_walker_costs(s)
+    # dominant kernel of the split path and its roofline position
+    dom = max(ksp, key=lambda k: ksp[k]["ms_per_step"])
+    t_dom = ksp[dom]["ms_per_step"] / max(1, ksp[dom]["launches_per_step"]) * 1e-3
+    if dom == "jastrow_een":
+        work = costs["een_flops"] * W
+        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": PEAK["fp64_TFs"], "unit": "TFLOP/s"}
+        roof["note"] = "FP64 FMA work of the e-e-n contraction priced against the 78.6 TFLOP/s FP64 peak"
+    elif dom == "mo_gemm":
+        work = costs["mo_flops"] * W / max(1, ksp[dom]["launches_per_step"])
+        peak = {"bf16x3": PEAK["bf16_TFs"] / 6, "fp32": PEAK["fp32_TFs"], "fp64": PEAK["fp64_TFs"]}[args.mode]
+        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": peak, "unit": "TFLOP/s"}
+    else:
+        work = costs["ao_bytes"] * W / max(1, ksp[dom]["launches_per_step"])
+        roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm_GBs"], "unit": "GB/s"}
+    roof["kernel"] = dom
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(s, x, W, args.cpu_seconds)
+
+    out = {
+        "metric": "walker-evals/sec (AO+MO+Jastrow VGL), split-prec vs FP64, at stated MO error",
+        "value": W / tsp,
+        "unit": "walker-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tsp * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"f64 (AO, Jastrow) + {args.mode} (MO product)",
+        "data": "synthetic (seeded)",
+        "config": {
+            "workload": f"{args.workload}: {s.nucl_num} nuclei, {Ne} electrons, {s.ao_num} AOs, "
+                        f"{Nmo} occupied MOs, {W} walkers, Jastrow 5/5/5, precision {s.precision}",
+            "walkers": W, "elec_num": Ne, "ao_num": s.ao_num, "mo_num": Nmo, "nucl_num": s.nucl_num,
+            "mo_precision": args.mode,
+        },
+        "fp64_path": {"value": W / t64, "ms_per_step": t64 * 1e3, "kernels": k64},
+        "split_path": {"kernels": ksp, "speedup_vs_fp64": t64 / tsp},
+        "mo_max_rel_err_vs_fp64": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
+        "jastrow_max_abs_dlogJ_vs_fp64": dJ,
+        "jastrow_max_abs_dgl_vs_fp64": dgl,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    q.context_destroy(ctx)
+    print(json.dumps(out))
+
+
+def cpu_baseline(s, x, W, seconds):
+    """The oracle (FP64 C restatement, OpenMP over walkers/points) on a bounded sample of
+    walkers of the same workload: AO VGL + MO VGL + Jastrow value and GL."""
+    import oracle
+
+    Ne = s.elec_num
+    coef = oracle.kept_mo_coef(s)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def run(nw):
+        xs = np.ascontiguousarray(x.reshape(3, W, Ne)[:, :nw, :].reshape(3, nw * Ne))
+        t0 = time.perf_counter()
+        ao = oracle.ao_vgl(s, xs)
+        oracle.mo_vgl(coef, ao)
+        oracle.jastrow(s, xs, nw, want=("value", "gl"))
+        return time.perf_counter() - t0
+
+    nw = max(1, threads)
+    t = run(nw)
+    log(f"cpu oracle: {nw} walkers in {t:.2f} s")
+    while t < seconds / 3 and nw * 2 <= W:
+        nw *= 2
+        t = run(nw)
+        log(f"cpu oracle: {nw} walkers in {t:.2f} s")
+    return {"value": nw / t, "unit": "walker-evals/s", "cores": threads, "kind": "port",
+            "sample": f"{nw} walkers of the same workload (AO VGL + MO VGL + Jastrow value/GL), "
+                      f"{t:.1f} s on {threads} OpenMP threads"}
+
+
+if __name__ == "__main__":
+    main()

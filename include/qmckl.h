@@ -1,0 +1,368 @@
+/*
+ * qmckl.h -- C-ABI of the MI355X-native per-walker wavefunction evaluator.
+ *
+ * This header declares the subset of the TREX-CoE/qmckl public API that the walker
+ * evaluation path needs (context, errors, numerical precision, points, nuclei,
+ * electrons, contracted-Gaussian AO basis, MO basis, CHAMP Jastrow), with the
+ * reference's names, argument order, argument meaning, return codes and host-pointer
+ * FP64 semantics, so that C callers, the reference's Fortran `bind(C)` module and its
+ * SWIG Python wrapper can link against libqmckl_amd.so instead of libqmckl.so.
+ * Each group cites the reference file:line whose declaration it replaces.
+ *
+ * Entry points whose name starts with `qmckl_amd_` have no counterpart in the reference:
+ * they expose device residency (inputs/outputs in GPU memory), the HIP stream and the
+ * precision of the AO->MO product.  They are listed at the end.
+ *
+ * Layouts are C row-major:
+ *   point coordinates    'N': [num][3]           'T': [3][num]
+ *   AO VGL               [point][5][ao_num]      (value, d/dx, d/dy, d/dz, Laplacian)
+ *   MO VGL               [point][5][mo_num]
+ *   Jastrow value        [walk_num]
+ *   Jastrow GL           [walk_num][4][elec_num] (d/dx, d/dy, d/dz, Laplacian of e^J)
+ * with point = walker * elec_num + electron, up-spin electrons first.
+ */
+#ifndef QMCKL_AMD_H
+#define QMCKL_AMD_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Types and constants: org/qmckl_context.org:104-105, org/qmckl_error.org:67,152-181,406-407 */
+typedef int64_t qmckl_context;
+typedef int32_t qmckl_exit_code;
+
+#define QMCKL_NULL_CONTEXT ((qmckl_context)0)
+
+#define QMCKL_SUCCESS ((qmckl_exit_code)0)
+#define QMCKL_INVALID_ARG_1 ((qmckl_exit_code)1)
+#define QMCKL_INVALID_ARG_2 ((qmckl_exit_code)2)
+#define QMCKL_INVALID_ARG_3 ((qmckl_exit_code)3)
+#define QMCKL_INVALID_ARG_4 ((qmckl_exit_code)4)
+#define QMCKL_INVALID_ARG_5 ((qmckl_exit_code)5)
+#define QMCKL_INVALID_ARG_6 ((qmckl_exit_code)6)
+#define QMCKL_INVALID_ARG_7 ((qmckl_exit_code)7)
+#define QMCKL_INVALID_ARG_8 ((qmckl_exit_code)8)
+#define QMCKL_INVALID_ARG_9 ((qmckl_exit_code)9)
+#define QMCKL_INVALID_ARG_10 ((qmckl_exit_code)10)
+#define QMCKL_INVALID_ARG_11 ((qmckl_exit_code)11)
+#define QMCKL_INVALID_ARG_12 ((qmckl_exit_code)12)
+#define QMCKL_INVALID_ARG_13 ((qmckl_exit_code)13)
+#define QMCKL_INVALID_ARG_14 ((qmckl_exit_code)14)
+#define QMCKL_INVALID_ARG_15 ((qmckl_exit_code)15)
+#define QMCKL_INVALID_ARG_16 ((qmckl_exit_code)16)
+#define QMCKL_INVALID_ARG_17 ((qmckl_exit_code)17)
+#define QMCKL_INVALID_ARG_18 ((qmckl_exit_code)18)
+#define QMCKL_INVALID_ARG_19 ((qmckl_exit_code)19)
+#define QMCKL_INVALID_ARG_20 ((qmckl_exit_code)20)
+#define QMCKL_FAILURE ((qmckl_exit_code)101)
+#define QMCKL_ERRNO ((qmckl_exit_code)102)
+#define QMCKL_INVALID_CONTEXT ((qmckl_exit_code)103)
+#define QMCKL_ALLOCATION_FAILED ((qmckl_exit_code)104)
+#define QMCKL_DEALLOCATION_FAILED ((qmckl_exit_code)105)
+#define QMCKL_NOT_PROVIDED ((qmckl_exit_code)106)
+#define QMCKL_OUT_OF_BOUNDS ((qmckl_exit_code)107)
+#define QMCKL_ALREADY_SET ((qmckl_exit_code)108)
+#define QMCKL_INVALID_EXIT_CODE ((qmckl_exit_code)109)
+
+#define QMCKL_MAX_FUN_LEN 256
+#define QMCKL_MAX_MSG_LEN 1024
+
+/* ---- Context: org/qmckl_context.org:221, 253, 302, 428, 467, 609, 648, 769 */
+qmckl_context qmckl_context_check(const qmckl_context context);
+qmckl_exit_code qmckl_context_touch(const qmckl_context context);
+qmckl_context qmckl_context_create(void);
+void qmckl_lock(qmckl_context context);
+void qmckl_unlock(qmckl_context context);
+qmckl_context qmckl_context_copy(const qmckl_context context);
+qmckl_exit_code qmckl_context_get_num_threads(const qmckl_context context, int32_t* const num_threads);
+qmckl_exit_code qmckl_context_set_num_threads(const qmckl_context context, const int32_t num_threads);
+qmckl_exit_code qmckl_context_destroy(const qmckl_context context);
+
+/* ---- Errors: org/qmckl_error.org:241, 433, 493, 550, 606, 660 */
+const char* qmckl_string_of_error(const qmckl_exit_code error);
+qmckl_exit_code qmckl_set_error(qmckl_context context, const qmckl_exit_code exit_code,
+                                const char* function_name, const char* message);
+qmckl_exit_code qmckl_get_error(qmckl_context context, qmckl_exit_code* exit_code,
+                                char* function_name, char* message);
+qmckl_exit_code qmckl_failwith(qmckl_context context, const qmckl_exit_code exit_code,
+                               const char* function, const char* message);
+qmckl_exit_code qmckl_last_error(qmckl_context context, char* buffer);
+qmckl_exit_code qmckl_check(qmckl_context context, qmckl_exit_code rc);
+
+/* ---- Numerical precision: org/qmckl_numprec.org:169, 224, 260, 314, 351, 412 */
+qmckl_exit_code qmckl_set_numprec_precision(const qmckl_context context, const int precision);
+int32_t qmckl_get_numprec_precision(const qmckl_context context);
+qmckl_exit_code qmckl_set_numprec_range(const qmckl_context context, const int range);
+int32_t qmckl_get_numprec_range(const qmckl_context context);
+double qmckl_get_numprec_epsilon(const qmckl_context context);
+int32_t qmckl_test_precision_64(double a, double b);
+int32_t qmckl_test_precision_32(float a, float b);
+
+/* ---- Points: org/qmckl_point.org:143, 188, 276 */
+qmckl_exit_code qmckl_get_point_num(const qmckl_context context, int64_t* const num);
+qmckl_exit_code qmckl_get_point(const qmckl_context context, const char transp, double* const coord,
+                                const int64_t size_max);
+qmckl_exit_code qmckl_set_point(qmckl_context context, const char transp, const int64_t num,
+                                const double* coord, const int64_t size_max);
+
+/* ---- Nuclei: org/qmckl_nucleus.org:161, 216, 285, 367, 417, 461, 529, 780, 954 */
+qmckl_exit_code qmckl_get_nucleus_num(const qmckl_context context, int64_t* const num);
+qmckl_exit_code qmckl_get_nucleus_charge(const qmckl_context context, double* const charge,
+                                         const int64_t size_max);
+qmckl_exit_code qmckl_get_nucleus_coord(const qmckl_context context, const char transp,
+                                        double* const coord, const int64_t size_max);
+bool qmckl_nucleus_provided(const qmckl_context context);
+qmckl_exit_code qmckl_set_nucleus_num(qmckl_context context, const int64_t num);
+qmckl_exit_code qmckl_set_nucleus_charge(qmckl_context context, const double* charge,
+                                         const int64_t size_max);
+qmckl_exit_code qmckl_set_nucleus_coord(qmckl_context context, const char transp,
+                                        const double* coord, const int64_t size_max);
+qmckl_exit_code qmckl_get_nucleus_nn_distance(qmckl_context context, double* distance,
+                                              const int64_t size_max);
+qmckl_exit_code qmckl_get_nucleus_repulsion(qmckl_context context, double* const energy);
+
+/* ---- Electrons: org/qmckl_electron.org:197, 222, 406, 504, 543, 776, 1035, 1203, 1489 */
+bool qmckl_electron_provided(const qmckl_context context);
+qmckl_exit_code qmckl_set_electron_num(qmckl_context context, const int64_t up_num,
+                                       const int64_t down_num);
+qmckl_exit_code qmckl_set_electron_coord(qmckl_context context, const char transp,
+                                         const int64_t walk_num, const double* coord,
+                                         const int64_t size_max);
+qmckl_exit_code qmckl_get_electron_num(const qmckl_context context, int64_t* const num);
+qmckl_exit_code qmckl_get_electron_up_num(const qmckl_context context, int64_t* const up_num);
+qmckl_exit_code qmckl_get_electron_down_num(const qmckl_context context, int64_t* const down_num);
+qmckl_exit_code qmckl_get_electron_walk_num(const qmckl_context context, int64_t* const walk_num);
+qmckl_exit_code qmckl_get_electron_coord(const qmckl_context context, const char transp,
+                                         double* const coord, const int64_t size_max);
+qmckl_exit_code qmckl_get_electron_ee_distance(qmckl_context context, double* const distance,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_get_electron_ee_potential(qmckl_context context, double* const ee_potential);
+qmckl_exit_code qmckl_get_electron_en_distance(qmckl_context context, double* const distance,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_get_electron_en_potential(qmckl_context context, double* const en_potential);
+
+/* ---- AO basis setters: org/qmckl_ao.org:422-1264 */
+qmckl_exit_code qmckl_set_ao_basis_type(qmckl_context context, const char basis_type);
+qmckl_exit_code qmckl_set_ao_basis_shell_num(qmckl_context context, const int64_t shell_num);
+qmckl_exit_code qmckl_set_ao_basis_prim_num(qmckl_context context, const int64_t prim_num);
+qmckl_exit_code qmckl_set_ao_basis_nucleus_shell_num(qmckl_context context, const int64_t* nucleus_shell_num,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_nucleus_index(qmckl_context context, const int64_t* nucleus_index,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_shell_ang_mom(qmckl_context context, const int32_t* shell_ang_mom,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_shell_prim_num(qmckl_context context, const int64_t* shell_prim_num,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_shell_prim_index(qmckl_context context, const int64_t* shell_prim_index,
+                                                    const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_shell_factor(qmckl_context context, const double* shell_factor,
+                                                const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_exponent(qmckl_context context, const double* exponent,
+                                            const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_coefficient(qmckl_context context, const double* coefficient,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_prim_factor(qmckl_context context, const double* prim_factor,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_ao_num(qmckl_context context, const int64_t ao_num);
+qmckl_exit_code qmckl_set_ao_basis_ao_factor(qmckl_context context, const double* ao_factor,
+                                             const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_r_power(qmckl_context context, const int32_t* r_power,
+                                           const int64_t size_max);
+qmckl_exit_code qmckl_set_ao_basis_cartesian(qmckl_context context, const bool cartesian);
+
+/* ---- AO basis getters: org/qmckl_ao.org:1483-2248 */
+qmckl_exit_code qmckl_get_ao_basis_type(const qmckl_context context, char* const basis_type);
+qmckl_exit_code qmckl_get_ao_basis_shell_num(const qmckl_context context, int64_t* const shell_num);
+qmckl_exit_code qmckl_get_ao_basis_prim_num(const qmckl_context context, int64_t* const prim_num);
+qmckl_exit_code qmckl_get_ao_basis_nucleus_shell_num(const qmckl_context context, int64_t* const nucleus_shell_num,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_nucleus_index(const qmckl_context context, int64_t* const nucleus_index,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_r_power(const qmckl_context context, int32_t* const r_power,
+                                           const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_shell_ang_mom(const qmckl_context context, int32_t* const shell_ang_mom,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_shell_prim_num(const qmckl_context context, int64_t* const shell_prim_num,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_shell_prim_index(const qmckl_context context, int64_t* const shell_prim_index,
+                                                    const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_shell_factor(const qmckl_context context, double* const shell_factor,
+                                                const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_exponent(const qmckl_context context, double* const exponent,
+                                            const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_coefficient(const qmckl_context context, double* const coefficient,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_prim_factor(const qmckl_context context, double* const prim_factor,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_ao_num(const qmckl_context context, int64_t* const ao_num);
+qmckl_exit_code qmckl_get_ao_basis_ao_factor(const qmckl_context context, double* const ao_factor,
+                                             const int64_t size_max);
+bool qmckl_ao_basis_provided(const qmckl_context context);
+
+/* ---- AO values: org/qmckl_ao.org:3417, 3486, 3547, 3609, 3675, 3736 */
+qmckl_exit_code qmckl_get_ao_basis_primitive_vgl(qmckl_context context, double* const primitive_vgl,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_shell_vgl(qmckl_context context, double* const shell_vgl,
+                                             const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_ao_vgl(qmckl_context context, double* const ao_vgl, const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_ao_vgl_inplace(qmckl_context context, double* const ao_vgl,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_ao_value(qmckl_context context, double* const ao_value,
+                                            const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_ao_value_inplace(qmckl_context context, double* const ao_value,
+                                                    const int64_t size_max);
+
+/* ---- MO basis: org/qmckl_mo.org:192, 705, 742, 799, 873, 992, 1048, 1628, 1684, 3479 */
+qmckl_exit_code qmckl_set_mo_basis_mo_num(qmckl_context context, const int64_t mo_num);
+qmckl_exit_code qmckl_set_mo_basis_coefficient(qmckl_context context, const double* coefficient,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_get_mo_basis_mo_num(const qmckl_context context, int64_t* mo_num);
+qmckl_exit_code qmckl_get_mo_basis_coefficient(const qmckl_context context, double* const coefficient,
+                                               const int64_t size_max);
+bool qmckl_mo_basis_provided(const qmckl_context context);
+qmckl_exit_code qmckl_mo_basis_select_mo(const qmckl_context context, const int32_t* keep,
+                                         const int64_t size_max);
+qmckl_exit_code qmckl_get_mo_basis_mo_value(qmckl_context context, double* const mo_value,
+                                            const int64_t size_max);
+qmckl_exit_code qmckl_get_mo_basis_mo_value_inplace(qmckl_context context, double* const mo_value,
+                                                    const int64_t size_max);
+qmckl_exit_code qmckl_get_mo_basis_mo_vgl(qmckl_context context, double* const mo_vgl, const int64_t size_max);
+qmckl_exit_code qmckl_get_mo_basis_mo_vgl_inplace(qmckl_context context, double* const mo_vgl,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_mo_basis_rescale(qmckl_context context, const double scaling_factor);
+
+/* ---- CHAMP Jastrow parameters: org/qmckl_jastrow_champ.org:531, 1439, 1462 */
+qmckl_exit_code qmckl_set_jastrow_champ_rescale_factor_ee(qmckl_context context, const double kappa_ee);
+qmckl_exit_code qmckl_set_jastrow_champ_rescale_factor_en(qmckl_context context, const double* kappa_en,
+                                                          const int64_t size_max);
+qmckl_exit_code qmckl_set_jastrow_champ_aord_num(qmckl_context context, const int64_t aord_num);
+qmckl_exit_code qmckl_set_jastrow_champ_bord_num(qmckl_context context, const int64_t bord_num);
+qmckl_exit_code qmckl_set_jastrow_champ_cord_num(qmckl_context context, const int64_t cord_num);
+qmckl_exit_code qmckl_set_jastrow_champ_type_nucl_num(qmckl_context context, const int64_t type_nucl_num);
+qmckl_exit_code qmckl_set_jastrow_champ_type_nucl_vector(qmckl_context context, const int64_t* type_nucl_vector,
+                                                         const int64_t size_max);
+qmckl_exit_code qmckl_set_jastrow_champ_a_vector(qmckl_context context, const double* a_vector,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_set_jastrow_champ_b_vector(qmckl_context context, const double* b_vector,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_set_jastrow_champ_c_vector(qmckl_context context, const double* c_vector,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_set_jastrow_champ_spin_independent(qmckl_context context, const int32_t spin_independent);
+qmckl_exit_code qmckl_get_jastrow_champ_aord_num(qmckl_context context, int64_t* const aord_num);
+qmckl_exit_code qmckl_get_jastrow_champ_bord_num(qmckl_context context, int64_t* const bord_num);
+qmckl_exit_code qmckl_get_jastrow_champ_cord_num(qmckl_context context, int64_t* const bord_num);
+qmckl_exit_code qmckl_get_jastrow_champ_type_nucl_num(qmckl_context context, int64_t* const type_nucl_num);
+qmckl_exit_code qmckl_get_jastrow_champ_type_nucl_vector(qmckl_context context, int64_t* const type_nucl_num,
+                                                         const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_a_vector(qmckl_context context, double* const a_vector,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_b_vector(qmckl_context context, double* const b_vector,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_c_vector(qmckl_context context, double* const c_vector,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_rescale_factor_ee(const qmckl_context context,
+                                                          double* const rescale_factor_ee);
+qmckl_exit_code qmckl_get_jastrow_champ_rescale_factor_en(const qmckl_context context,
+                                                          double* const rescale_factor_en, const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_dim_c_vector(qmckl_context context, int64_t* const dim_c_vector);
+qmckl_exit_code qmckl_get_jastrow_champ_spin_independent(qmckl_context context, int32_t* const spin_independent);
+bool qmckl_jastrow_champ_provided(const qmckl_context context);
+
+/* ---- CHAMP Jastrow intermediates and results:
+ *      org/qmckl_jastrow_champ.org:2143, 2553, 2944, 3405, 4013, 5677, 5931, 6356, 6839, 7323,
+ *      8938, 9438, 10556, 10935, 13836, 14394, 16409, 16767, 16823 */
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasb(qmckl_context context, double* const asymp_jasb,
+                                                   const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasa(qmckl_context context, double* const asymp_jasa,
+                                                   const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_ee_distance_rescaled(qmckl_context context, double* const distance_rescaled,
+                                                             int64_t const size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_ee_distance_rescaled_gl(qmckl_context context,
+                                                                double* const distance_rescaled_gl,
+                                                                const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_en_distance_rescaled(qmckl_context context, double* const distance_rescaled,
+                                                             const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_en_distance_rescaled_gl(qmckl_context context,
+                                                                double* const distance_rescaled_gl,
+                                                                const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_e(qmckl_context context, double* const een_rescaled_e,
+                                                       const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_e_gl(qmckl_context context, double* const een_rescaled_e_gl,
+                                                          const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_n(qmckl_context context, double* const een_rescaled_n,
+                                                       const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_n_gl(qmckl_context context, double* const een_rescaled_n_gl,
+                                                          const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_tmp_c(qmckl_context context, double* const tmp_c, const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_dtmp_c(qmckl_context context, double* const dtmp_c, const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee(qmckl_context context, double* const factor_ee,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee_gl(qmckl_context context, double* const factor_ee_gl,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en(qmckl_context context, double* const factor_en,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en_gl(qmckl_context context, double* const factor_en_gl,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een(qmckl_context context, double* const factor_een,
+                                                   const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_gl(qmckl_context context, double* const factor_een_gl,
+                                                      const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_value(qmckl_context context, double* const value, const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_gl(qmckl_context context, double* const gl, const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_grad(qmckl_context context, double* const grad, const int64_t size_max);
+
+/* ======================================================================================
+ * Extensions without a counterpart in the reference (device residency, streams, MO GEMM
+ * precision).  Device pointers are HIP device addresses on the context's device.
+ * ====================================================================================== */
+
+/* Precision of the AO->MO product (default QMCKL_AMD_MO_FP64, the reference's arithmetic).
+ * FP32: the AO VGL and the MO coefficients are rounded to FP32 and multiplied on FP32
+ * matrix cores with FP32 accumulation.  BF16X3: both operands are split into three
+ * bfloat16 terms and the six leading cross products are accumulated in FP32 on bf16
+ * matrix cores.  AO values, distances and the whole Jastrow stay FP64 in every mode. */
+#define QMCKL_AMD_MO_FP64 ((int32_t)0)
+#define QMCKL_AMD_MO_FP32 ((int32_t)1)
+#define QMCKL_AMD_MO_BF16X3 ((int32_t)2)
+qmckl_exit_code qmckl_amd_set_mo_precision(qmckl_context context, const int32_t mode);
+qmckl_exit_code qmckl_amd_get_mo_precision(qmckl_context context, int32_t* const mode);
+
+/* HIP stream (a hipStream_t, or NULL for the context's own stream) used by every kernel
+ * of this context. */
+qmckl_exit_code qmckl_amd_set_stream(qmckl_context context, void* stream);
+/* Waits for all work of the context. */
+qmckl_exit_code qmckl_amd_synchronize(qmckl_context context);
+
+/* qmckl_set_electron_coord with coordinates already in device memory. */
+qmckl_exit_code qmckl_amd_set_electron_coord_device(qmckl_context context, const char transp,
+                                                    const int64_t walk_num, const double* d_coord,
+                                                    const int64_t size_max);
+/* Results written straight into caller-owned device buffers (same layouts and size_max
+ * rules as the host getters; nothing is copied to the host). */
+qmckl_exit_code qmckl_amd_get_ao_basis_ao_vgl_device(qmckl_context context, double* const d_ao_vgl,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_amd_get_mo_basis_mo_vgl_device(qmckl_context context, double* const d_mo_vgl,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_amd_get_jastrow_champ_value_device(qmckl_context context, double* const d_value,
+                                                         const int64_t size_max);
+qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, double* const d_gl,
+                                                      const int64_t size_max);
+
+/* Kernel timing: when profiling is on, every kernel launch of the context is bracketed by
+ * HIP events on the context's stream.  qmckl_amd_get_kernel_times waits for the stream,
+ * returns the accumulated milliseconds and launch counts per slot and resets them.
+ * Slots: 0 AO VGL, 1 AO->MO GEMM, 2 Jastrow e-e/e-n, 3 Jastrow e-e-n, 4 Jastrow totals. */
+#define QMCKL_AMD_KERNEL_SLOTS 5
+qmckl_exit_code qmckl_amd_set_profiling(qmckl_context context, const int32_t on);
+qmckl_exit_code qmckl_amd_get_kernel_times(qmckl_context context, double* const ms, int64_t* const counts,
+                                           const int64_t size_max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QMCKL_AMD_H */

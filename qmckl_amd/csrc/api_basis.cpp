@@ -1,0 +1,837 @@
+// Contracted-Gaussian AO basis and MO basis: setters, getters, basis finalisation and
+// the providers of AO VGL and MO VGL.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "internal.hpp"
+
+using namespace qmckl_amd;
+
+namespace {
+
+// ---- AO setter scaffolding, org/qmckl_ao.org:386-420 (mask bits and ALREADY_SET rule)
+qmckl_exit_code ao_pre(Context* c, qmckl_context ctx, uint32_t mask) {
+  if (mask != 0 && !(c->ao_uninit & mask)) return fail(ctx, QMCKL_ALREADY_SET, "qmckl_set_ao_*", nullptr);
+  return QMCKL_SUCCESS;
+}
+
+// Builds the device-oriented tables once the 14 mandatory items are set;
+// org/qmckl_ao.org:2965-3183 (qmckl_finalize_basis).
+qmckl_exit_code finalize_basis(Context* c, qmckl_context ctx) {
+  AoTables& t = c->tables;
+  t = AoTables();
+  const int64_t Nn = c->nucl_num;
+  if (Nn <= 0) return fail(ctx, QMCKL_NOT_PROVIDED, "qmckl_finalize_basis", "nucleus data is not provided");
+  if (c->ao_type != 'G')
+    return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "only Gaussian ('G') shells are implemented");
+  if (c->r_power.empty()) c->r_power.assign(c->shell_num, 0);
+  t.nucl_num = Nn;
+  t.shell_num = c->shell_num;
+  t.prim_num = c->prim_num;
+  t.ao_num = c->ao_num;
+  t.nucl_range.assign(Nn, 0.0);
+  t.nucl_max_l.assign(Nn, 0);
+  int64_t ao = 0;
+  for (int64_t a = 0; a < Nn; ++a) {
+    const int64_t s0 = c->nucleus_index[a], s1 = s0 + c->nucleus_shell_num[a];
+    if (s0 < 0 || s1 > c->shell_num)
+      return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "nucleus_index/nucleus_shell_num out of range");
+    for (int64_t s = s0; s < s1; ++s) {
+      const int l = c->shell_ang_mom[s];
+      if (l < 0 || l > 10) return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "angular momentum outside [0,10]");
+      const int64_t p0 = c->shell_prim_index[s], np = c->shell_prim_num[s];
+      if (p0 < 0 || np < 0 || p0 + np > c->prim_num)
+        return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "shell primitives out of range");
+      t.shell_nucl.push_back((int32_t)a);
+      t.shell_l.push_back(l);
+      t.shell_prim_start.push_back((int32_t)t.prim_expo.size());
+      t.shell_prim_count.push_back((int32_t)np);
+      t.shell_ao_start.push_back((int32_t)ao);
+      for (int64_t p = p0; p < p0 + np; ++p) {
+        t.prim_expo.push_back(c->exponent[p]);
+        t.prim_coef.push_back(c->coefficient[p] * c->prim_factor[p] * c->shell_factor[s]);
+        const double range = (double)(c->r_power[s] + 1) / c->exponent[p];
+        t.nucl_range[a] = std::max(t.nucl_range[a], range);
+      }
+      t.nucl_max_l[a] = std::max(t.nucl_max_l[a], l);
+      t.lmax = std::max(t.lmax, (int32_t)l);
+      // Cartesian polynomials in the order of org/qmckl_ao.org:6270-6300:
+      // a = l..0, b = l-a..0, c = l-a-b.
+      for (int pa = l; pa >= 0; --pa)
+        for (int pb = l - pa; pb >= 0; --pb) {
+          const int pc = l - pa - pb;
+          t.ao_shell.push_back((int32_t)(t.shell_l.size() - 1));
+          t.ao_pow.push_back(pa | (pb << 8) | (pc << 16));
+          ++ao;
+        }
+    }
+  }
+  if (ao != c->ao_num)
+    return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "ao_num does not match the shells' angular momenta");
+  if ((int64_t)t.shell_l.size() != c->shell_num)
+    return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "the nuclei's shell ranges do not cover shell_num shells");
+  for (int64_t a = 1; a < Nn; ++a)
+    if (c->nucleus_index[a] != c->nucleus_index[a - 1] + c->nucleus_shell_num[a - 1])
+      return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "shells must be stored nucleus by nucleus");
+  t.shell_num = (int64_t)t.shell_l.size();
+  t.ao_factor = c->ao_factor;
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code ao_post(Context* c, qmckl_context ctx, uint32_t mask) {
+  c->ao_uninit &= ~mask;
+  c->ao_provided = (c->ao_uninit == 0);
+  touch_const(c);
+  if (c->ao_provided) return finalize_basis(c, ctx);
+  return QMCKL_SUCCESS;
+}
+
+template <class T>
+qmckl_exit_code set_array(Context* c, qmckl_context ctx, uint32_t mask, const T* src, int64_t n,
+                          int64_t size_max, std::vector<T>& dst, const char* fn, const char* what) {
+  qmckl_exit_code rc = ao_pre(c, ctx, mask);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (n <= 0) return fail(ctx, QMCKL_FAILURE, fn, what);
+  if (src == nullptr) return fail(ctx, QMCKL_INVALID_ARG_2, fn, "null pointer");
+  if (size_max < n) return fail(ctx, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  dst.assign(src, src + n);
+  return ao_post(c, ctx, mask);
+}
+
+template <class T>
+qmckl_exit_code get_array(Context* c, qmckl_context ctx, uint32_t mask, T* dst, int64_t n, int64_t size_max,
+                          const std::vector<T>& src, const char* fn) {
+  if (dst == nullptr) return fail(ctx, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (c->ao_uninit & mask) return fail(ctx, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (size_max < n) return fail(ctx, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  std::copy(src.begin(), src.begin() + n, dst);
+  return QMCKL_SUCCESS;
+}
+
+}  // namespace
+
+namespace qmckl_amd {
+
+// Uploads the AO tables and nuclear coordinates (once per context date change).
+static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* fn) {
+  qmckl_exit_code rc = device_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (c->dc.date >= c->const_date) return QMCKL_SUCCESS;
+  const AoTables& t = c->tables;
+  DevConst& d = c->dc;
+  hipError_t e = hipSuccess;
+  auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+    if (e != hipSuccess) return;
+    e = b.reserve(bytes);
+    if (e == hipSuccess && bytes) e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  up(d.shell_nucl, t.shell_nucl.data(), 4 * t.shell_nucl.size());
+  up(d.shell_l, t.shell_l.data(), 4 * t.shell_l.size());
+  up(d.shell_prim_start, t.shell_prim_start.data(), 4 * t.shell_prim_start.size());
+  up(d.shell_prim_count, t.shell_prim_count.data(), 4 * t.shell_prim_count.size());
+  up(d.shell_ao_start, t.shell_ao_start.data(), 4 * t.shell_ao_start.size());
+  up(d.prim_expo, t.prim_expo.data(), 8 * t.prim_expo.size());
+  up(d.prim_coef, t.prim_coef.data(), 8 * t.prim_coef.size());
+  up(d.ao_shell, t.ao_shell.data(), 4 * t.ao_shell.size());
+  up(d.ao_pow, t.ao_pow.data(), 4 * t.ao_pow.size());
+  up(d.ao_factor, t.ao_factor.data(), 8 * t.ao_factor.size());
+  up(d.nucl_range, t.nucl_range.data(), 8 * t.nucl_range.size());
+  up(d.nucl_coord, c->nucl_coord.data(), 8 * c->nucl_coord.size());
+  {
+    // Nucleus of each primitive in the caller's order, org/qmckl_ao.org:3000-3015
+    // (nucleus_prim_index from the first shell of each nucleus).
+    std::vector<int32_t> pn(c->prim_num, 0);
+    for (int64_t a = 0; a < c->nucl_num; ++a) {
+      const int64_t p0 = c->shell_prim_index[c->nucleus_index[a]];
+      const int64_t p1 = (a + 1 < c->nucl_num) ? c->shell_prim_index[c->nucleus_index[a + 1]] : c->prim_num;
+      for (int64_t p = p0; p < p1 && p < c->prim_num; ++p) pn[p] = (int32_t)a;
+    }
+    up(d.prim_nucl_c, pn.data(), 4 * pn.size());
+    up(d.prim_expo_c, c->exponent.data(), 8 * c->exponent.size());
+  }
+  if (e == hipSuccess && c->mo_provided) {
+    // coefficient_t [ao][mo_pad] (org/qmckl_mo.org:316-375), padded to a multiple of 16
+    // columns with zeros so the GEMM tiles need no column guard on B.
+    const int64_t Nao = c->ao_num, Nmo = c->mo_num;
+    const int64_t pad = ((Nmo + 15) / 16) * 16;
+    std::vector<double> ct((size_t)Nao * pad, 0.0);
+    for (int64_t k = 0; k < Nao; ++k)
+      for (int64_t j = 0; j < Nmo; ++j) ct[k * pad + j] = c->mo_coef[j * Nao + k];
+    d.mo_pad = pad;
+    up(d.coef_t, ct.data(), 8 * ct.size());
+    if (e == hipSuccess) e = d.coef_t32.reserve(4 * ct.size());
+    if (e == hipSuccess) e = launch_to_f32(d.coef_t.as<double>(), (int64_t)ct.size(), d.coef_t32.as<float>(), c->stream);
+    if (e == hipSuccess) e = d.coef_t_bf.reserve(2 * 3 * ct.size());
+    if (e == hipSuccess)
+      e = launch_split_bf16x3(d.coef_t.as<double>(), (int64_t)ct.size(), d.coef_t_bf.as<uint16_t>(),
+                              (int64_t)ct.size(), c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  d.date = c->const_date;
+  return QMCKL_SUCCESS;
+}
+
+static AoDevView ao_view(Context* c) {
+  const DevConst& d = c->dc;
+  AoDevView v;
+  v.nucl_num = c->nucl_num;
+  v.shell_num = c->tables.shell_num;
+  v.ao_num = c->ao_num;
+  v.shell_nucl = d.shell_nucl.as<int32_t>();
+  v.shell_l = d.shell_l.as<int32_t>();
+  v.shell_prim_start = d.shell_prim_start.as<int32_t>();
+  v.shell_prim_count = d.shell_prim_count.as<int32_t>();
+  v.shell_ao_start = d.shell_ao_start.as<int32_t>();
+  v.prim_expo = d.prim_expo.as<double>();
+  v.prim_coef = d.prim_coef.as<double>();
+  v.ao_shell = d.ao_shell.as<int32_t>();
+  v.ao_pow = d.ao_pow.as<int32_t>();
+  v.ao_factor = d.ao_factor.as<double>();
+  v.nucl_range = d.nucl_range.as<double>();
+  v.nucl_coord = d.nucl_coord.as<double>();
+  // cutoff = -log(epsilon), epsilon = 2^(1-p): org/qmckl_ao.org:118-120, :7811-7815
+  v.cutoff = -log(ldexp(1.0, 1 - c->precision));
+  return v;
+}
+
+static qmckl_exit_code ao_ready(Context* c, qmckl_context handle, const char* fn) {
+  if (!c->ao_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (c->point_num <= 0) return fail(handle, QMCKL_NOT_PROVIDED, fn, "no points set");
+  qmckl_exit_code rc = upload_ao(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  return upload_coords(c, handle, fn);
+}
+
+// AO VGL of all points into the context buffer; org/qmckl_ao.org:9022-9226.
+qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn) {
+  if (c->ao_vgl_date >= c->date) return QMCKL_SUCCESS;
+  qmckl_exit_code rc = ao_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t P = c->point_num, Nao = c->ao_num;
+  hipError_t e = c->d_ao_vgl.reserve(sizeof(double) * P * 5 * Nao);
+  if (e == hipSuccess) {
+    ProfScope ps(c, SLOT_AO);
+    e = launch_ao_vgl(ao_view(c), c->d_coord.as<double>(), P, 0, P, c->d_ao_vgl.ptr, 0, Nao, 0, 0, c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  c->ao_vgl_date = c->date;
+  return QMCKL_SUCCESS;
+}
+
+// MO VGL into d_out (or into the context buffer when d_out is null); org/qmckl_mo.org:1749-1878.
+// The AO VGL is produced chunk by chunk in the precision of the product and consumed by
+// the GEMM, unless the FP64 AO VGL of these points is already resident.
+qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, const char* fn) {
+  if (!c->mo_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (d_out == nullptr && c->mo_vgl_date >= c->date) return QMCKL_SUCCESS;
+  qmckl_exit_code rc = ao_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t P = c->point_num, Nao = c->ao_num, Nmo = c->mo_num;
+  hipError_t e = hipSuccess;
+  double* out = d_out;
+  if (out == nullptr) {
+    e = c->d_mo_vgl.reserve(sizeof(double) * P * 5 * Nmo);
+    if (e != hipSuccess) return hip_fail(handle, e, fn);
+    out = c->d_mo_vgl.as<double>();
+  }
+  const int mode = c->mo_precision;
+  const int64_t pad = c->dc.mo_pad;
+  if (mode == QMCKL_AMD_MO_FP64 && c->ao_vgl_date >= c->date) {
+    ProfScope ps(c, SLOT_MO);
+    e = launch_gemm_f64(c->d_ao_vgl.as<double>(), Nao, c->dc.coef_t.as<double>(), pad, out, Nmo, 5 * P, Nmo,
+                        Nao, c->stream);
+  } else {
+    // Chunk of points whose AO VGL stays within a 1 GiB working buffer.
+    const size_t elem = mode == QMCKL_AMD_MO_FP64 ? 8 : (mode == QMCKL_AMD_MO_FP32 ? 4 : 6);
+    const size_t per_point = elem * 5 * (size_t)Nao;
+    int64_t chunk = (int64_t)((size_t(1) << 30) / per_point);
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, P));
+    e = c->d_ao_work.reserve(per_point * chunk);
+    for (int64_t p0 = 0; e == hipSuccess && p0 < P; p0 += chunk) {
+      const int64_t np = std::min(chunk, P - p0);
+      const int64_t M = 5 * np;
+      double* o = out + p0 * 5 * Nmo;
+      const int kind = mode == QMCKL_AMD_MO_FP64 ? 0 : (mode == QMCKL_AMD_MO_FP32 ? 1 : 2);
+      const int64_t plane = M * Nao;
+      {
+        ProfScope ps(c, SLOT_AO);
+        e = launch_ao_vgl(ao_view(c), c->d_coord.as<double>(), P, p0, np, c->d_ao_work.ptr, kind, Nao,
+                          kind == 2 ? plane : 0, 0, c->stream);
+      }
+      if (e != hipSuccess) break;
+      ProfScope ps(c, SLOT_MO);
+      if (mode == QMCKL_AMD_MO_FP64)
+        e = launch_gemm_f64(c->d_ao_work.as<double>(), Nao, c->dc.coef_t.as<double>(), pad, o, Nmo, M, Nmo, Nao,
+                            c->stream);
+      else if (mode == QMCKL_AMD_MO_FP32)
+        e = launch_gemm_f32(c->d_ao_work.as<float>(), Nao, c->dc.coef_t32.as<float>(), pad, o, Nmo, M, Nmo, Nao,
+                            c->stream);
+      else
+        e = launch_gemm_bf16x3(c->d_ao_work.as<uint16_t>(), Nao, plane, c->dc.coef_t_bf.as<uint16_t>(), pad,
+                               Nao * pad, o, Nmo, M, Nmo, Nao, c->stream);
+    }
+  }
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  if (d_out == nullptr) c->mo_vgl_date = c->date;
+  return QMCKL_SUCCESS;
+}
+
+}  // namespace qmckl_amd
+
+namespace {
+
+qmckl_exit_code copy_out(Context* c, qmckl_context context, const void* d_src, void* dst, size_t bytes,
+                         const char* fn) {
+  hipError_t e = hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* ------------------------------------------------------------------ AO setters */
+// org/qmckl_ao.org:422-450
+qmckl_exit_code qmckl_set_ao_basis_type(qmckl_context context, const char basis_type) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = ao_pre(c, context, 1u);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (basis_type != 'G' && basis_type != 'S')
+    return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_type", nullptr);
+  c->ao_type = basis_type;
+  return ao_post(c, context, 1u);
+}
+
+// org/qmckl_ao.org:451-488
+qmckl_exit_code qmckl_set_ao_basis_shell_num(qmckl_context context, const int64_t shell_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = ao_pre(c, context, 1u << 1);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (shell_num <= 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_shell_num", "shell_num <= 0");
+  if (0 < c->prim_num && c->prim_num < shell_num)
+    return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_shell_num", "shell_num > prim_num");
+  c->shell_num = shell_num;
+  return ao_post(c, context, 1u << 1);
+}
+
+// org/qmckl_ao.org:489-532
+qmckl_exit_code qmckl_set_ao_basis_prim_num(qmckl_context context, const int64_t prim_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = ao_pre(c, context, 1u << 2);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (prim_num <= 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_prim_num", "prim_num must be positive");
+  if (c->shell_num <= 0) return fail(context, QMCKL_FAILURE, "qmckl_set_ao_basis_prim_num", "shell_num is not set");
+  if (prim_num < c->shell_num) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_prim_num", "prim_num < shell_num");
+  c->prim_num = prim_num;
+  return ao_post(c, context, 1u << 2);
+}
+
+// org/qmckl_ao.org:534-595
+qmckl_exit_code qmckl_set_ao_basis_nucleus_shell_num(qmckl_context context, const int64_t* nucleus_shell_num,
+                                                     const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 3, nucleus_shell_num, c->nucl_num, size_max, c->nucleus_shell_num,
+                   "qmckl_set_ao_basis_nucleus_shell_num", "nucl_num is not set");
+}
+
+// org/qmckl_ao.org:596-655
+qmckl_exit_code qmckl_set_ao_basis_nucleus_index(qmckl_context context, const int64_t* nucleus_index,
+                                                 const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 4, nucleus_index, c->nucl_num, size_max, c->nucleus_index,
+                   "qmckl_set_ao_basis_nucleus_index", "nucl_num is not set");
+}
+
+// org/qmckl_ao.org:657-718
+qmckl_exit_code qmckl_set_ao_basis_shell_ang_mom(qmckl_context context, const int32_t* shell_ang_mom,
+                                                 const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 5, shell_ang_mom, c->shell_num, size_max, c->shell_ang_mom,
+                   "qmckl_set_ao_basis_shell_ang_mom", "shell_num is not set");
+}
+
+// org/qmckl_ao.org:720-781
+qmckl_exit_code qmckl_set_ao_basis_shell_prim_num(qmckl_context context, const int64_t* shell_prim_num,
+                                                  const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 6, shell_prim_num, c->shell_num, size_max, c->shell_prim_num,
+                   "qmckl_set_ao_basis_shell_prim_num", "shell_num is not set");
+}
+
+// org/qmckl_ao.org:783-843
+qmckl_exit_code qmckl_set_ao_basis_shell_prim_index(qmckl_context context, const int64_t* shell_prim_index,
+                                                    const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 7, shell_prim_index, c->shell_num, size_max, c->shell_prim_index,
+                   "qmckl_set_ao_basis_shell_prim_index", "shell_num is not set");
+}
+
+// org/qmckl_ao.org:845-905
+qmckl_exit_code qmckl_set_ao_basis_shell_factor(qmckl_context context, const double* shell_factor,
+                                                const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 8, shell_factor, c->shell_num, size_max, c->shell_factor,
+                   "qmckl_set_ao_basis_shell_factor", "shell_num is not set");
+}
+
+// org/qmckl_ao.org:907-967
+qmckl_exit_code qmckl_set_ao_basis_exponent(qmckl_context context, const double* exponent, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 9, exponent, c->prim_num, size_max, c->exponent, "qmckl_set_ao_basis_exponent",
+                   "prim_num is not set");
+}
+
+// org/qmckl_ao.org:969-1029
+qmckl_exit_code qmckl_set_ao_basis_coefficient(qmckl_context context, const double* coefficient,
+                                               const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 10, coefficient, c->prim_num, size_max, c->coefficient,
+                   "qmckl_set_ao_basis_coefficient", "prim_num is not set");
+}
+
+// org/qmckl_ao.org:1031-1091
+qmckl_exit_code qmckl_set_ao_basis_prim_factor(qmckl_context context, const double* prim_factor,
+                                               const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 11, prim_factor, c->prim_num, size_max, c->prim_factor,
+                   "qmckl_set_ao_basis_prim_factor", "prim_num is not set");
+}
+
+// org/qmckl_ao.org:1093-1136
+qmckl_exit_code qmckl_set_ao_basis_ao_num(qmckl_context context, const int64_t ao_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = ao_pre(c, context, 1u << 12);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (ao_num <= 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_ao_num", "ao_num must be positive");
+  if (c->shell_num <= 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_ao_num", "shell_num is not set");
+  if (ao_num < c->shell_num) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_ao_num", "ao_num < shell_num");
+  c->ao_num = ao_num;
+  return ao_post(c, context, 1u << 12);
+}
+
+// org/qmckl_ao.org:1138-1197
+qmckl_exit_code qmckl_set_ao_basis_ao_factor(qmckl_context context, const double* ao_factor, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  return set_array(c, context, 1u << 13, ao_factor, c->ao_num, size_max, c->ao_factor, "qmckl_set_ao_basis_ao_factor",
+                   "ao_num is not set");
+}
+
+// org/qmckl_ao.org:1199-1262 (optional; Gaussian shells require zeros)
+qmckl_exit_code qmckl_set_ao_basis_r_power(qmckl_context context, const int32_t* r_power, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (c->shell_num == 0) return fail(context, QMCKL_FAILURE, "qmckl_set_ao_basis_r_power", "shell_num is not set");
+  if (r_power == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_ao_basis_r_power", "null pointer");
+  if (size_max < c->shell_num) return fail(context, QMCKL_INVALID_ARG_3, "qmckl_set_ao_basis_r_power", "input array too small");
+  c->r_power.assign(r_power, r_power + c->shell_num);
+  return ao_post(c, context, 0u);
+}
+
+// org/qmckl_ao.org:1264-1283 (stored; only Cartesian polynomials are evaluated, as in the reference)
+qmckl_exit_code qmckl_set_ao_basis_cartesian(qmckl_context context, const bool cartesian) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  c->ao_cartesian = cartesian;
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ AO getters, org/qmckl_ao.org:1483-2260 */
+qmckl_exit_code qmckl_get_ao_basis_type(const qmckl_context context, char* const basis_type) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (basis_type == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_ao_basis_type", "NULL pointer");
+  if (c->ao_uninit & 1u) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_ao_basis_type", nullptr);
+  *basis_type = c->ao_type;
+  return QMCKL_SUCCESS;
+}
+qmckl_exit_code qmckl_get_ao_basis_shell_num(const qmckl_context context, int64_t* const shell_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (shell_num == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_ao_basis_shell_num", "NULL pointer");
+  if (c->ao_uninit & (1u << 1)) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_ao_basis_shell_num", nullptr);
+  *shell_num = c->shell_num;
+  return QMCKL_SUCCESS;
+}
+qmckl_exit_code qmckl_get_ao_basis_prim_num(const qmckl_context context, int64_t* const prim_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (prim_num == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_ao_basis_prim_num", "NULL pointer");
+  if (c->ao_uninit & (1u << 2)) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_ao_basis_prim_num", nullptr);
+  *prim_num = c->prim_num;
+  return QMCKL_SUCCESS;
+}
+qmckl_exit_code qmckl_get_ao_basis_nucleus_shell_num(const qmckl_context context, int64_t* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 3, v, c->nucl_num, size_max, c->nucleus_shell_num, "qmckl_get_ao_basis_nucleus_shell_num");
+}
+qmckl_exit_code qmckl_get_ao_basis_nucleus_index(const qmckl_context context, int64_t* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 4, v, c->nucl_num, size_max, c->nucleus_index, "qmckl_get_ao_basis_nucleus_index");
+}
+qmckl_exit_code qmckl_get_ao_basis_r_power(const qmckl_context context, int32_t* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (v == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_ao_basis_r_power", "NULL pointer");
+  if (c->r_power.empty()) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_ao_basis_r_power", nullptr);
+  if (size_max < c->shell_num) return fail(context, QMCKL_INVALID_ARG_3, "qmckl_get_ao_basis_r_power", "input array too small");
+  std::copy(c->r_power.begin(), c->r_power.end(), v);
+  return QMCKL_SUCCESS;
+}
+qmckl_exit_code qmckl_get_ao_basis_shell_ang_mom(const qmckl_context context, int32_t* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 5, v, c->shell_num, size_max, c->shell_ang_mom, "qmckl_get_ao_basis_shell_ang_mom");
+}
+qmckl_exit_code qmckl_get_ao_basis_shell_prim_num(const qmckl_context context, int64_t* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 6, v, c->shell_num, size_max, c->shell_prim_num, "qmckl_get_ao_basis_shell_prim_num");
+}
+qmckl_exit_code qmckl_get_ao_basis_shell_prim_index(const qmckl_context context, int64_t* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 7, v, c->shell_num, size_max, c->shell_prim_index, "qmckl_get_ao_basis_shell_prim_index");
+}
+qmckl_exit_code qmckl_get_ao_basis_shell_factor(const qmckl_context context, double* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 8, v, c->shell_num, size_max, c->shell_factor, "qmckl_get_ao_basis_shell_factor");
+}
+qmckl_exit_code qmckl_get_ao_basis_exponent(const qmckl_context context, double* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 9, v, c->prim_num, size_max, c->exponent, "qmckl_get_ao_basis_exponent");
+}
+qmckl_exit_code qmckl_get_ao_basis_coefficient(const qmckl_context context, double* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 10, v, c->prim_num, size_max, c->coefficient, "qmckl_get_ao_basis_coefficient");
+}
+qmckl_exit_code qmckl_get_ao_basis_prim_factor(const qmckl_context context, double* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 11, v, c->prim_num, size_max, c->prim_factor, "qmckl_get_ao_basis_prim_factor");
+}
+qmckl_exit_code qmckl_get_ao_basis_ao_num(const qmckl_context context, int64_t* const ao_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (ao_num == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_ao_basis_ao_num", "NULL pointer");
+  if (c->ao_uninit & (1u << 12)) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_ao_basis_ao_num", nullptr);
+  *ao_num = c->ao_num;
+  return QMCKL_SUCCESS;
+}
+qmckl_exit_code qmckl_get_ao_basis_ao_factor(const qmckl_context context, double* const v, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return get_array(c, context, 1u << 13, v, c->ao_num, size_max, c->ao_factor, "qmckl_get_ao_basis_ao_factor");
+}
+bool qmckl_ao_basis_provided(const qmckl_context context) {
+  Context* c = ctx_of(context);
+  return c ? c->ao_provided : false;
+}
+
+/* ------------------------------------------------------------------ AO values */
+// org/qmckl_ao.org:3417-3480: primitive_vgl[point][5][prim_num]
+qmckl_exit_code qmckl_get_ao_basis_primitive_vgl(qmckl_context context, double* const out, const int64_t size_max) {
+  const char* fn = "qmckl_get_ao_basis_primitive_vgl";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = ao_ready(c, context, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t n = c->point_num * 5 * c->prim_num;
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  hipError_t e = c->d_jint.reserve(sizeof(double) * n);
+  if (e == hipSuccess)
+    e = launch_prim_vgl(ao_view(c), c->prim_num, c->dc.prim_nucl_c.as<int32_t>(), c->dc.prim_expo_c.as<double>(),
+                        c->d_coord.as<double>(), c->point_num, c->d_jint.as<double>(), c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  c->jint_date = 0;
+  return copy_out(c, context, c->d_jint.ptr, out, sizeof(double) * n, fn);
+}
+
+// org/qmckl_ao.org:3486-3545: shell_vgl[point][5][shell_num]
+qmckl_exit_code qmckl_get_ao_basis_shell_vgl(qmckl_context context, double* const out, const int64_t size_max) {
+  const char* fn = "qmckl_get_ao_basis_shell_vgl";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = ao_ready(c, context, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t n = c->point_num * 5 * c->shell_num;
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  hipError_t e = c->d_jint.reserve(sizeof(double) * n);
+  if (e == hipSuccess) e = launch_shell_vgl(ao_view(c), c->d_coord.as<double>(), c->point_num, c->d_jint.as<double>(), c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  c->jint_date = 0;
+  return copy_out(c, context, c->d_jint.ptr, out, sizeof(double) * n, fn);
+}
+
+// org/qmckl_ao.org:3547-3600
+qmckl_exit_code qmckl_get_ao_basis_ao_vgl(qmckl_context context, double* const ao_vgl, const int64_t size_max) {
+  const char* fn = "qmckl_get_ao_basis_ao_vgl";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->ao_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (ao_vgl == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * 5 * c->ao_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  qmckl_exit_code rc = provide_ao_vgl(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  return copy_out(c, context, c->d_ao_vgl.ptr, ao_vgl, sizeof(double) * n, fn);
+}
+
+// org/qmckl_ao.org:3609-3670: same result, the caller's array is the output
+qmckl_exit_code qmckl_get_ao_basis_ao_vgl_inplace(qmckl_context context, double* const ao_vgl, const int64_t size_max) {
+  return qmckl_get_ao_basis_ao_vgl(context, ao_vgl, size_max);
+}
+
+// org/qmckl_ao.org:3675-3735: ao_value[point][ao_num]
+qmckl_exit_code qmckl_get_ao_basis_ao_value(qmckl_context context, double* const ao_value, const int64_t size_max) {
+  const char* fn = "qmckl_get_ao_basis_ao_value";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->ao_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (ao_value == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * c->ao_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  qmckl_exit_code rc = ao_ready(c, context, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  hipError_t e = c->d_jint.reserve(sizeof(double) * n);
+  if (e == hipSuccess)
+    e = launch_ao_vgl(ao_view(c), c->d_coord.as<double>(), c->point_num, 0, c->point_num, c->d_jint.ptr, 0,
+                      c->ao_num, 0, 1, c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  c->jint_date = 0;
+  return copy_out(c, context, c->d_jint.ptr, ao_value, sizeof(double) * n, fn);
+}
+
+qmckl_exit_code qmckl_get_ao_basis_ao_value_inplace(qmckl_context context, double* const ao_value, const int64_t size_max) {
+  return qmckl_get_ao_basis_ao_value(context, ao_value, size_max);
+}
+
+qmckl_exit_code qmckl_amd_get_ao_basis_ao_vgl_device(qmckl_context context, double* const d_ao_vgl, const int64_t size_max) {
+  const char* fn = "qmckl_amd_get_ao_basis_ao_vgl_device";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->ao_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (d_ao_vgl == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * 5 * c->ao_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  qmckl_exit_code rc = ao_ready(c, context, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  hipError_t e = launch_ao_vgl(ao_view(c), c->d_coord.as<double>(), c->point_num, 0, c->point_num, d_ao_vgl, 0,
+                               c->ao_num, 0, 0, c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ MO basis */
+// org/qmckl_mo.org:187-300
+qmckl_exit_code qmckl_set_mo_basis_mo_num(qmckl_context context, const int64_t mo_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!(c->mo_uninit & 1u)) return fail(context, QMCKL_ALREADY_SET, "qmckl_set_mo_basis_mo_num", nullptr);
+  if (mo_num <= 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_mo_basis_mo_num", "mo_num <= 0");
+  c->mo_num = mo_num;
+  c->mo_uninit &= ~1u;
+  c->mo_provided = (c->mo_uninit == 0);
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_mo.org:230-300 (may be called again to replace the coefficients)
+qmckl_exit_code qmckl_set_mo_basis_coefficient(qmckl_context context, const double* coefficient, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (c->mo_num <= 0) return fail(context, QMCKL_FAILURE, "qmckl_set_mo_basis_coefficient", "mo_num is not set");
+  if (c->ao_num <= 0) return fail(context, QMCKL_FAILURE, "qmckl_set_mo_basis_coefficient", "ao_num is not set");
+  if (coefficient == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_mo_basis_coefficient", "NULL pointer");
+  const int64_t n = c->mo_num * c->ao_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, "qmckl_set_mo_basis_coefficient", "input array too small");
+  c->mo_coef.assign(coefficient, coefficient + n);
+  c->mo_uninit &= ~2u;
+  c->mo_provided = (c->mo_uninit == 0);
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_mo.org:705-740
+qmckl_exit_code qmckl_get_mo_basis_mo_num(const qmckl_context context, int64_t* mo_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (mo_num == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_mo_basis_mo_num", "NULL pointer");
+  if (c->mo_uninit & 1u) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_mo_basis_mo_num", nullptr);
+  *mo_num = c->mo_num;
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_mo.org:742-797
+qmckl_exit_code qmckl_get_mo_basis_coefficient(const qmckl_context context, double* const coefficient, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (coefficient == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_mo_basis_coefficient", "NULL pointer");
+  if (c->mo_uninit & 2u) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_mo_basis_coefficient", nullptr);
+  const int64_t n = c->mo_num * c->ao_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, "qmckl_get_mo_basis_coefficient", "input array too small");
+  std::copy(c->mo_coef.begin(), c->mo_coef.begin() + n, coefficient);
+  return QMCKL_SUCCESS;
+}
+
+bool qmckl_mo_basis_provided(const qmckl_context context) {
+  Context* c = ctx_of(context);
+  return c ? c->mo_provided : false;
+}
+
+// org/qmckl_mo.org:882-985: keep the MOs with keep[i] != 0
+qmckl_exit_code qmckl_mo_basis_select_mo(const qmckl_context context, const int32_t* keep, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_mo_basis_select_mo", nullptr);
+  if (keep == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_mo_basis_select_mo", "NULL pointer");
+  if (size_max < c->mo_num)
+    return fail(context, QMCKL_INVALID_ARG_3, "qmckl_get_mo_basis_select_mo", "Array too small: expected mo_num.");
+  const int64_t Nao = c->ao_num;
+  std::vector<double> coef;
+  int64_t k = 0;
+  for (int64_t i = 0; i < c->mo_num; ++i)
+    if (keep[i] != 0) {
+      coef.insert(coef.end(), c->mo_coef.begin() + i * Nao, c->mo_coef.begin() + (i + 1) * Nao);
+      ++k;
+    }
+  c->mo_coef.swap(coef);
+  c->mo_num = k;
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_mo.org:3472-3530: multiplies all MO coefficients by scaling_factor
+qmckl_exit_code qmckl_mo_basis_rescale(qmckl_context context, const double scaling_factor) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_mo_basis_rescale", nullptr);
+  for (double& x : c->mo_coef) x *= scaling_factor;
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_mo.org:1626-1686
+qmckl_exit_code qmckl_get_mo_basis_mo_vgl(qmckl_context context, double* const mo_vgl, const int64_t size_max) {
+  const char* fn = "qmckl_get_mo_basis_mo_vgl";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (mo_vgl == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * 5 * c->mo_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  qmckl_exit_code rc = provide_mo_vgl(context, c, nullptr, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  return copy_out(c, context, c->d_mo_vgl.ptr, mo_vgl, sizeof(double) * n, fn);
+}
+
+qmckl_exit_code qmckl_get_mo_basis_mo_vgl_inplace(qmckl_context context, double* const mo_vgl, const int64_t size_max) {
+  return qmckl_get_mo_basis_mo_vgl(context, mo_vgl, size_max);
+}
+
+// org/qmckl_mo.org:988-1100: mo_value[point][mo_num] (the value row of the MO VGL)
+qmckl_exit_code qmckl_get_mo_basis_mo_value(qmckl_context context, double* const mo_value, const int64_t size_max) {
+  const char* fn = "qmckl_get_mo_basis_mo_value";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (mo_value == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * c->mo_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  qmckl_exit_code rc = provide_mo_vgl(context, c, nullptr, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  hipError_t e = hipMemcpy2DAsync(mo_value, sizeof(double) * c->mo_num, c->d_mo_vgl.ptr, sizeof(double) * 5 * c->mo_num,
+                                  sizeof(double) * c->mo_num, c->point_num, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_get_mo_basis_mo_value_inplace(qmckl_context context, double* const mo_value, const int64_t size_max) {
+  return qmckl_get_mo_basis_mo_value(context, mo_value, size_max);
+}
+
+qmckl_exit_code qmckl_amd_set_mo_precision(qmckl_context context, const int32_t mode) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (mode != QMCKL_AMD_MO_FP64 && mode != QMCKL_AMD_MO_FP32 && mode != QMCKL_AMD_MO_BF16X3)
+    return fail(context, QMCKL_INVALID_ARG_2, "qmckl_amd_set_mo_precision", "unknown mode");
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  c->mo_precision = mode;
+  c->mo_vgl_date = 0;
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_amd_get_mo_precision(qmckl_context context, int32_t* const mode) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (mode == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_amd_get_mo_precision", "NULL pointer");
+  *mode = c->mo_precision;
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_amd_get_mo_basis_mo_vgl_device(qmckl_context context, double* const d_mo_vgl, const int64_t size_max) {
+  const char* fn = "qmckl_amd_get_mo_basis_mo_vgl_device";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (d_mo_vgl == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * 5 * c->mo_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  return provide_mo_vgl(context, c, d_mo_vgl, fn);
+}
+
+}  // extern "C"

@@ -1,0 +1,612 @@
+// CHAMP Jastrow factor: parameter setters/getters, finalisation (asymptotic shifts),
+// the provider of value and gradient/Laplacian, and the getters of its intermediates.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "internal.hpp"
+
+using namespace qmckl_amd;
+
+namespace {
+
+// org/qmckl_jastrow_champ.org:11365-11405 (qmckl_compute_dim_c_vector)
+int64_t dim_c_of(int64_t cord) {
+  int64_t n = 0;
+  for (int64_t p = 2; p <= cord; ++p)
+    for (int64_t k = p - 1; k >= 0; --k) {
+      const int64_t lmax = (k != 0) ? p - k : p - k - 2;
+      for (int64_t l = lmax; l >= 0; --l)
+        if (((p - k - l) & 1) == 0) ++n;
+    }
+  return n;
+}
+
+// pre2 of org/qmckl_jastrow_champ.org:545-560
+qmckl_exit_code jas_pre(Context* c, qmckl_context ctx, uint32_t mask) {
+  if (mask != 0 && !(c->jas_uninit & mask)) return fail(ctx, QMCKL_ALREADY_SET, "qmckl_set_jastrow_champ_*", nullptr);
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_jastrow_champ.org:1036-1085 (qmckl_finalize_jastrow_champ): the asymptotic
+// shifts asymp_jasa (:5830-5875) and asymp_jasb (:2271-2320).
+qmckl_exit_code finalize_jastrow(Context* c, qmckl_context ctx) {
+  if (!c->elec_provided) return fail(ctx, QMCKL_NOT_PROVIDED, "qmckl_electron", nullptr);
+  if (!c->nucl_provided) return fail(ctx, QMCKL_NOT_PROVIDED, "qmckl_nucleus", nullptr);
+  const double kinv = 1.0 / c->rescale_ee;
+  const double* b = c->b_vector.data();
+  const double asym_one = b[0] * kinv / (1.0 + b[1] * kinv);
+  c->asymp_jasb[0] = (c->spin_independent == 1) ? asym_one : 0.5 * asym_one;
+  c->asymp_jasb[1] = asym_one;
+  double x = kinv;
+  for (int64_t p = 2; p <= c->bord; ++p) {
+    x *= kinv;
+    c->asymp_jasb[0] += b[p] * x;
+    c->asymp_jasb[1] += b[p] * x;
+  }
+  c->asymp_jasa.assign(c->type_nucl_num, 0.0);
+  for (int64_t t = 0; t < c->type_nucl_num; ++t) {
+    const double* a = c->a_vector.data() + t * (c->aord + 1);
+    const double ki = 1.0 / c->rescale_en[t];
+    double v = a[0] * ki / (1.0 + a[1] * ki);
+    double y = ki;
+    for (int64_t p = 2; p <= c->aord; ++p) {
+      y *= ki;
+      v += a[p] * y;
+    }
+    c->asymp_jasa[t] = v;
+  }
+  touch_const(c);
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code jas_post(Context* c, qmckl_context ctx, uint32_t mask) {
+  c->jas_uninit &= ~mask;
+  c->jas_provided = (c->jas_uninit == 0);
+  touch_const(c);
+  if (c->jas_provided) return finalize_jastrow(c, ctx);
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code upload_jastrow(Context* c, qmckl_context handle, const char* fn) {
+  qmckl_exit_code rc = device_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (c->dc.jas_date >= c->const_date) return QMCKL_SUCCESS;
+  DevConst& d = c->dc;
+  hipError_t e = hipSuccess;
+  auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+    if (e != hipSuccess) return;
+    e = b.reserve(bytes);
+    if (e == hipSuccess && bytes) e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  // c_vector_full [dim_c][nucl], org/qmckl_jastrow_champ.org:11865-11895
+  const int64_t Nn = c->nucl_num, dim_c = c->dim_c;
+  std::vector<double> cfull((size_t)std::max<int64_t>(dim_c, 1) * Nn, 0.0);
+  for (int64_t n = 0; n < dim_c; ++n)
+    for (int64_t a = 0; a < Nn; ++a) cfull[n * Nn + a] = c->c_vector[c->type_nucl_vector[a] * dim_c + n];
+  up(d.jas_a, c->a_vector.data(), 8 * c->a_vector.size());
+  up(d.jas_b, c->b_vector.data(), 8 * c->b_vector.size());
+  up(d.jas_cfull, cfull.data(), 8 * cfull.size());
+  up(d.jas_type, c->type_nucl_vector.data(), 8 * c->type_nucl_vector.size());
+  up(d.jas_kappa_en, c->rescale_en.data(), 8 * c->rescale_en.size());
+  up(d.jas_asymp_a, c->asymp_jasa.data(), 8 * c->asymp_jasa.size());
+  up(d.nucl_coord, c->nucl_coord.data(), 8 * c->nucl_coord.size());
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  d.jas_date = c->const_date;
+  return QMCKL_SUCCESS;
+}
+
+JasDevView jas_view(Context* c) {
+  JasDevView v;
+  v.walk_num = c->walk_num;
+  v.elec_num = c->elec_num;
+  v.up_num = c->up_num;
+  v.nucl_num = c->nucl_num;
+  v.aord = c->aord;
+  v.bord = c->bord;
+  v.cord = c->cord;
+  v.dim_c = c->dim_c;
+  v.spin_independent = c->spin_independent;
+  v.rescale_ee = c->rescale_ee;
+  v.coord = c->d_coord.as<double>();
+  v.coord_ld = c->point_num;
+  v.nucl_coord = c->dc.nucl_coord.as<double>();
+  v.type_nucl = c->dc.jas_type.as<int64_t>();
+  v.kappa_en = c->dc.jas_kappa_en.as<double>();
+  v.a_vector = c->dc.jas_a.as<double>();
+  v.b_vector = c->dc.jas_b.as<double>();
+  v.cfull = c->dc.jas_cfull.as<double>();
+  v.asymp_jasb0 = c->asymp_jasb[0];
+  v.asymp_jasb1 = c->asymp_jasb[1];
+  v.asymp_jasa = c->dc.jas_asymp_a.as<double>();
+  return v;
+}
+
+qmckl_exit_code jas_ready(Context* c, qmckl_context handle, const char* fn) {
+  if (!c->jas_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (!c->elec_provided || c->walk_num <= 0) return fail(handle, QMCKL_NOT_PROVIDED, fn, "electron coordinates not set");
+  qmckl_exit_code rc = upload_jastrow(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  return upload_coords(c, handle, fn);
+}
+
+qmckl_exit_code copy_out(Context* c, qmckl_context context, const void* d_src, double* dst, int64_t n, const char* fn) {
+  hipError_t e = hipMemcpyAsync(dst, d_src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+enum Which { F_EE, F_EN, F_EEN, F_EE_GL, F_EN_GL, F_EEN_GL, J_VALUE, J_GL };
+
+qmckl_exit_code get_result(qmckl_context context, Which which, double* out, int64_t size_max, const char* fn) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  const int64_t W = c->walk_num, Ne = c->elec_num;
+  const bool is_gl = which == F_EE_GL || which == F_EN_GL || which == F_EEN_GL || which == J_GL;
+  const int64_t n = is_gl ? W * 4 * Ne : W;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const DevBuf* src[] = {&c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl, &c->d_feen_gl, &c->d_jvalue, &c->d_jgl};
+  return copy_out(c, context, src[which]->ptr, out, n, fn);
+}
+
+qmckl_exit_code get_inter(qmckl_context context, int kind, double* out, int64_t size_max, const char* fn) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  qmckl_exit_code rc = jas_ready(c, context, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const JasDevView v = jas_view(c);
+  const int64_t n = jastrow_inter_size(v, kind);
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  if (!(c->jint_date >= c->date && c->jint_kind == kind)) {
+    hipError_t e = c->d_jint.reserve(sizeof(double) * std::max<int64_t>(n, 1));
+    if (e == hipSuccess) e = launch_jastrow_inter(v, kind, c->d_jint.as<double>(), c->stream);
+    if (e != hipSuccess) return hip_fail(context, e, fn);
+    c->jint_date = c->date;
+    c->jint_kind = kind;
+  }
+  return copy_out(c, context, c->d_jint.ptr, out, n, fn);
+}
+
+}  // namespace
+
+namespace qmckl_amd {
+
+// Value, factors and their gradients/Laplacians of all walkers in one pass over the
+// device; org/qmckl_jastrow_champ.org:16481-16560 (value) and :16887-16960 (gl).
+qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn) {
+  if (c->jastrow_date >= c->date) return QMCKL_SUCCESS;
+  qmckl_exit_code rc = jas_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t W = c->walk_num, Ne = c->elec_num;
+  const JasDevView v = jas_view(c);
+  hipError_t e = hipSuccess;
+  DevBuf* w1[] = {&c->d_fee, &c->d_fen, &c->d_feen, &c->d_jvalue};
+  DevBuf* w4[] = {&c->d_fee_gl, &c->d_fen_gl, &c->d_feen_gl, &c->d_jgl};
+  for (DevBuf* b : w1)
+    if (e == hipSuccess) e = b->reserve(sizeof(double) * W);
+  for (DevBuf* b : w4)
+    if (e == hipSuccess) e = b->reserve(sizeof(double) * W * 4 * Ne);
+  if (e == hipSuccess) e = c->d_jscratch.reserve(sizeof(double) * std::max<int64_t>(jastrow_scratch_doubles(v), 1));
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  JasOut o;
+  o.fee = c->d_fee.as<double>();
+  o.fen = c->d_fen.as<double>();
+  o.feen = c->d_feen.as<double>();
+  o.fee_gl = c->d_fee_gl.as<double>();
+  o.fen_gl = c->d_fen_gl.as<double>();
+  o.feen_gl = c->d_feen_gl.as<double>();
+  o.value = c->d_jvalue.as<double>();
+  o.gl = c->d_jgl.as<double>();
+  o.scratch = c->d_jscratch.as<double>();
+  {
+    ProfScope ps(c, SLOT_JPAIR);
+    e = launch_jastrow_pair(v, o, c->stream);
+  }
+  if (e == hipSuccess) {
+    ProfScope ps(c, SLOT_JEEN);
+    e = launch_jastrow_een(v, o, c->stream);
+  }
+  if (e == hipSuccess) {
+    ProfScope ps(c, SLOT_JTOTAL);
+    e = launch_jastrow_total(v, o, c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  c->jastrow_date = c->date;
+  return QMCKL_SUCCESS;
+}
+
+}  // namespace qmckl_amd
+
+extern "C" {
+
+/* ---- setters, org/qmckl_jastrow_champ.org:566-1000 */
+qmckl_exit_code qmckl_set_jastrow_champ_aord_num(qmckl_context context, const int64_t aord_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 0);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (aord_num < 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_jastrow_champ_aord_num", "aord_num < 0");
+  c->aord = aord_num;
+  c->jas_uninit |= (1u << 5);
+  return jas_post(c, context, 1u << 0);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_bord_num(qmckl_context context, const int64_t bord_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 1);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (bord_num < 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_jastrow_champ_bord_num", "bord_num < 0");
+  c->bord = bord_num;
+  c->jas_uninit |= (1u << 6);
+  return jas_post(c, context, 1u << 1);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_cord_num(qmckl_context context, const int64_t cord_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 2);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (cord_num < 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_jastrow_champ_cord_num", "cord_num < 0");
+  if (cord_num > 8)
+    return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_jastrow_champ_cord_num", "cord_num > 8 is not supported");
+  c->cord = cord_num;
+  c->dim_c = dim_c_of(cord_num);
+  if (cord_num > 0)
+    c->jas_uninit |= (1u << 7);
+  else
+    c->jas_uninit &= ~(1u << 7);
+  return jas_post(c, context, 1u << 2);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_type_nucl_num(qmckl_context context, const int64_t type_nucl_num) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 3);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (type_nucl_num <= 0)
+    return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_jastrow_champ_type_nucl_num", "type_nucl_num < 0");
+  c->type_nucl_num = type_nucl_num;
+  return jas_post(c, context, 1u << 3);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_type_nucl_vector(qmckl_context context, const int64_t* type_nucl_vector,
+                                                         const int64_t size_max) {
+  const char* fn = "qmckl_set_jastrow_champ_type_nucl_vector";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 4);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t Nn = c->nucl_num;
+  if (c->type_nucl_num <= 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "type_nucl_num not initialized");
+  if (type_nucl_vector == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "type_nucl_vector = NULL");
+  if (Nn <= 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "nucl_num not initialized");
+  if (size_max < Nn) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  for (int64_t i = 0; i < Nn; ++i) {
+    if (type_nucl_vector[i] < 0) return fail(context, QMCKL_INVALID_ARG_2, fn, "Inconsistent values of type_nucl_vector (<0)");
+    if (type_nucl_vector[i] >= c->type_nucl_num)
+      return fail(context, QMCKL_INVALID_ARG_2, fn, "Inconsistent values of type_nucl_vector (>=type_nucl_num)");
+  }
+  c->type_nucl_vector.assign(type_nucl_vector, type_nucl_vector + Nn);
+  return jas_post(c, context, 1u << 4);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_a_vector(qmckl_context context, const double* a_vector, const int64_t size_max) {
+  const char* fn = "qmckl_set_jastrow_champ_a_vector";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 5);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (c->aord < 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "aord_num not set");
+  if (c->type_nucl_num <= 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "type_nucl_num not set");
+  if (a_vector == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "a_vector = NULL");
+  const int64_t n = (c->aord + 1) * c->type_nucl_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected (aord_num+1)*type_nucl_num");
+  c->a_vector.assign(a_vector, a_vector + n);
+  return jas_post(c, context, 1u << 5);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_b_vector(qmckl_context context, const double* b_vector, const int64_t size_max) {
+  const char* fn = "qmckl_set_jastrow_champ_b_vector";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 6);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (c->bord < 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "bord_num not set");
+  if (b_vector == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "b_vector = NULL");
+  if (size_max < c->bord + 1) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected (bord_num+1)");
+  c->b_vector.assign(b_vector, b_vector + c->bord + 1);
+  return jas_post(c, context, 1u << 6);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_c_vector(qmckl_context context, const double* c_vector, const int64_t size_max) {
+  const char* fn = "qmckl_set_jastrow_champ_c_vector";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 7);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (c->type_nucl_num <= 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "type_nucl_num not set");
+  if (c->dim_c < 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "cord_num not set");
+  if (c_vector == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "c_vector = NULL");
+  const int64_t n = c->dim_c * c->type_nucl_num;
+  if (size_max < n) {
+    char msg[256];
+    snprintf(msg, sizeof msg, "Array too small. Expected dim_c_vector*type_nucl_num = %ld", (long)n);
+    return fail(context, QMCKL_INVALID_ARG_3, fn, msg);
+  }
+  c->c_vector.assign(c_vector, c_vector + n);
+  return jas_post(c, context, 1u << 7);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_rescale_factor_ee(qmckl_context context, const double kappa_ee) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 8);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (kappa_ee <= 0.0)
+    return fail(context, QMCKL_INVALID_ARG_2, "qmckl_set_jastrow_champ_rescale_factor_ee", "rescale_factor_ee <= 0.0");
+  c->rescale_ee = kappa_ee;
+  return jas_post(c, context, 1u << 8);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_rescale_factor_en(qmckl_context context, const double* kappa_en,
+                                                          const int64_t size_max) {
+  const char* fn = "qmckl_set_jastrow_champ_rescale_factor_en";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 9);
+  if (rc != QMCKL_SUCCESS) return rc;
+  if (c->type_nucl_num <= 0) return fail(context, QMCKL_NOT_PROVIDED, fn, "type_nucl_num not set");
+  if (kappa_en == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "Null pointer");
+  if (size_max < c->type_nucl_num) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected type_nucl_num.");
+  for (int64_t i = 0; i < c->type_nucl_num; ++i)
+    if (kappa_en[i] <= 0.0) return fail(context, QMCKL_INVALID_ARG_2, fn, "rescale_factor_en <= 0.0");
+  c->rescale_en.assign(kappa_en, kappa_en + c->type_nucl_num);
+  return jas_post(c, context, 1u << 9);
+}
+
+qmckl_exit_code qmckl_set_jastrow_champ_spin_independent(qmckl_context context, const int32_t spin_independent) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  qmckl_exit_code rc = jas_pre(c, context, 1u << 10);
+  if (rc != QMCKL_SUCCESS) return rc;
+  c->spin_independent = spin_independent;
+  return jas_post(c, context, 1u << 10);
+}
+
+/* ---- parameter getters, org/qmckl_jastrow_champ.org:1437-1901 */
+#define JAS_GET_SCALAR(NAME, FIELD, BIT, T)                                                       \
+  qmckl_exit_code NAME(qmckl_context context, T* const out) {                                     \
+    Context* c = ctx_of(context);                                                                 \
+    if (!c) return QMCKL_INVALID_CONTEXT;                                                         \
+    if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, #NAME, "NULL pointer");          \
+    if (c->jas_uninit & (1u << BIT)) return fail(context, QMCKL_NOT_PROVIDED, #NAME, nullptr);     \
+    *out = (T)c->FIELD;                                                                           \
+    return QMCKL_SUCCESS;                                                                         \
+  }
+JAS_GET_SCALAR(qmckl_get_jastrow_champ_aord_num, aord, 0, int64_t)
+JAS_GET_SCALAR(qmckl_get_jastrow_champ_bord_num, bord, 1, int64_t)
+JAS_GET_SCALAR(qmckl_get_jastrow_champ_cord_num, cord, 2, int64_t)
+JAS_GET_SCALAR(qmckl_get_jastrow_champ_type_nucl_num, type_nucl_num, 3, int64_t)
+JAS_GET_SCALAR(qmckl_get_jastrow_champ_spin_independent, spin_independent, 10, int32_t)
+#undef JAS_GET_SCALAR
+
+qmckl_exit_code qmckl_get_jastrow_champ_rescale_factor_ee(const qmckl_context context, double* const out) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_jastrow_champ_rescale_factor_ee", "NULL pointer");
+  if (c->jas_uninit & (1u << 8)) return fail(context, QMCKL_NOT_PROVIDED, "qmckl_get_jastrow_champ_rescale_factor_ee", nullptr);
+  *out = c->rescale_ee;
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_get_jastrow_champ_dim_c_vector(qmckl_context context, int64_t* const out) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_get_jastrow_champ_dim_c_vector", "NULL pointer");
+  *out = c->dim_c;
+  return QMCKL_SUCCESS;
+}
+
+static qmckl_exit_code jas_get_vec(qmckl_context context, uint32_t bit, const std::vector<double>& v, double* out,
+                                   int64_t size_max, const char* fn) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (c->jas_uninit & (1u << bit)) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (size_max < (int64_t)v.size()) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  std::copy(v.begin(), v.end(), out);
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_get_jastrow_champ_a_vector(qmckl_context context, double* const a, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return jas_get_vec(context, 5, c->a_vector, a, size_max, "qmckl_get_jastrow_champ_a_vector");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_b_vector(qmckl_context context, double* const b, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return jas_get_vec(context, 6, c->b_vector, b, size_max, "qmckl_get_jastrow_champ_b_vector");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_c_vector(qmckl_context context, double* const cv, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return jas_get_vec(context, 7, c->c_vector, cv, size_max, "qmckl_get_jastrow_champ_c_vector");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_rescale_factor_en(const qmckl_context context, double* const k, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  return jas_get_vec(context, 9, c->rescale_en, k, size_max, "qmckl_get_jastrow_champ_rescale_factor_en");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_type_nucl_vector(qmckl_context context, int64_t* const t, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  const char* fn = "qmckl_get_jastrow_champ_type_nucl_vector";
+  if (t == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (c->jas_uninit & (1u << 4)) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (size_max < (int64_t)c->type_nucl_vector.size()) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  std::copy(c->type_nucl_vector.begin(), c->type_nucl_vector.end(), t);
+  return QMCKL_SUCCESS;
+}
+
+bool qmckl_jastrow_champ_provided(const qmckl_context context) {
+  Context* c = ctx_of(context);
+  return c ? c->jas_provided : false;
+}
+
+// org/qmckl_jastrow_champ.org:2143-2185
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasb(qmckl_context context, double* const asymp_jasb, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  const char* fn = "qmckl_get_jastrow_champ_asymp_jasb";
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (asymp_jasb == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (size_max < 2) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected 2");
+  asymp_jasb[0] = c->asymp_jasb[0];
+  asymp_jasb[1] = c->asymp_jasb[1];
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_jastrow_champ.org:5677-5720
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasa(qmckl_context context, double* const asymp_jasa, const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  const char* fn = "qmckl_get_jastrow_champ_asymp_jasa";
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (asymp_jasa == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (size_max < c->type_nucl_num) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected type_nucl_num");
+  std::copy(c->asymp_jasa.begin(), c->asymp_jasa.end(), asymp_jasa);
+  return QMCKL_SUCCESS;
+}
+
+/* ---- intermediates (materialised on demand) */
+qmckl_exit_code qmckl_get_jastrow_champ_ee_distance_rescaled(qmckl_context ctx, double* const out, int64_t const n) {
+  return get_inter(ctx, J_EE_RESCALED, out, n, "qmckl_get_jastrow_champ_ee_distance_rescaled");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_ee_distance_rescaled_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EE_RESCALED_GL, out, n, "qmckl_get_jastrow_champ_ee_distance_rescaled_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_en_distance_rescaled(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EN_RESCALED, out, n, "qmckl_get_jastrow_champ_en_distance_rescaled");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_en_distance_rescaled_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EN_RESCALED_GL, out, n, "qmckl_get_jastrow_champ_en_distance_rescaled_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_e(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EEN_E, out, n, "qmckl_get_jastrow_champ_een_rescaled_e");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_e_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EEN_E_GL, out, n, "qmckl_get_jastrow_champ_een_rescaled_e_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_n(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EEN_N, out, n, "qmckl_get_jastrow_champ_een_rescaled_n");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_een_rescaled_n_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_EEN_N_GL, out, n, "qmckl_get_jastrow_champ_een_rescaled_n_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_tmp_c(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_TMP_C, out, n, "qmckl_get_jastrow_champ_tmp_c");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_dtmp_c(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_inter(ctx, J_DTMP_C, out, n, "qmckl_get_jastrow_champ_dtmp_c");
+}
+
+/* ---- results, org/qmckl_jastrow_champ.org:3405, 4013, 6839, 7323, 13836, 14394, 16409, 16767 */
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, F_EE, out, n, "qmckl_get_jastrow_champ_factor_ee");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, F_EE_GL, out, n, "qmckl_get_jastrow_champ_factor_ee_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, F_EN, out, n, "qmckl_get_jastrow_champ_factor_en");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, F_EN_GL, out, n, "qmckl_get_jastrow_champ_factor_en_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, F_EEN, out, n, "qmckl_get_jastrow_champ_factor_een");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, F_EEN_GL, out, n, "qmckl_get_jastrow_champ_factor_een_gl");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_value(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, J_VALUE, out, n, "qmckl_get_jastrow_champ_value");
+}
+qmckl_exit_code qmckl_get_jastrow_champ_gl(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_result(ctx, J_GL, out, n, "qmckl_get_jastrow_champ_gl");
+}
+
+// org/qmckl_jastrow_champ.org:16823-16880 and :17314-17350: grad[walk][3][elec] = e^J grad J
+qmckl_exit_code qmckl_get_jastrow_champ_grad(qmckl_context context, double* const grad, const int64_t size_max) {
+  const char* fn = "qmckl_get_jastrow_champ_grad";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (grad == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  const int64_t W = c->walk_num, Ne = c->elec_num;
+  if (size_max < W * 3 * Ne) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  // The first three rows of each walker's GL block are the gradient.
+  hipError_t e = hipMemcpy2DAsync(grad, sizeof(double) * 3 * Ne, c->d_jgl.ptr, sizeof(double) * 4 * Ne,
+                                  sizeof(double) * 3 * Ne, W, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+/* ---- device-resident results */
+qmckl_exit_code qmckl_amd_get_jastrow_champ_value_device(qmckl_context context, double* const d_value, const int64_t size_max) {
+  const char* fn = "qmckl_amd_get_jastrow_champ_value_device";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (d_value == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (size_max < c->walk_num) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  hipError_t e = hipMemcpyAsync(d_value, c->d_jvalue.ptr, sizeof(double) * c->walk_num, hipMemcpyDeviceToDevice, c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, double* const d_gl, const int64_t size_max) {
+  const char* fn = "qmckl_amd_get_jastrow_champ_gl_device";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (d_gl == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  const int64_t n = c->walk_num * 4 * c->elec_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  hipError_t e = hipMemcpyAsync(d_gl, c->d_jgl.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+}  // extern "C"

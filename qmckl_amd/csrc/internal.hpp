@@ -1,0 +1,258 @@
+// Internal state of a context and the interface between the host runtime (api_*.cpp)
+// and the HIP kernels (kernels_*.hip).  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/qmckl.h"
+
+namespace qmckl_amd {
+
+// Growable device allocation owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  // Makes the buffer at least `n` bytes; the contents are not kept.
+  hipError_t reserve(size_t n);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Basis tables laid out for the AO kernel (built when the basis is complete).
+struct AoTables {
+  int64_t nucl_num = 0, shell_num = 0, prim_num = 0, ao_num = 0;
+  int32_t lmax = 0;
+  // per shell
+  std::vector<int32_t> shell_nucl, shell_l, shell_prim_start, shell_prim_count;
+  std::vector<int32_t> shell_ao_start;
+  // per primitive (shell order)
+  std::vector<double> prim_expo, prim_coef;  // coef = coefficient*prim_factor*shell_factor
+  // per AO
+  std::vector<int32_t> ao_shell;
+  std::vector<int32_t> ao_pow;  // a | b<<8 | c<<16
+  std::vector<double> ao_factor;
+  // per nucleus
+  std::vector<double> nucl_range;  // max 1/alpha over its primitives
+  std::vector<int32_t> nucl_max_l;
+};
+
+// Device copies of the constant data (uploaded lazily before the first kernel).
+struct DevConst {
+  DevBuf shell_nucl, shell_l, shell_prim_start, shell_prim_count, shell_ao_start;
+  DevBuf prim_expo, prim_coef, ao_shell, ao_pow, ao_factor, nucl_range;
+  DevBuf nucl_coord;  // [3][nucl]
+  DevBuf coef_t;      // MO coefficients transposed [ao][mo_pad] (FP64)
+  DevBuf coef_t32;    // FP32 copy
+  DevBuf coef_t_bf;   // three bf16 planes
+  int64_t mo_pad = 0;
+  DevBuf jas_a, jas_b, jas_cfull, jas_type, jas_kappa_en, jas_asymp_a;
+  DevBuf prim_nucl_c, prim_expo_c;  // caller-order primitives (primitive VGL getter)
+  uint64_t date = 0;      // context date of the basis upload
+  uint64_t jas_date = 0;  // context date of the Jastrow upload
+};
+
+struct ProfEvent {
+  int slot;
+  hipEvent_t start, stop;
+};
+
+struct Context {
+  uint32_t tag = 0xBEEFFACEu;
+  std::recursive_mutex mutex;
+  uint64_t date = 1;        // bumped by every setter
+  uint64_t const_date = 1;  // bumped by every setter except the coordinate setters
+  int32_t num_threads = 1;
+
+  // error record, org/qmckl_error.org:433-470
+  qmckl_exit_code err_code = QMCKL_SUCCESS;
+  std::string err_function, err_message;
+
+  // numprec, org/qmckl_numprec.org:101-147
+  int32_t precision = 53, range = 11;
+
+  // point, org/qmckl_point.org:105-109: coord is [3][num]
+  int64_t point_num = 0;
+  std::vector<double> point_coord;
+  bool point_on_device = false;  // last coordinates came from device memory
+  uint64_t point_date = 0;
+
+  // nucleus, org/qmckl_nucleus.org
+  int64_t nucl_num = 0;
+  std::vector<double> nucl_charge, nucl_coord;  // coord [3][num]
+  uint32_t nucl_uninit = 0x7;
+  bool nucl_provided = false;
+
+  // electron, org/qmckl_electron.org:144-165
+  int64_t elec_num = 0, up_num = 0, down_num = 0, walk_num = 0;
+  uint32_t elec_uninit = 0x1;
+  bool elec_provided = false;
+
+  // AO basis, org/qmckl_ao.org:298-381
+  char ao_type = 0;
+  int64_t shell_num = 0, prim_num = 0, ao_num = 0;
+  std::vector<int64_t> nucleus_index, nucleus_shell_num, shell_prim_num, shell_prim_index;
+  std::vector<int32_t> shell_ang_mom, r_power;
+  std::vector<double> shell_factor, exponent, coefficient, prim_factor, ao_factor;
+  bool ao_cartesian = true;
+  uint32_t ao_uninit = (1u << 14) - 1;
+  bool ao_provided = false;
+  AoTables tables;
+
+  // MO basis, org/qmckl_mo.org:135-160
+  int64_t mo_num = 0;
+  std::vector<double> mo_coef;  // [mo][ao]
+  uint32_t mo_uninit = 0x3;
+  bool mo_provided = false;
+  int32_t mo_precision = QMCKL_AMD_MO_FP64;
+
+  // CHAMP Jastrow, org/qmckl_jastrow_champ.org:407-488
+  int64_t aord = -1, bord = -1, cord = -1, dim_c = -1, type_nucl_num = -1;
+  int32_t spin_independent = -1;
+  std::vector<int64_t> type_nucl_vector;
+  std::vector<double> a_vector, b_vector, c_vector, rescale_en;
+  double rescale_ee = 0.0;
+  uint32_t jas_uninit = (1u << 11) - 1;
+  bool jas_provided = false;
+  double asymp_jasb[2] = {0, 0};
+  std::vector<double> asymp_jasa;
+
+  // ---- device side ---------------------------------------------------------------
+  int device = -1;
+  hipStream_t stream = nullptr;      // stream in use
+  hipStream_t own_stream = nullptr;  // created by the context
+  DevConst dc;
+  DevBuf d_coord;  // [3][point_num]
+  uint64_t d_coord_date = 0;
+
+  DevBuf d_ao_vgl;  // [P][5][ao] FP64
+  uint64_t ao_vgl_date = 0;
+  DevBuf d_ao_work;  // chunk scratch for the AO->MO product
+  DevBuf d_mo_vgl;   // [P][5][mo]
+  uint64_t mo_vgl_date = 0;
+
+  DevBuf d_ee_dist, d_en_dist;
+  uint64_t ee_dist_date = 0, en_dist_date = 0;
+
+  // Jastrow results
+  DevBuf d_fee, d_fen, d_feen;           // [W]
+  DevBuf d_fee_gl, d_fen_gl, d_feen_gl;  // [W][4][Ne]
+  DevBuf d_jvalue, d_jgl;                // [W], [W][4][Ne]
+  DevBuf d_jscratch;                     // partial sums
+  uint64_t jastrow_date = 0;
+  // Jastrow intermediates (materialised only when asked for)
+  DevBuf d_jint;
+  uint64_t jint_date = 0;
+  int jint_kind = -1;
+
+  // kernel timing (profiling extension)
+  bool profiling = false;
+  std::vector<ProfEvent> prof_pending;
+  double prof_ms[QMCKL_AMD_KERNEL_SLOTS] = {0};
+  int64_t prof_count[QMCKL_AMD_KERNEL_SLOTS] = {0};
+};
+
+// Brackets the kernel launches of its scope with HIP events when profiling is on.
+struct ProfScope {
+  Context* c;
+  int slot;
+  hipEvent_t start = nullptr;
+  ProfScope(Context* ctx, int s);
+  ~ProfScope();
+};
+enum { SLOT_AO = 0, SLOT_MO = 1, SLOT_JPAIR = 2, SLOT_JEEN = 3, SLOT_JTOTAL = 4 };
+
+// ---- helpers shared by the api_*.cpp files ----------------------------------------
+// A change of constant data (basis, MOs, Jastrow parameters, nuclei, precision).
+inline void touch_const(Context* c) {
+  c->date += 1;
+  c->const_date = c->date;
+}
+Context* ctx_of(qmckl_context context);  // nullptr if not a live context
+qmckl_exit_code fail(qmckl_context context, qmckl_exit_code code, const char* fn, const char* msg);
+// Makes the device, stream and constant tables ready.
+qmckl_exit_code device_ready(Context* c, qmckl_context handle, const char* fn);
+qmckl_exit_code upload_coords(Context* c, qmckl_context handle, const char* fn);
+qmckl_exit_code hip_fail(qmckl_context handle, hipError_t e, const char* fn);
+
+// providers
+qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn);
+qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, const char* fn);
+qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn);
+qmckl_exit_code provide_ee_distance(qmckl_context handle, Context* c, const char* fn);
+qmckl_exit_code provide_en_distance(qmckl_context handle, Context* c, const char* fn);
+
+// ---- kernel launchers (kernels_*.hip) ----------------------------------------------
+struct AoDevView {
+  int64_t nucl_num, shell_num, ao_num;
+  const int32_t *shell_nucl, *shell_l, *shell_prim_start, *shell_prim_count, *shell_ao_start;
+  const double *prim_expo, *prim_coef;
+  const int32_t *ao_shell, *ao_pow;
+  const double *ao_factor, *nucl_range, *nucl_coord;
+  double cutoff;
+};
+// out_kind: 0 = FP64 [P][5][ao] (ld = ao_num), 1 = FP32, 2 = bf16x3 planes
+hipError_t launch_ao_vgl(const AoDevView& v, const double* coord, int64_t coord_ld, int64_t p0,
+                         int64_t np, void* out, int out_kind, int64_t ld_out, int64_t plane_stride,
+                         int values_only, hipStream_t s);
+hipError_t launch_shell_vgl(const AoDevView& v, const double* coord, int64_t P, double* out, hipStream_t s);
+hipError_t launch_prim_vgl(const AoDevView& v, int64_t prim_num, const int32_t* prim_nucl,
+                           const double* prim_expo, const double* coord, int64_t P, double* out,
+                           hipStream_t s);
+
+// C[m][n] = sum_k A[m][k] B[k][n], row-major; A has leading dim lda, B is [K][ldb], C ldc.
+hipError_t launch_gemm_f64(const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                           int64_t ldc, int64_t M, int64_t N, int64_t K, hipStream_t s);
+hipError_t launch_gemm_f32(const float* A, int64_t lda, const float* B, int64_t ldb, double* C,
+                           int64_t ldc, int64_t M, int64_t N, int64_t K, hipStream_t s);
+// A and B given as three bf16 planes each (plane stride in elements).
+hipError_t launch_gemm_bf16x3(const uint16_t* A, int64_t lda, int64_t a_plane, const uint16_t* B,
+                              int64_t ldb, int64_t b_plane, double* C, int64_t ldc, int64_t M,
+                              int64_t N, int64_t K, hipStream_t s);
+hipError_t launch_split_bf16x3(const double* src, int64_t n, uint16_t* dst, int64_t plane, hipStream_t s);
+hipError_t launch_to_f32(const double* src, int64_t n, float* dst, hipStream_t s);
+
+struct JasDevView {
+  int64_t walk_num, elec_num, up_num, nucl_num, aord, bord, cord, dim_c;
+  int32_t spin_independent;
+  double rescale_ee;
+  const double* coord;  // [3][W*Ne]
+  int64_t coord_ld;     // = W*Ne
+  const double* nucl_coord;  // [3][Nn]
+  const int64_t* type_nucl;  // [Nn]
+  const double* kappa_en;    // [type]
+  const double* a_vector;    // [type][aord+1]
+  const double* b_vector;    // [bord+1]
+  const double* cfull;       // [dim_c][Nn]
+  double asymp_jasb0, asymp_jasb1;
+  const double* asymp_jasa;  // [type]
+};
+struct JasOut {
+  double *fee, *fen, *feen;           // [W]
+  double *fee_gl, *fen_gl, *feen_gl;  // [W][4][Ne]
+  double *value, *gl;                 // [W], [W][4][Ne]
+  double* scratch;                    // partial sums, size jastrow_scratch_doubles()
+};
+int64_t jastrow_scratch_doubles(const JasDevView& v);
+// The three kernels of one evaluation, in this order (same stream).
+hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s);
+hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s);
+hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_t s);
+// Intermediates with the reference layouts; kind selects which one.
+enum JasInter {
+  J_EE_RESCALED = 0, J_EE_RESCALED_GL, J_EN_RESCALED, J_EN_RESCALED_GL,
+  J_EEN_E, J_EEN_E_GL, J_EEN_N, J_EEN_N_GL, J_TMP_C, J_DTMP_C
+};
+int64_t jastrow_inter_size(const JasDevView& v, int kind);
+hipError_t launch_jastrow_inter(const JasDevView& v, int kind, double* out, hipStream_t s);
+
+hipError_t launch_ee_distance(const double* coord, int64_t W, int64_t Ne, double* out, hipStream_t s);
+hipError_t launch_en_distance(const double* coord, int64_t P, const double* nucl_coord, int64_t Nn,
+                              double* out, hipStream_t s);
+
+}  // namespace qmckl_amd

@@ -1,0 +1,77 @@
+"""The reference's own test systems, rebuilt from tests/golden/systems.npz.
+
+chbrclf  CHBrClF molecule of include/chbrclf.h: 5 nuclei, 72 shells, 297 primitives,
+         263 AOs, 224 MOs, 2 walkers of 68 electrons (no Jastrow).
+n2       N2 molecule of include/n2.h: 2 nuclei, 2 walkers of 10 electrons, CHAMP Jastrow
+         orders 5/5/5, kappa_ee = kappa_en = 0.6 (org/qmckl_jastrow_champ.org:2014-2025).
+"""
+
+import json
+import os
+
+import numpy as np
+
+from qmckl_amd.system import System
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_DATA = None
+
+
+def data():
+    global _DATA
+    if _DATA is None:
+        _DATA = dict(np.load(os.path.join(HERE, "golden", "systems.npz"), allow_pickle=False))
+    return _DATA
+
+
+def goldens(var=None, file=None):
+    with open(os.path.join(HERE, "golden", "goldens.json")) as f:
+        g = json.load(f)
+    return [x for x in g if (var is None or x["var"] == var) and (file is None or x["file"].endswith(file))]
+
+
+def chbrclf():
+    """Returns (System, coord_T [3][W*Ne], walk_num)."""
+    d = data()
+    p = "chbrclf/chbrclf_"
+    nn = int(d[p + "nucl_num"])
+    ao = int(d[p + "ao_num"])
+    mo = int(d[p + "mo_num"])
+    W, Ne = int(d[p + "walk_num"]), int(d[p + "elec_num"])
+    up, dn = int(d[p + "elec_up_num"]), int(d[p + "elec_dn_num"])
+    s = System(
+        nucl_charge=d[p + "charge"], nucl_coord=d[p + "nucl_coord"].reshape(3, nn), up=up, dn=dn,
+        shell_ang_mom=d[p + "basis_shell_ang_mom"].astype(np.int32),
+        shell_prim_num=d[p + "basis_shell_prim_num"], shell_prim_index=d[p + "basis_shell_prim_index"],
+        nucleus_index=d[p + "basis_nucleus_index"], nucleus_shell_num=d[p + "basis_nucleus_shell_num"],
+        shell_factor=d[p + "basis_shell_factor"], exponent=d[p + "basis_exponent"],
+        coefficient=d[p + "basis_coefficient"], prim_factor=d[p + "basis_prim_factor"],
+        ao_factor=d[p + "basis_ao_factor"], mo_coef=d[p + "mo_coef"].reshape(mo, ao),
+        type_nucl_vector=np.zeros(nn, np.int64), a_vector=np.zeros((1, 6)), b_vector=np.zeros(6),
+        c_vector=np.zeros((1, 23)), rescale_en=np.ones(1), name="chbrclf",
+    )
+    x = d[p + "elec_coord"].reshape(W * Ne, 3).T.copy()
+    return s, x, W
+
+
+def n2():
+    """Returns (System, coord_T [3][W*Ne], walk_num); the AO/MO fields are unused."""
+    d = data()
+    p = "n2/n2_"
+    nn = int(d[p + "nucl_num"])
+    W, Ne = int(d[p + "walk_num"]), int(d[p + "elec_num"])
+    up, dn = int(d[p + "elec_up_num"]), int(d[p + "elec_dn_num"])
+    t = int(d[p + "type_nucl_num"])
+    s = System(
+        nucl_charge=d[p + "charge"], nucl_coord=d[p + "nucl_coord"].reshape(3, nn), up=up, dn=dn,
+        shell_ang_mom=np.zeros(0, np.int32), shell_prim_num=np.zeros(0, np.int64),
+        shell_prim_index=np.zeros(0, np.int64), nucleus_index=np.zeros(nn, np.int64),
+        nucleus_shell_num=np.zeros(nn, np.int64), shell_factor=np.zeros(0), exponent=np.zeros(0),
+        coefficient=np.zeros(0), prim_factor=np.zeros(0), ao_factor=np.zeros(0), mo_coef=np.zeros((0, 0)),
+        type_nucl_vector=d[p + "type_nucl_vector"], a_vector=d[p + "a_vector"].reshape(t, 6),
+        b_vector=d[p + "b_vector"], c_vector=d[p + "c_vector"].reshape(t, 23),
+        rescale_en=np.full(t, 0.6), rescale_ee=0.6, aord=int(d[p + "aord_num"]), bord=int(d[p + "bord_num"]),
+        cord=int(d[p + "cord_num"]), spin_independent=0, name="n2",
+    )
+    x = d[p + "elec_coord"].reshape(W * Ne, 3).T.copy()
+    return s, x, W
