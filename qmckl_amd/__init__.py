@@ -168,3 +168,85 @@ def get_scalar(ctx, name, ctype=ctypes.c_int64):
 
 def synchronize(ctx):
     _call(ctx, "qmckl_amd_synchronize")
+
+
+# ---- device buffers over the HIP runtime the library itself uses --------------------
+# (torch ships its own HIP runtime; mixing two runtimes in one process is not supported,
+# so device memory for the qmckl_amd_*_device entry points is managed here.)
+
+_HIP = None
+
+
+def hip():
+    """The HIP runtime libqmckl_amd.so is linked against."""
+    global _HIP
+    if _HIP is None:
+        lib()  # loads libamdhip64 as a dependency of the library
+        H = ctypes.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        H.hipFree.argtypes = [ctypes.c_void_p]
+        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        _HIP = H
+    return _HIP
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    if hip().hipGetDeviceCount(ctypes.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+class DeviceBuffer:
+    """A device allocation; `ptr` is the device address passed to the C-ABI."""
+
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        rc = hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 8))
+        if rc != 0:
+            raise MemoryError(f"hipMalloc({self.nbytes}) failed with {rc}")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        rc = hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, self.H2D)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy H2D failed with {rc}")
+
+    def to_array(self, shape, dtype=np.float64):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        rc = hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy D2H failed with {rc}")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    rc = hip().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed with {rc}")

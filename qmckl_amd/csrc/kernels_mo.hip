@@ -13,9 +13,11 @@
 //            through v_mfma_f32_16x16x32_bf16 into an FP32 accumulator that is then added
 //            into an FP64 one.
 //
-// MFMA operand layouts (16x16 blocks, 64 lanes, i = lane % 16, g = lane / 16):
-//   16x16x4 : A[i][g], B[g][i]; result rows 4g..4g+3 of column i.
-//   16x16x32: A[i][8g..8g+7], B[8g..8g+7][i]; result rows 4g..4g+3 of column i.
+// MFMA operand layouts (16x16 blocks, 64 lanes, i = lane % 16, g = lane / 16), measured
+// on gfx950:
+//   f32 16x16x4  : A[i][g], B[g][i]; result register q holds row 4g + q, column i.
+//   f64 16x16x4  : A[i][g], B[g][i]; result register q holds row g + 4q, column i.
+//   bf16 16x16x32: A[i][8g..8g+7], B[8g..8g+7][i]; register q holds row 4g + q, column i.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -77,6 +79,7 @@ __global__ void __launch_bounds__(256) k_gemm_f64(const double* __restrict__ A, 
     }
     __syncthreads();
   }
+  // FP64 16x16x4 result layout: register q of lane (g, i) holds row g + 4q, column i.
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -84,7 +87,7 @@ __global__ void __launch_bounds__(256) k_gemm_f64(const double* __restrict__ A, 
       const int64_t col = n0 + c * 16 + li;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int64_t row = m0 + wave * 32 + r * 16 + 4 * lg + q;
+        const int64_t row = m0 + wave * 32 + r * 16 + lg + 4 * q;
         if (row < M && col < N) C[row * ldc + col] = acc[r][c][q];
       }
     }
