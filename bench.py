@@ -125,7 +125,6 @@ def main():
     log("timing FP64 path")
     t64 = timed(q.MO_FP64)
     mo64 = mo.to_array((P, 5, Nmo))
-    gl64 = gl.to_array(n_gl)
     J64 = jastrow_sum()
     log(f"FP64: {t64 * 1e3:.3f} ms/step")
     k64 = kernel_times(q.MO_FP64)
@@ -139,7 +138,6 @@ def main():
     mo_err = [float(np.abs(a[:, c] - mo64[:, c]).max() / np.abs(mo64[:, c]).max()) for c in range(5)]
     del a, mo64
     dJ = float(np.abs(jastrow_sum() - J64).max())
-    dgl = float(np.abs(gl.to_array(n_gl) - gl64).max())
 
     costs = per_walker_costs(s)
     # dominant kernel of the split path and its roofline position
@@ -187,7 +185,6 @@ def main():
         "split_path": {"kernels": ksp, "speedup_vs_fp64": t64 / tsp},
         "mo_max_rel_err_vs_fp64": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
         "jastrow_max_abs_dJ_vs_fp64": dJ,
-        "jastrow_max_abs_dgl_vs_fp64": dgl,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,117 @@
+// Device helpers shared by the AO kernels and the fused AO->MO kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace qmckl_amd {
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// Three-term bfloat16 split by truncation of f = float(x): hi keeps the top 8 significand
+// bits of f, mid the next 8 of the (exact) remainder and lo the rest, so that
+// hi + mid + lo == f exactly; the only rounding is x -> float(x) (relative 2^-24).
+__device__ __forceinline__ void split3_trunc(double x, uint16_t& h, uint16_t& m, uint16_t& l) {
+  const float f = (float)x;
+  const uint32_t uf = __float_as_uint(f);
+  h = (uint16_t)(uf >> 16);
+  const float r1 = f - __uint_as_float(uf & 0xFFFF0000u);
+  const uint32_t u1 = __float_as_uint(r1);
+  m = (uint16_t)(u1 >> 16);
+  const float r2 = r1 - __uint_as_float(u1 & 0xFFFF0000u);
+  l = (uint16_t)(__float_as_uint(r2) >> 16);
+}
+
+// Three-term bfloat16 split of a double: x ~ hi + mid + lo, each term rounded to nearest.
+__device__ __forceinline__ void split3(double x, uint16_t& h, uint16_t& m, uint16_t& l) {
+  h = bf16_rne((float)x);
+  const double r1 = x - (double)bf16_to_f(h);
+  m = bf16_rne((float)r1);
+  const double r2 = r1 - (double)bf16_to_f(m);
+  l = bf16_rne((float)r2);
+}
+
+// Value, gradient and Laplacian of x^a y^b z^c at Y = (x,y,z) + 1e-20, with the
+// multiplication order of qmckl_ao_polynomial_vgl_doc.
+// y^n, y^(n-1) and y^(n-2) by repeated multiplication (negative powers are returned as 1:
+// they are always multiplied by a zero factor).
+__device__ __forceinline__ void powers3(double y, int n, double& p0, double& p1, double& p2) {
+  p0 = 1.0;
+  p1 = 1.0;
+  p2 = 1.0;
+  for (int i = 0; i < n; ++i) {
+    p2 = p1;
+    p1 = p0;
+    p0 = p0 * y;
+  }
+}
+
+__device__ __forceinline__ void monomial_vgl(double yx, double yy, double yz, int a, int b, int c, double p[5]) {
+  double xa, xa1, xa2, yb, yb1, yb2, zc, zc1, zc2;
+  powers3(yx, a, xa, xa1, xa2);
+  powers3(yy, b, yb, yb1, yb2);
+  powers3(yz, c, zc, zc1, zc2);
+  const double da = a, db = b, dc = c;
+  double xy = xa * yb;
+  double yzv = yb * zc;
+  double xz = xa * zc;
+  p[0] = xy * zc;
+  xy = dc * xy;
+  xz = db * xz;
+  yzv = da * yzv;
+  p[1] = xa1 * yzv;
+  p[2] = yb1 * xz;
+  p[3] = zc1 * xy;
+  p[4] = (da - 1.0) * xa2 * yzv + (db - 1.0) * yb2 * xz + (dc - 1.0) * zc2 * xy;
+}
+
+// Radial value, gradient and Laplacian of shell s (on nucleus a) at displacement
+// (x, y, z) = r - R_a, r2 = |r - R_a|^2; org/qmckl_ao.org:4769-4868: the nucleus is skipped
+// when r2 > cutoff * range, a primitive when alpha r2 > cutoff.
+__device__ __forceinline__ void shell_radial(const AoDevView& v, int s, int a, double x, double y, double z,
+                                             double r2, double r[5]) {
+  const double cutoff = v.cutoff;
+  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+  if (!(r2 > cutoff * v.nucl_range[a])) {
+    const int p0 = v.shell_prim_start[s], np = v.shell_prim_count[s];
+    for (int i = 0; i < np; ++i) {
+      const double al = v.prim_expo[p0 + i];
+      const double ar2 = al * r2;
+      if (ar2 > cutoff) continue;
+      const double e = v.prim_coef[p0 + i] * exp(-ar2);
+      const double two_a = -2.0 * al * e;
+      v0 += e;
+      v1 += two_a * x;
+      v2 += two_a * y;
+      v3 += two_a * z;
+      v4 += two_a * (3.0 - 2.0 * ar2);
+    }
+  }
+  r[0] = v0;
+  r[1] = v1;
+  r[2] = v2;
+  r[3] = v3;
+  r[4] = v4;
+}
+
+// AO VGL from the monomial of AO k at displacement d = (x, y, z) and its shell's radial
+// VGL sh[5]; org/qmckl_ao.org:8530-8565.
+__device__ __forceinline__ void ao_from_shell(const AoDevView& v, int k, double x, double y, double z,
+                                              const double* sh, double r[5]) {
+  const int pw = v.ao_pow[k];
+  double pv[5];
+  monomial_vgl(x + 1.e-20, y + 1.e-20, z + 1.e-20, pw & 0xff, (pw >> 8) & 0xff, (pw >> 16) & 0xff, pv);
+  const double f = v.ao_factor[k];
+  r[0] = pv[0] * sh[0] * f;
+  r[1] = (pv[1] * sh[0] + pv[0] * sh[1]) * f;
+  r[2] = (pv[2] * sh[0] + pv[0] * sh[2]) * f;
+  r[3] = (pv[3] * sh[0] + pv[0] * sh[3]) * f;
+  r[4] = (pv[4] * sh[0] + pv[0] * sh[4] + 2.0 * (pv[1] * sh[1] + pv[2] * sh[2] + pv[3] * sh[3])) * f;
+}
+
+}  // namespace qmckl_amd

@@ -76,6 +76,11 @@ qmckl_exit_code finalize_basis(Context* c, qmckl_context ctx) {
       return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "shells must be stored nucleus by nucleus");
   t.shell_num = (int64_t)t.shell_l.size();
   t.ao_factor = c->ao_factor;
+  t.chunk_shells_max = 1;
+  for (int64_t k0 = 0; k0 < ao; k0 += 32) {
+    const int64_t k1 = std::min<int64_t>(k0 + 32, ao) - 1;
+    t.chunk_shells_max = std::max<int32_t>(t.chunk_shells_max, t.ao_shell[k1] - t.ao_shell[k0] + 1);
+  }
   touch_const(c);
   return QMCKL_SUCCESS;
 }
@@ -152,21 +157,26 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     up(d.prim_expo_c, c->exponent.data(), 8 * c->exponent.size());
   }
   if (e == hipSuccess && c->mo_provided) {
-    // coefficient_t [ao][mo_pad] (org/qmckl_mo.org:316-375), padded to a multiple of 16
-    // columns with zeros so the GEMM tiles need no column guard on B.
+    // coefficient_t (org/qmckl_mo.org:316-375) zero-padded to k_pad = 32*ceil(ao/32) rows
+    // and mo_pad = 16*ceil(mo/16) columns, so the product kernels need no guards on it.
     const int64_t Nao = c->ao_num, Nmo = c->mo_num;
-    const int64_t pad = ((Nmo + 15) / 16) * 16;
-    std::vector<double> ct((size_t)Nao * pad, 0.0);
+    const int64_t pad = ((Nmo + 15) / 16) * 16, kpad = ((Nao + 31) / 32) * 32;
+    std::vector<double> ct((size_t)kpad * pad, 0.0), nk((size_t)pad * kpad, 0.0);
     for (int64_t k = 0; k < Nao; ++k)
-      for (int64_t j = 0; j < Nmo; ++j) ct[k * pad + j] = c->mo_coef[j * Nao + k];
+      for (int64_t j = 0; j < Nmo; ++j) {
+        ct[k * pad + j] = c->mo_coef[j * Nao + k];
+        nk[j * kpad + k] = c->mo_coef[j * Nao + k];
+      }
     d.mo_pad = pad;
+    d.k_pad = kpad;
     up(d.coef_t, ct.data(), 8 * ct.size());
+    up(d.coef_nk, nk.data(), 8 * nk.size());
     if (e == hipSuccess) e = d.coef_t32.reserve(4 * ct.size());
     if (e == hipSuccess) e = launch_to_f32(d.coef_t.as<double>(), (int64_t)ct.size(), d.coef_t32.as<float>(), c->stream);
-    if (e == hipSuccess) e = d.coef_t_bf.reserve(2 * 3 * ct.size());
+    if (e == hipSuccess) e = d.coef_t_bf.reserve(2 * 3 * nk.size());
     if (e == hipSuccess)
-      e = launch_split_bf16x3(d.coef_t.as<double>(), (int64_t)ct.size(), d.coef_t_bf.as<uint16_t>(),
-                              (int64_t)ct.size(), c->stream);
+      e = launch_split_bf16x3(d.coef_nk.as<double>(), (int64_t)nk.size(), d.coef_t_bf.as<uint16_t>(),
+                              (int64_t)nk.size(), c->stream);
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   d.date = c->const_date;
@@ -221,8 +231,8 @@ qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn)
 }
 
 // MO VGL into d_out (or into the context buffer when d_out is null); org/qmckl_mo.org:1749-1878.
-// The AO VGL is produced chunk by chunk in the precision of the product and consumed by
-// the GEMM, unless the FP64 AO VGL of these points is already resident.
+// The AO VGL is evaluated inside the product kernel (it is never stored), unless the FP64
+// AO VGL of these points is already resident, in which case it is multiplied directly.
 qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, const char* fn) {
   if (!c->mo_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
   if (d_out == nullptr && c->mo_vgl_date >= c->date) return QMCKL_SUCCESS;
@@ -237,41 +247,18 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     out = c->d_mo_vgl.as<double>();
   }
   const int mode = c->mo_precision;
-  const int64_t pad = c->dc.mo_pad;
+  const int64_t pad = c->dc.mo_pad, kpad = c->dc.k_pad;
+  ProfScope ps(c, SLOT_MO);
   if (mode == QMCKL_AMD_MO_FP64 && c->ao_vgl_date >= c->date) {
-    ProfScope ps(c, SLOT_MO);
+    // The FP64 AO VGL of these points is resident: plain GEMM on it.
     e = launch_gemm_f64(c->d_ao_vgl.as<double>(), Nao, c->dc.coef_t.as<double>(), pad, out, Nmo, 5 * P, Nmo,
                         Nao, c->stream);
   } else {
-    // Chunk of points whose AO VGL stays within a 1 GiB working buffer.
-    const size_t elem = mode == QMCKL_AMD_MO_FP64 ? 8 : (mode == QMCKL_AMD_MO_FP32 ? 4 : 6);
-    const size_t per_point = elem * 5 * (size_t)Nao;
-    int64_t chunk = (int64_t)((size_t(1) << 30) / per_point);
-    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, P));
-    e = c->d_ao_work.reserve(per_point * chunk);
-    for (int64_t p0 = 0; e == hipSuccess && p0 < P; p0 += chunk) {
-      const int64_t np = std::min(chunk, P - p0);
-      const int64_t M = 5 * np;
-      double* o = out + p0 * 5 * Nmo;
-      const int kind = mode == QMCKL_AMD_MO_FP64 ? 0 : (mode == QMCKL_AMD_MO_FP32 ? 1 : 2);
-      const int64_t plane = M * Nao;
-      {
-        ProfScope ps(c, SLOT_AO);
-        e = launch_ao_vgl(ao_view(c), c->d_coord.as<double>(), P, p0, np, c->d_ao_work.ptr, kind, Nao,
-                          kind == 2 ? plane : 0, 0, c->stream);
-      }
-      if (e != hipSuccess) break;
-      ProfScope ps(c, SLOT_MO);
-      if (mode == QMCKL_AMD_MO_FP64)
-        e = launch_gemm_f64(c->d_ao_work.as<double>(), Nao, c->dc.coef_t.as<double>(), pad, o, Nmo, M, Nmo, Nao,
-                            c->stream);
-      else if (mode == QMCKL_AMD_MO_FP32)
-        e = launch_gemm_f32(c->d_ao_work.as<float>(), Nao, c->dc.coef_t32.as<float>(), pad, o, Nmo, M, Nmo, Nao,
-                            c->stream);
-      else
-        e = launch_gemm_bf16x3(c->d_ao_work.as<uint16_t>(), Nao, plane, c->dc.coef_t_bf.as<uint16_t>(), pad,
-                               Nao * pad, o, Nmo, M, Nmo, Nao, c->stream);
-    }
+    const void* B = mode == QMCKL_AMD_MO_FP64 ? c->dc.coef_t.ptr
+                    : mode == QMCKL_AMD_MO_FP32 ? c->dc.coef_t32.ptr
+                                                : c->dc.coef_t_bf.ptr;
+    e = launch_ao_mo(ao_view(c), c->d_coord.as<double>(), P, 0, P, mode, B, kpad, pad, out, Nmo, Nmo,
+                     c->tables.chunk_shells_max, c->stream);
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   if (d_out == nullptr) c->mo_vgl_date = c->date;

@@ -40,6 +40,7 @@ struct AoTables {
   // per nucleus
   std::vector<double> nucl_range;  // max 1/alpha over its primitives
   std::vector<int32_t> nucl_max_l;
+  int32_t chunk_shells_max = 1;  // most shells met by the AOs of one 32-AO chunk
 };
 
 // Device copies of the constant data (uploaded lazily before the first kernel).
@@ -47,10 +48,11 @@ struct DevConst {
   DevBuf shell_nucl, shell_l, shell_prim_start, shell_prim_count, shell_ao_start;
   DevBuf prim_expo, prim_coef, ao_shell, ao_pow, ao_factor, nucl_range;
   DevBuf nucl_coord;  // [3][nucl]
-  DevBuf coef_t;      // MO coefficients transposed [ao][mo_pad] (FP64)
+  DevBuf coef_t;      // MO coefficients transposed [k_pad][mo_pad] (FP64, zero-padded)
   DevBuf coef_t32;    // FP32 copy
-  DevBuf coef_t_bf;   // three bf16 planes
-  int64_t mo_pad = 0;
+  DevBuf coef_nk;     // FP64 [mo_pad][k_pad] (source of the bf16 planes)
+  DevBuf coef_t_bf;   // three bf16 planes [3][mo_pad][k_pad]
+  int64_t mo_pad = 0, k_pad = 0;
   DevBuf jas_a, jas_b, jas_cfull, jas_type, jas_kappa_en, jas_asymp_a;
   DevBuf prim_nucl_c, prim_expo_c;  // caller-order primitives (primitive VGL getter)
   uint64_t date = 0;      // context date of the basis upload
@@ -215,6 +217,12 @@ hipError_t launch_gemm_bf16x3(const uint16_t* A, int64_t lda, int64_t a_plane, c
                               int64_t ldb, int64_t b_plane, double* C, int64_t ldc, int64_t M,
                               int64_t N, int64_t K, hipStream_t s);
 hipError_t launch_split_bf16x3(const double* src, int64_t n, uint16_t* dst, int64_t plane, hipStream_t s);
+// Fused AO VGL -> MO VGL for points p0 .. p0+np-1 (mode 0 FP64, 1 FP32, 2 BF16X3).
+// B: [k_pad][mo_pad] (FP64/FP32) or [3][mo_pad][k_pad] bf16 planes; C: [points][5][ldc].
+// ns_max: largest number of shells among the AOs of one 32-AO chunk.
+hipError_t launch_ao_mo(const AoDevView& v, const double* coord, int64_t coord_ld, int64_t p0, int64_t np, int mode,
+                        const void* B, int64_t Kpad, int64_t Npad, double* C, int64_t ldc, int64_t N, int ns_max,
+                        hipStream_t s);
 hipError_t launch_to_f32(const double* src, int64_t n, float* dst, hipStream_t s);
 
 struct JasDevView {

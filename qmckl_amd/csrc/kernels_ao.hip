@@ -15,62 +15,13 @@
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
+#include "ao_device.hpp"
 
 namespace qmckl_amd {
 
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-
-// Three-term bfloat16 split of a double: x ~ hi + mid + lo, each term rounded to nearest.
-__device__ __forceinline__ void split3(double x, uint16_t& h, uint16_t& m, uint16_t& l) {
-  h = bf16_rne((float)x);
-  const double r1 = x - (double)bf16_to_f(h);
-  m = bf16_rne((float)r1);
-  const double r2 = r1 - (double)bf16_to_f(m);
-  l = bf16_rne((float)r2);
-}
-
-// Value, gradient and Laplacian of x^a y^b z^c at Y = (x,y,z) + 1e-20, with the
-// multiplication order of qmckl_ao_polynomial_vgl_doc.
-// y^n, y^(n-1) and y^(n-2) by repeated multiplication (negative powers are returned as 1:
-// they are always multiplied by a zero factor).
-__device__ __forceinline__ void powers3(double y, int n, double& p0, double& p1, double& p2) {
-  p0 = 1.0;
-  p1 = 1.0;
-  p2 = 1.0;
-  for (int i = 0; i < n; ++i) {
-    p2 = p1;
-    p1 = p0;
-    p0 = p0 * y;
-  }
-}
-
-__device__ __forceinline__ void monomial_vgl(double yx, double yy, double yz, int a, int b, int c, double p[5]) {
-  double xa, xa1, xa2, yb, yb1, yb2, zc, zc1, zc2;
-  powers3(yx, a, xa, xa1, xa2);
-  powers3(yy, b, yb, yb1, yb2);
-  powers3(yz, c, zc, zc1, zc2);
-  const double da = a, db = b, dc = c;
-  double xy = xa * yb;
-  double yzv = yb * zc;
-  double xz = xa * zc;
-  p[0] = xy * zc;
-  xy = dc * xy;
-  xz = db * xz;
-  yzv = da * yzv;
-  p[1] = xa1 * yzv;
-  p[2] = yb1 * xz;
-  p[3] = zc1 * xy;
-  p[4] = (da - 1.0) * xa2 * yzv + (db - 1.0) * yb2 * xz + (dc - 1.0) * zc2 * xy;
-}
 
 // Stages per-(point, nucleus) displacement and per-(point, shell) radial VGL in LDS.
 __device__ void stage_shells(const AoDevView& v, const double* __restrict__ coord, int64_t coord_ld, int64_t pbase,
@@ -89,28 +40,14 @@ __device__ void stage_shells(const AoDevView& v, const double* __restrict__ coor
     o[3] = x * x + y * y + z * z;
   }
   __syncthreads();
-  const double cutoff = v.cutoff;
   for (int t = threadIdx.x; t < tp * Ns; t += blockDim.x) {
     const int q = t / Ns, s = t - q * Ns;
     const int a = v.shell_nucl[s];
     const double* d = s_nuc + 4 * (q * Nn + a);
     const double x = d[0], y = d[1], z = d[2], r2 = d[3];
-    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
-    if (!(r2 > cutoff * v.nucl_range[a])) {
-      const int p0 = v.shell_prim_start[s], np = v.shell_prim_count[s];
-      for (int i = 0; i < np; ++i) {
-        const double al = v.prim_expo[p0 + i];
-        const double ar2 = al * r2;
-        if (ar2 > cutoff) continue;
-        const double e = v.prim_coef[p0 + i] * exp(-ar2);
-        const double two_a = -2.0 * al * e;
-        v0 += e;
-        v1 += two_a * x;
-        v2 += two_a * y;
-        v3 += two_a * z;
-        v4 += two_a * (3.0 - 2.0 * ar2);
-      }
-    }
+    double r[5];
+    shell_radial(v, s, a, x, y, z, r2, r);
+    const double v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3], v4 = r[4];
     double* o = s_shell + 5 * t;
     o[0] = v0;
     o[1] = v1;
