@@ -22,6 +22,8 @@
 // Sums are reduced in a fixed order (no atomics), so results are run-to-run reproducible.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace qmckl_amd {
@@ -80,7 +82,7 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
 // R0 scratch layout: [W][Ne][cord-1][5]; gN: [W][Nn][Ne] = exp(-kappa_a |r_i - R_a|).
 // ------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
-                                                     double* __restrict__ gN) {
+                                                     double* __restrict__ gN, double* __restrict__ Sg) {
   extern __shared__ double lds[];
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
   const int64_t w = blockIdx.x;
@@ -236,6 +238,25 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   if (threadIdx.x == 0) {
     o.fee[w] = see;
     o.fen[w] = sen;
+  }
+  // Sg[w][a][q-1] = sum_i g_a(r_i)^q, q = 1..cord-1: the k = 0 sums of the e-e-n term over
+  // all electrons (the barriers of block_sum make this block's gN writes visible).
+  if (Sg != nullptr && C >= 2) {
+    for (int a = threadIdx.x; a < Nn; a += kBlock) {
+      double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      const double* ga = gN + (w * Nn + a) * Ne;
+#pragma unroll 4
+      for (int i = 0; i < Ne; ++i) {
+        const double g = ga[i];
+        double p = g;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          acc[q] += p;
+          p *= g;
+        }
+      }
+      for (int q = 0; q < C - 1 && q < 7; ++q) Sg[(w * Nn + a) * (C - 1) + q] = acc[q];
+    }
   }
 }
 
@@ -438,6 +459,384 @@ __global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, cons
   }
 }
 
+// ------------------------------------------------------------------------------------
+// e-e-n term on FP64 matrix cores (cord 2..6).
+// Workgroup = (walker, tile of 16 electrons i).  With w = (1, u_x, u_y, u_z, 1/r_ij) the
+// sums over j of k_jas_een are, for k = 1..C-1 and q = 1..C-k,
+//   X_c[k][q](i, a) = sum_j A_{k,c}(i, j) B_q(j, a),  A_{k,c} = g_e(r_ij)^k w_c,  B_q = g_a(r_j)^q,
+// evaluated per 16-nucleus tile with v_mfma_f64_16x16x4_f64 (rows i, columns a, contraction
+// over j four at a time).  X_0 is tmp_c; X_1..3 and X_4 give dtmp_c.  Wave (g, c) owns
+// component c and the powers k in {g + 1, C - g - 1}: C accumulator tiles per wave, the
+// same for every wave.  The epilogue of factor_een and factor_een_gl (:14121-14175,
+// :14875-14945) is linear in P and dP, so each wave folds its own sums into per-(i, a)
+// contributions to (J, grad_i J, lap_i J); these are summed over the nuclei across lanes and
+// over the waves in LDS, in a fixed order.  The q = 0 sums come from k_jas_pair (R0); the
+// k = 0 sums S[q] = sum_{j != i} g_a(r_j)^q are the sums over all j less the j = i term.
+// ------------------------------------------------------------------------------------
+constexpr int EM = 16;       // electrons i per workgroup
+constexpr int ECMP = 5;      // components V, Tx, Ty, Tz, L
+constexpr int EMFMA_MAX_C = 6;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int C>
+struct EenGroups {
+  static constexpr int n = (C - 1) / 2 + (C % 2 == 0 ? 1 : 0);  // pairs {k, C-k}, plus k = C/2
+  static constexpr int waves = ECMP * n;
+};
+
+template <int C, int G>
+struct EenGroup {
+  static constexpr int ka = G + 1, kb = C - G - 1;
+  static constexpr bool has_b = kb > ka;
+  static constexpr int na = C - ka;          // X[ka][q], q = 1..na, in acc[q - 1]
+  static constexpr int nb = has_b ? ka : 0;  // X[kb][q], q = 1..nb, in acc[C - q]
+};
+
+// One pass of wave (G, cmp) over nsteps x 4 electrons j held in LDS.  Lane (li, lg) feeds
+// A(i = li, j = 4 s + lg) and B(j = 4 s + lg, a = li) of v_mfma_f64_16x16x4_f64.
+template <int C, int G>
+__device__ __forceinline__ void een_mfma_pass(d4 (&acc)[C], const double* pdG, const double* pdR, const double* xj,
+                                              const double* gch, double xi_c, int JS, int nsteps, int cmp, int li,
+                                              int lg) {
+  using Gr = EenGroup<C, G>;
+  const double* xjc = xj + (cmp >= 1 && cmp <= 3 ? (cmp - 1) * JS : 0);
+  for (int st = 0; st < nsteps; ++st) {
+    const int jj = 4 * st + lg;
+    const double Ge = pdG[jj * EM + li];
+    const double ri = pdR[jj * EM + li];
+    // w_c: 1, u_x, u_y, u_z or 1/r (u = (r_i - r_j)/r_ij; 0 for j == i since 1/r is stored 0)
+    const double wc = (cmp == 0) ? 1.0 : (cmp == 4) ? ri : (xi_c - xjc[jj]) * ri;
+    const double g = gch[jj * 16 + li];
+    double gq[C];
+    gq[1] = g;
+#pragma unroll
+    for (int q = 2; q < C; ++q) gq[q] = gq[q - 1] * g;
+    double Gk = Ge;
+#pragma unroll
+    for (int k = 2; k <= Gr::ka; ++k) Gk *= Ge;
+    const double Aa = Gk * wc;
+#pragma unroll
+    for (int q = 1; q <= Gr::na; ++q) acc[q - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aa, gq[q], acc[q - 1], 0, 0, 0);
+    if (Gr::has_b) {
+#pragma unroll
+      for (int k = Gr::ka + 1; k <= Gr::kb; ++k) Gk *= Ge;
+      const double Ab = Gk * wc;
+#pragma unroll
+      for (int q = 1; q <= Gr::nb; ++q) acc[C - q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ab, gq[q], acc[C - q], 0, 0, 0);
+    }
+  }
+}
+
+// Contributions of component CMP, powers {ka, kb} of group G (and k = 0 for the first
+// wave), of the (l, k, m) terms to (J, gl_x, gl_y, gl_z, gl_lap) of one (i, a) pair.
+// Xa[q] = X[ka][q], Xb[q] = X[kb][q], X0[q] = X[0][q]; gp[p] = g_a(r_i)^p.  With
+// h = -kappa_a (r_i - R_a)/R the derivatives of g^p are d_c g^p = p h_c g^p (c < 3) and
+// lap g^p = (p^2 kappa_a^2 - 2 p kappa_a / R) g^p, so every gradient component is h_c times
+// one scalar sum over the terms, and the Laplacian is a combination of three such sums.
+template <int C, int G, int CMP>
+__device__ __forceinline__ void een_fold(const double* Xa, const double* Xb, const double* X0, const double* gp,
+                                         const double* h, double ka2, double kr, const double* cf, double kee,
+                                         double* out) {
+  using Gr = EenGroup<C, G>;
+  constexpr Lkm<C> T{};
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+  for (int n = 0; n < T.dim; ++n) {
+    const int l = T.l[n], k = T.k[n], m = T.m[n];
+    const double* X;
+    if (k == Gr::ka)
+      X = Xa;
+    else if (Gr::has_b && k == Gr::kb)
+      X = Xb;
+    else if (k == 0 && CMP == 0 && G == 0)
+      X = X0;
+    else
+      continue;
+    const double cn = cf[16 * n];  // c-coefficient n of this nucleus
+    const double am = X[m] * gp[m + l], aml = X[m + l] * gp[m];  // P[k][m] g^{m+l}, P[k][m+l] g^m
+    const double kk = (double)k * kee;
+    if (CMP == 0) {
+      s0 += cn * aml;                                                           // J
+      s1 += cn * ((double)(m + l) * am + (double)m * aml);                      // grad (x h_c)
+      s2 += cn * ((double)((m + l) * (m + l)) * am + (double)(m * m) * aml);  // lap, kappa^2 part
+      if (k != 0) s3 += cn * kk * kk * (am + aml);                              // lap of g_e^k
+    } else if (CMP <= 3) {
+      s0 += cn * kk * (am + aml);                                          // -grad_c
+      s1 += cn * kk * ((double)(m + l) * am + (double)m * aml);            // cross term (x -2 h_c)
+    } else {
+      s0 += cn * kk * (am + aml);                                          // lap (x -2)
+    }
+  }
+  if (CMP == 0) {
+    out[0] += s0;
+    out[1] += h[0] * s1;
+    out[2] += h[1] * s1;
+    out[3] += h[2] * s1;
+    out[4] += ka2 * s2 - kr * s1 + s3;
+  } else if (CMP <= 3) {
+    out[CMP] -= s0;
+    out[4] -= 2.0 * h[CMP - 1] * s1;
+  } else {
+    out[4] -= 2.0 * s0;
+  }
+}
+
+template <int C, int G>
+__device__ __forceinline__ void een_epilogue(const d4 (&acc)[C], int cmp, int r, const double* rr,
+                                             const double* Sl_a, const double* gp, const double* h, double ka2,
+                                             double kr, const double* cf, double kee, double* out) {
+  using Gr = EenGroup<C, G>;
+  double Xa[C], Xb[C], X0[C];
+  Xa[0] = rr[(Gr::ka - 1) * 5 + cmp];
+#pragma unroll
+  for (int q = 1; q <= Gr::na; ++q) {
+    const d4 x = acc[q - 1];
+    Xa[q] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
+  }
+  if (Gr::has_b) {
+    Xb[0] = rr[(Gr::kb - 1) * 5 + cmp];
+#pragma unroll
+    for (int q = 1; q <= Gr::nb; ++q) {
+      const d4 x = acc[C - q];
+      Xb[q] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
+    }
+  }
+  X0[0] = 0.0;
+  if (G == 0 && cmp == 0)
+#pragma unroll
+    for (int q = 1; q < C; ++q) X0[q] = Sl_a[q - 1] - gp[q];
+  switch (cmp) {
+    case 0: een_fold<C, G, 0>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
+    case 1: een_fold<C, G, 1>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
+    case 2: een_fold<C, G, 2>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
+    case 3: een_fold<C, G, 3>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
+    default: een_fold<C, G, 4>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
+  }
+}
+
+// Dynamic LDS of k_jas_een_mfma, in doubles, for a j range of JS electrons per pass.
+struct EenMfmaLds {
+  int JS, C, dim, NW;
+  int xt, xj, pdG, pdR, gch, ep, r0t, st, cfl, red, jrow, total;
+  __host__ __device__ EenMfmaLds(int JS_, int C_, int dim_, int NW_) : JS(JS_), C(C_), dim(dim_), NW(NW_) {
+    int o = 0;
+    xt = o;   o += EM * 3;               // tile electrons
+    xj = o;   o += 3 * JS;               // electrons j of the pass
+    pdG = o;  o += JS * EM;              // g_e(r_ij) = exp(-kappa r_ij), [j][i]; 0 for j == i
+    pdR = o;  o += JS * EM;              // 1 / r_ij, [j][i]; 0 for j == i
+    gch = o;  o += JS * 16;              // g_a(r_j) of the nucleus tile, [j][a]
+    ep = o;   o += (C + 5) * EM * 16;    // per (i, a): g^1..g^C, h_x, h_y, h_z, 2 kappa/R, kappa^2
+    r0t = o;  o += EM * (C - 1) * 5;     // q = 0 sums of the tile electrons
+    st = o;   o += 16 * (C - 1);         // sum over all j of g_a(r_j)^q
+    cfl = o;  o += dim * 16;             // c-coefficients of the nucleus tile
+    red = o;  o += NW * EM * 5;
+    jrow = o; o += EM;
+    total = o;
+  }
+};
+
+template <int C>
+__global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDevView v, JasOut o,
+                                                                          const double* __restrict__ R0,
+                                                                          const double* __restrict__ gN,
+                                                                          const double* __restrict__ Sg,
+                                                                          double* __restrict__ Jt, int ntiles, int JS) {
+  constexpr int K1 = C - 1;
+  constexpr int NW = EenGroups<C>::waves;
+  constexpr int NT = 64 * NW;
+  constexpr int DIM = Lkm<C>::dim;
+  extern __shared__ double lds[];
+  const EenMfmaLds L(JS, C, DIM, NW);
+  double* xt = lds + L.xt;
+  double* xj = lds + L.xj;
+  double* pdG = lds + L.pdG;
+  double* pdR = lds + L.pdR;
+  double* gch = lds + L.gch;
+  double* ep = lds + L.ep;
+  double* r0t = lds + L.r0t;
+  double* st = lds + L.st;
+  double* cfl = lds + L.cfl;
+  double* red = lds + L.red;
+  double* jrow = lds + L.jrow;
+  const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
+  const int64_t w = blockIdx.x / ntiles;
+  const int tile = (int)(blockIdx.x - w * ntiles);
+  const int i0 = tile * EM;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const int grp = wave / ECMP, cmp = wave - ECMP * grp;
+  const double kee = v.rescale_ee;
+  const double* coord = v.coord;
+  const int64_t cld = v.coord_ld;
+  const int npass = (Ne + JS - 1) / JS;
+  for (int t = tid; t < EM * 3; t += NT) {
+    const int il = t / 3, d = t - 3 * il;
+    xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
+  }
+  for (int t = tid; t < EM * K1 * 5; t += NT) {
+    const int il = t / (K1 * 5);
+    r0t[t] = (i0 + il < Ne) ? R0[(w * Ne + i0) * K1 * 5 + t] : 0.0;
+  }
+  __syncthreads();
+  // this lane's component of r_i - r_j needs x_i of row li
+  const double xi_c = (cmp >= 1 && cmp <= 3) ? xt[li * 3 + cmp - 1] : 0.0;
+
+  double total = 0.0;  // thread t < 5 EM: (J, gl) output t % 5 of row t / 5, summed over nuclei
+  for (int a0 = 0; a0 < Nn; a0 += 16) {
+    // per-tile constants: c-coefficients, the k = 0 sums, and per (i, a) the powers of
+    // g_a(r_i) and the factors of their derivatives
+    for (int t = tid; t < DIM * 16; t += NT) {
+      const int n = t >> 4, al = t & 15;
+      cfl[t] = (a0 + al < Nn) ? v.cfull[(int64_t)n * Nn + a0 + al] : 0.0;
+    }
+    for (int t = tid; t < 16 * K1; t += NT) {
+      const int al = t / K1;
+      st[t] = (a0 + al < Nn) ? Sg[(w * Nn + a0) * K1 + t] : 0.0;
+    }
+    for (int t = tid; t < EM * 16; t += NT) {
+      const int il = t >> 4, al = t & 15, i = i0 + il, a = a0 + al;
+      double gp[C + 1], h[3] = {0.0, 0.0, 0.0}, kr = 0.0, ka2 = 0.0;
+#pragma unroll
+      for (int p = 0; p <= C; ++p) gp[p] = 0.0;
+      if (i < Ne && a < Nn) {
+        const double ka = v.kappa_en[v.type_nucl[a]];
+        const double dx = xt[il * 3] - v.nucl_coord[a], dy = xt[il * 3 + 1] - v.nucl_coord[a + Nn],
+                     dz = xt[il * 3 + 2] - v.nucl_coord[a + 2 * Nn];
+        const double R = sqrt(dx * dx + dy * dy + dz * dz);
+        const double rinv = 1.0 / R;
+        const double g1 = gN[(w * Nn + a) * Ne + i];  // exp(-kappa_a R)
+        gp[1] = g1;
+#pragma unroll
+        for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * g1;
+        h[0] = -ka * (dx * rinv);
+        h[1] = -ka * (dy * rinv);
+        h[2] = -ka * (dz * rinv);
+        kr = 2.0 * ka * rinv;
+        ka2 = ka * ka;
+      }
+#pragma unroll
+      for (int p = 1; p <= C; ++p) ep[(p - 1) * EM * 16 + t] = gp[p];
+      ep[(C + 0) * EM * 16 + t] = h[0];
+      ep[(C + 1) * EM * 16 + t] = h[1];
+      ep[(C + 2) * EM * 16 + t] = h[2];
+      ep[(C + 3) * EM * 16 + t] = kr;
+      ep[(C + 4) * EM * 16 + t] = ka2;
+    }
+
+    d4 acc[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int pass = 0; pass < npass; ++pass) {
+      const int j0 = pass * JS;
+      const int jn = min(JS, Ne - j0);
+      const int jn4 = (jn + 3) & ~3;
+      if (npass > 1 || a0 == 0) {
+        if (npass > 1) __syncthreads();  // previous pass done with the pair data
+        for (int t = tid; t < 3 * jn4; t += NT) {
+          const int d = t / jn4, jj = t - d * jn4;
+          xj[d * JS + jj] = (jj < jn) ? coord[w * Ne + j0 + jj + d * cld] : 0.0;
+        }
+        for (int t = tid; t < EM * jn4; t += NT) {
+          const int il = t & (EM - 1), jj = t / EM;
+          const int i = i0 + il, j = j0 + jj;
+          double Ge = 0.0, ri = 0.0;
+          if (i < Ne && jj < jn && i != j) {
+            const double dx = xt[il * 3] - coord[w * Ne + j];
+            const double dy = xt[il * 3 + 1] - coord[w * Ne + j + cld];
+            const double dz = xt[il * 3 + 2] - coord[w * Ne + j + 2 * cld];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            ri = 1.0 / d;
+            Ge = exp(-kee * d);
+          }
+          pdG[t] = Ge;  // [jj][il]
+          pdR[t] = ri;
+        }
+      }
+      for (int t = tid; t < 16 * jn4; t += NT) {
+        const int jj = t % jn4, al = t / jn4;
+        const int a = a0 + al;
+        gch[jj * 16 + al] = (a < Nn && jj < jn) ? gN[(w * Nn + a) * Ne + j0 + jj] : 0.0;
+      }
+      __syncthreads();
+      const int nsteps = jn4 >> 2;
+      switch (grp) {
+        case 0: een_mfma_pass<C, 0>(acc, pdG, pdR, xj, gch, xi_c, JS, nsteps, cmp, li, lg); break;
+        case 1:
+          if constexpr (EenGroups<C>::n > 1) een_mfma_pass<C, 1>(acc, pdG, pdR, xj, gch, xi_c, JS, nsteps, cmp, li, lg);
+          break;
+        default:
+          if constexpr (EenGroups<C>::n > 2) een_mfma_pass<C, 2>(acc, pdG, pdR, xj, gch, xi_c, JS, nsteps, cmp, li, lg);
+          break;
+      }
+      if (pass + 1 < npass) __syncthreads();  // before gch of the next pass is written
+    }
+
+    // Epilogue: lane (li, lg) holds rows lg + 4 r (register r), column a = a0 + li.
+    const bool va = a0 + li < Nn;
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      const int il = lg + 4 * r, i = i0 + il;
+      double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (va && i < Ne) {
+        const int pidx = il * 16 + li;
+        double gp[C + 1], h[3];
+        gp[0] = 1.0;
+#pragma unroll
+        for (int p = 1; p <= C; ++p) gp[p] = ep[(p - 1) * EM * 16 + pidx];
+        h[0] = ep[(C + 0) * EM * 16 + pidx];
+        h[1] = ep[(C + 1) * EM * 16 + pidx];
+        h[2] = ep[(C + 2) * EM * 16 + pidx];
+        const double kr = ep[(C + 3) * EM * 16 + pidx], ka2 = ep[(C + 4) * EM * 16 + pidx];
+        const double* rr = r0t + il * K1 * 5;
+        const double* cf = cfl + li;
+        switch (grp) {
+          case 0: een_epilogue<C, 0>(acc, cmp, r, rr, st + li * K1, gp, h, ka2, kr, cf, kee, out); break;
+          case 1:
+            if constexpr (EenGroups<C>::n > 1) een_epilogue<C, 1>(acc, cmp, r, rr, st + li * K1, gp, h, ka2, kr, cf, kee, out);
+            break;
+          default:
+            if constexpr (EenGroups<C>::n > 2) een_epilogue<C, 2>(acc, cmp, r, rr, st + li * K1, gp, h, ka2, kr, cf, kee, out);
+            break;
+        }
+      }
+      // sum over the 16 nuclei of the tile: lanes li = 0..15 of this row group
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double x = out[c];
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        out[c] = x;
+      }
+      if (li == 0)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) red[(wave * EM + il) * 5 + c] = out[c];
+    }
+    __syncthreads();
+    if (tid < 5 * EM) {
+#pragma unroll
+      for (int wv = 0; wv < NW; ++wv) total += red[wv * EM * 5 + tid];
+    }
+    // the next tile rewrites cfl, st, ep and gch only after this barrier
+    __syncthreads();
+  }
+  if (tid < 5 * EM) {
+    const int il = tid / 5, c = tid - 5 * il, i = i0 + il;
+    if (c == 0)
+      jrow[il] = (i < Ne) ? total : 0.0;
+    else if (i < Ne)
+      o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = total;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sj = 0.0;
+    for (int il = 0; il < EM; ++il) sj += jrow[il];
+    Jt[w * ntiles + tile] = sj;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, const double* __restrict__ Jt,
                                                       int ntiles, int have_een) {
   __shared__ double sval;
@@ -473,6 +872,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, co
 }
 
 struct EenShape {
+  bool mfma;  // matrix-core kernel (cord <= EMFMA_MAX_C)
   int TI, JC, ntiles;
   size_t lds;
 };
@@ -480,6 +880,17 @@ struct EenShape {
 EenShape een_shape(const JasDevView& v) {
   EenShape s;
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
+  s.mfma = v.cord >= 2 && v.cord <= EMFMA_MAX_C;
+  if (s.mfma) {
+    // electrons j per LDS pass: all of them when they fit, else passes of 256
+    const int C = (int)v.cord;
+    s.TI = EM;
+    s.JC = std::min((Ne + 3) & ~3, 256);
+    s.ntiles = (Ne + EM - 1) / EM;
+    const int ng = (C - 1) / 2 + (C % 2 == 0 ? 1 : 0);
+    s.lds = sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, ECMP * ng).total;
+    return s;
+  }
   s.TI = kBlock / Nn;
   if (s.TI < 1) s.TI = 1;
   if (s.TI > Ne) s.TI = Ne;
@@ -687,12 +1098,13 @@ __global__ void k_en_distance(const double* __restrict__ coord, int64_t P, const
 int64_t jastrow_scratch_doubles(const JasDevView& v) {
   const int64_t C = v.cord;
   const int64_t r0 = C >= 2 ? v.walk_num * v.elec_num * (C - 1) * 5 : 0;
+  const int64_t sg = C >= 2 ? v.walk_num * v.nucl_num * (C - 1) : 0;
   const EenShape s = een_shape(v);
-  return r0 + v.walk_num * v.nucl_num * v.elec_num + v.walk_num * s.ntiles;
+  return r0 + v.walk_num * v.nucl_num * v.elec_num + v.walk_num * s.ntiles + sg;
 }
 
 struct ScratchView {
-  double *R0, *gN, *Jt;
+  double *R0, *gN, *Jt, *Sg;
 };
 
 ScratchView scratch_view(const JasDevView& v, const JasOut& o) {
@@ -702,6 +1114,7 @@ ScratchView scratch_view(const JasDevView& v, const JasOut& o) {
   sv.R0 = o.scratch;
   sv.gN = o.scratch + r0n;
   sv.Jt = sv.gN + v.walk_num * v.nucl_num * v.elec_num;
+  sv.Sg = sv.Jt + v.walk_num * een_shape(v).ntiles;
   return sv;
 }
 
@@ -709,7 +1122,8 @@ hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t
   const ScratchView sv = scratch_view(v, o);
   const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kBlock);
   if (lds_pair > 160 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_jas_pair, dim3((unsigned)v.walk_num), dim3(kBlock), lds_pair, s, v, o, sv.R0, sv.gN);
+  hipLaunchKernelGGL(k_jas_pair, dim3((unsigned)v.walk_num), dim3(kBlock), lds_pair, s, v, o, sv.R0, sv.gN,
+                     v.cord >= 2 ? sv.Sg : nullptr);
   return hipGetLastError();
 }
 
@@ -718,8 +1132,35 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
   const int C = (int)v.cord;
   if (!(C >= 2 && v.dim_c > 0)) return hipSuccess;
   const EenShape sh = een_shape(v);
-  if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(v.walk_num * sh.ntiles));
+  if (sh.mfma) {
+    if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
+#define EEN_MFMA_CASE(CC)                                                                                      \
+  case CC: {                                                                                                   \
+    static bool attr_set = false;                                                                              \
+    if (!attr_set) {                                                                                           \
+      const hipError_t ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_mfma<CC>),            \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);       \
+      if (ea != hipSuccess) return ea;                                                                         \
+      attr_set = true;                                                                                         \
+    }                                                                                                          \
+    hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenGroups<CC>::waves), sh.lds, s, v, o, sv.R0, sv.gN, \
+                       sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                         \
+    break;                                                                                                     \
+  }
+    switch (C) {
+      EEN_MFMA_CASE(2)
+      EEN_MFMA_CASE(3)
+      EEN_MFMA_CASE(4)
+      EEN_MFMA_CASE(5)
+      EEN_MFMA_CASE(6)
+      default:
+        return hipErrorInvalidValue;
+    }
+#undef EEN_MFMA_CASE
+    return hipGetLastError();
+  }
+  if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
 #define EEN_CASE(CC)                                                                                     \
   case CC:                                                                                               \
     hipLaunchKernelGGL(k_jas_een<CC>, grid, dim3(kBlock), sh.lds, s, v, o, sv.R0, sv.gN, sv.Jt, sh.TI, sh.JC, \

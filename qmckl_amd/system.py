@@ -251,15 +251,15 @@ def _place_nuclei(rng, n, box, min_sep):
     return np.array(pts).T.copy()  # [3][n]
 
 
-def _jastrow(rng, types, n_types):
-    dim_c = dim_c_vector(5)
+def _jastrow(rng, types, n_types, aord=5, bord=5, cord=5):
+    dim_c = dim_c_vector(cord)
     return dict(
         type_nucl_vector=np.array(types, np.int64),
-        a_vector=rng.uniform(-0.1, 0.1, (n_types, 6)),
-        b_vector=rng.uniform(-0.1, 0.1, 6),
-        c_vector=rng.uniform(-0.1, 0.1, (n_types, dim_c)),
+        a_vector=rng.uniform(-0.1, 0.1, (n_types, aord + 1)),
+        b_vector=rng.uniform(-0.1, 0.1, bord + 1),
+        c_vector=rng.uniform(-0.1, 0.1, (n_types, max(dim_c, 1))),
         rescale_en=np.ones(n_types),
-        rescale_ee=0.6, aord=5, bord=5, cord=5, spin_independent=0,
+        rescale_ee=0.6, aord=aord, bord=bord, cord=cord, spin_independent=0,
     )
 
 
@@ -323,3 +323,23 @@ def workload(name, walk_num=None, seed=None):
     W = W if walk_num is None else walk_num
     x = electron_coords(rng, s, W)
     return s, x, W
+
+
+def jastrow_workload(nucl_num, up, dn, aord=5, bord=5, cord=5, walk_num=16, seed=7):
+    """A synthetic system of C and H nuclei (alternating, two Jastrow types) in a cube of
+    side 2 Nn^(1/3) bohr, with Jastrow orders aord/bord/cord and an H-type basis on every
+    nucleus; returns (System, coord_T[3][W*Ne], walk_num).  Used to vary the shapes the
+    Jastrow kernels see (electron and nucleus counts off the tile sizes, every order)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    coord = _place_nuclei(rng, nucl_num, 2.0 * nucl_num ** (1.0 / 3.0), 1.4)
+    elements = ["C" if a % 2 == 0 else "H" for a in range(nucl_num)]
+    charge = np.array([6.0 if e == "C" else 1.0 for e in elements])
+    types = [0 if e == "C" else 1 for e in elements]
+    basis = _basis(rng, elements, {"C": (3, 2, 1), "H": (3, 2, 1)})
+    nao = basis["ao_factor"].size
+    mo = _orthonormal_rows(rng, max(up, dn), nao)
+    s = System(nucl_charge=charge, nucl_coord=coord, up=up, dn=dn, mo_coef=mo, precision=53,
+               name=f"jastrow-{nucl_num}n-{up + dn}e-c{cord}", **basis,
+               **_jastrow(rng, types, 2, aord, bord, cord))
+    x = electron_coords(rng, s, walk_num)
+    return s, x, walk_num

@@ -234,6 +234,18 @@ def test_jastrow_synthetic_vs_oracle(name, W):
     q.context_destroy(ctx)
 
 
+@pytest.mark.parametrize("nn,up,dn,aord,bord,cord", [
+    (20, 19, 18, 5, 5, 2), (20, 19, 18, 3, 4, 3), (17, 24, 16, 5, 5, 4), (33, 40, 39, 2, 6, 6),
+    (20, 19, 18, 5, 5, 7), (9, 12, 11, 4, 3, 8), (3, 3, 2, 5, 5, 5), (1, 1, 0, 5, 5, 5)])
+def test_jastrow_orders_and_shapes_vs_oracle(nn, up, dn, aord, bord, cord):
+    """Every e-e-n order the kernels are built for (2..8), with electron and nucleus counts
+    off the 16-wide tiles and the 32-electron chunks, down to a single electron."""
+    s, x, W = S.jastrow_workload(nn, up, dn, aord, bord, cord, walk_num=8)
+    ctx = make_ctx(s, x, W, mo=False)
+    _compare_jastrow(_jastrow_gpu(ctx, W, s.elec_num), oracle.jastrow(s, x, W), s.name)
+    q.context_destroy(ctx)
+
+
 def test_coordinate_update_recomputes():
     """A new set of coordinates invalidates every cached result (date semantics)."""
     s, x, W = S.workload("small", walk_num=16)
