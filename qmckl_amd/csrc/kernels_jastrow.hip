@@ -465,185 +465,182 @@ __global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, cons
 // sums over j of k_jas_een are, for k = 1..C-1 and q = 1..C-k,
 //   X_c[k][q](i, a) = sum_j A_{k,c}(i, j) B_q(j, a),  A_{k,c} = g_e(r_ij)^k w_c,  B_q = g_a(r_j)^q,
 // evaluated per 16-nucleus tile with v_mfma_f64_16x16x4_f64 (rows i, columns a, contraction
-// over j four at a time).  X_0 is tmp_c; X_1..3 and X_4 give dtmp_c.  Wave (g, c) owns
-// component c and the powers k in {g + 1, C - g - 1}: C accumulator tiles per wave, the
-// same for every wave.  The epilogue of factor_een and factor_een_gl (:14121-14175,
-// :14875-14945) is linear in P and dP, so each wave folds its own sums into per-(i, a)
-// contributions to (J, grad_i J, lap_i J); these are summed over the nuclei across lanes and
-// over the waves in LDS, in a fixed order.  The q = 0 sums come from k_jas_pair (R0); the
-// k = 0 sums S[q] = sum_{j != i} g_a(r_j)^q are the sums over all j less the j = i term.
+// over j four at a time).  X_0 is tmp_c; X_1..3 and X_4 give dtmp_c.  One wave per (k, q):
+// five accumulator tiles (the five components) and five products per step for every wave.
+// The epilogue of factor_een and factor_een_gl (:14121-14175, :14875-14945) is linear in P
+// and dP: every term n = (l, k, m) enters through X[k][m] g_a(r_i)^(m+l) and
+// X[k][m+l] g_a(r_i)^m, so the wave owning (k, q) applies the terms in which q is m or m + l
+// to all five components at once.  The per-(i, a) sums are reduced over the nuclei across
+// lanes and over the waves in LDS, in a fixed order.  The q = 0 sums come from k_jas_pair
+// (R0) and are applied by the waves with q = 1; the k = 0 sums S[q] = sum_{j != i} g_a(r_j)^q
+// (sums over all j from k_jas_pair less the j = i term) by the waves with k = 1.
 // ------------------------------------------------------------------------------------
-constexpr int EM = 16;       // electrons i per workgroup
-constexpr int ECMP = 5;      // components V, Tx, Ty, Tz, L
+constexpr int EM = 16;  // electrons i per workgroup
 constexpr int EMFMA_MAX_C = 6;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// (k, q) of wave w, k-major: (1,1), (1,2), ..., (1,C-1), (2,1), ...
+constexpr int een_wave_k(int C, int w) {
+  for (int k = 1; k < C; ++k) {
+    if (w < C - k) return k;
+    w -= C - k;
+  }
+  return 0;
+}
+constexpr int een_wave_q(int C, int w) {
+  for (int k = 1; k < C; ++k) {
+    if (w < C - k) return w + 1;
+    w -= C - k;
+  }
+  return 0;
+}
 template <int C>
-struct EenGroups {
-  static constexpr int n = (C - 1) / 2 + (C % 2 == 0 ? 1 : 0);  // pairs {k, C-k}, plus k = C/2
-  static constexpr int waves = ECMP * n;
+struct EenWaves {
+  static constexpr int n = C * (C - 1) / 2;
 };
 
-template <int C, int G>
-struct EenGroup {
-  static constexpr int ka = G + 1, kb = C - G - 1;
-  static constexpr bool has_b = kb > ka;
-  static constexpr int na = C - ka;          // X[ka][q], q = 1..na, in acc[q - 1]
-  static constexpr int nb = has_b ? ka : 0;  // X[kb][q], q = 1..nb, in acc[C - q]
-};
-
-// One pass of wave (G, cmp) over nsteps x 4 electrons j held in LDS.  Lane (li, lg) feeds
+// One pass of wave (K, Q) over nsteps x 4 electrons j held in LDS.  Lane (li, lg) feeds
 // A(i = li, j = 4 s + lg) and B(j = 4 s + lg, a = li) of v_mfma_f64_16x16x4_f64.
-template <int C, int G>
-__device__ __forceinline__ void een_mfma_pass(d4 (&acc)[C], const double* pdG, const double* pdR, const double* xj,
-                                              const double* gch, double xi_c, int JS, int nsteps, int cmp, int li,
+template <int C, int K, int Q>
+__device__ __forceinline__ void een_mfma_pass(d4 (&acc)[5], const double* pdG, const double* pdR, const double* xj,
+                                              const double* gch, const double* xi, int JSP, int nsteps, int li,
                                               int lg) {
-  using Gr = EenGroup<C, G>;
-  const double* xjc = xj + (cmp >= 1 && cmp <= 3 ? (cmp - 1) * JS : 0);
   for (int st = 0; st < nsteps; ++st) {
     const int jj = 4 * st + lg;
-    const double Ge = pdG[jj * EM + li];
-    const double ri = pdR[jj * EM + li];
-    // w_c: 1, u_x, u_y, u_z or 1/r (u = (r_i - r_j)/r_ij; 0 for j == i since 1/r is stored 0)
-    const double wc = (cmp == 0) ? 1.0 : (cmp == 4) ? ri : (xi_c - xjc[jj]) * ri;
-    const double g = gch[jj * 16 + li];
-    double gq[C];
-    gq[1] = g;
+    const double Ge = pdG[jj * EM + li], ri = pdR[jj * EM + li], g = gch[jj * 16 + li];
+    const double ux = (xi[0] - xj[jj]) * ri, uy = (xi[1] - xj[JSP + jj]) * ri, uz = (xi[2] - xj[2 * JSP + jj]) * ri;
+    double Gk = Ge, gq = g;
 #pragma unroll
-    for (int q = 2; q < C; ++q) gq[q] = gq[q - 1] * g;
-    double Gk = Ge;
+    for (int k = 2; k <= K; ++k) Gk *= Ge;
 #pragma unroll
-    for (int k = 2; k <= Gr::ka; ++k) Gk *= Ge;
-    const double Aa = Gk * wc;
-#pragma unroll
-    for (int q = 1; q <= Gr::na; ++q) acc[q - 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aa, gq[q], acc[q - 1], 0, 0, 0);
-    if (Gr::has_b) {
-#pragma unroll
-      for (int k = Gr::ka + 1; k <= Gr::kb; ++k) Gk *= Ge;
-      const double Ab = Gk * wc;
-#pragma unroll
-      for (int q = 1; q <= Gr::nb; ++q) acc[C - q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ab, gq[q], acc[C - q], 0, 0, 0);
-    }
+    for (int q = 2; q <= Q; ++q) gq *= g;
+    // w = (1, u_x, u_y, u_z, 1/r); u and 1/r are 0 for j == i (1/r is stored 0 there)
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk, gq, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ux, gq, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uy, gq, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uz, gq, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ri, gq, acc[4], 0, 0, 0);
   }
 }
 
-// Contributions of component CMP, powers {ka, kb} of group G (and k = 0 for the first
-// wave), of the (l, k, m) terms to (J, gl_x, gl_y, gl_z, gl_lap) of one (i, a) pair.
-// Xa[q] = X[ka][q], Xb[q] = X[kb][q], X0[q] = X[0][q]; gp[p] = g_a(r_i)^p.  With
-// h = -kappa_a (r_i - R_a)/R the derivatives of g^p are d_c g^p = p h_c g^p (c < 3) and
-// lap g^p = (p^2 kappa_a^2 - 2 p kappa_a / R) g^p, so every gradient component is h_c times
-// one scalar sum over the terms, and the Laplacian is a combination of three such sums.
-template <int C, int G, int CMP>
-__device__ __forceinline__ void een_fold(const double* Xa, const double* Xb, const double* X0, const double* gp,
-                                         const double* h, double ka2, double kr, const double* cf, double kee,
-                                         double* out) {
-  using Gr = EenGroup<C, G>;
+// Partial sums of one (i, a) pair, assembled into (J, grad, lap) at the end of the row:
+//   J = sJ;  grad_c = h_c s1 - t0[c];  lap = kappa^2 s2 - (2 kappa / R) s1 + s3 - 2 h.t1 - 2 sL
+// with h = -kappa_a (r_i - R_a)/R; d_c g^p = p h_c g^p, lap g^p = (p^2 kappa^2 - 2 p kappa/R) g^p.
+struct EenRow {
+  double sJ = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, sL = 0.0;
+  double t0[3] = {0.0, 0.0, 0.0}, t1[3] = {0.0, 0.0, 0.0};
+};
+
+// One role of term n: the value X[k][px] (5 components) times g^PG; JROLE when px = m + l
+// (the role that carries the value J).  kk = k kappa_ee.
+template <int PG, bool JROLE, bool KZERO>
+__device__ __forceinline__ void een_role(EenRow& R, double cn, const double* gp, const double* X, double kk) {
+  const double t = cn * gp[PG];
+  const double tv = t * X[0];
+  if (JROLE) R.sJ += tv;
+  R.s1 += (double)PG * tv;
+  R.s2 += (double)(PG * PG) * tv;
+  if (!KZERO) {
+    R.s3 += kk * kk * tv;
+    const double tk = kk * t;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R.t0[c] += tk * X[1 + c];
+      R.t1[c] += (double)PG * tk * X[1 + c];
+    }
+    R.sL += tk * X[4];
+  }
+}
+
+// Applies every term of wave (K, Q): X = X[K][Q] (from the products), XR = X[K][0] (Q = 1),
+// X0 = X[0][Q] value only (K = 1).  Terms are visited at compile time.
+template <int C, int K, int Q, int N = 0>
+__device__ __forceinline__ void een_fold(EenRow& R, const double* cf, const double* gp, const double* X,
+                                         const double* XR, const double* X0, double kee) {
   constexpr Lkm<C> T{};
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-  for (int n = 0; n < T.dim; ++n) {
-    const int l = T.l[n], k = T.k[n], m = T.m[n];
-    const double* X;
-    if (k == Gr::ka)
-      X = Xa;
-    else if (Gr::has_b && k == Gr::kb)
-      X = Xb;
-    else if (k == 0 && CMP == 0 && G == 0)
-      X = X0;
-    else
-      continue;
-    const double cn = cf[16 * n];  // c-coefficient n of this nucleus
-    const double am = X[m] * gp[m + l], aml = X[m + l] * gp[m];  // P[k][m] g^{m+l}, P[k][m+l] g^m
-    const double kk = (double)k * kee;
-    if (CMP == 0) {
-      s0 += cn * aml;                                                           // J
-      s1 += cn * ((double)(m + l) * am + (double)m * aml);                      // grad (x h_c)
-      s2 += cn * ((double)((m + l) * (m + l)) * am + (double)(m * m) * aml);  // lap, kappa^2 part
-      if (k != 0) s3 += cn * kk * kk * (am + aml);                              // lap of g_e^k
-    } else if (CMP <= 3) {
-      s0 += cn * kk * (am + aml);                                          // -grad_c
-      s1 += cn * kk * ((double)(m + l) * am + (double)m * aml);            // cross term (x -2 h_c)
-    } else {
-      s0 += cn * kk * (am + aml);                                          // lap (x -2)
+  if constexpr (N < T.dim) {
+    constexpr int l = T.l[N], k = T.k[N], m = T.m[N];
+    constexpr bool rm = (k == K && m == Q), rml = (k == K && m + l == Q);
+    constexpr bool rm0 = (Q == 1 && k == K && m == 0), rml0 = (Q == 1 && k == K && m + l == 0);
+    constexpr bool zm = (K == 1 && k == 0 && m == Q), zml = (K == 1 && k == 0 && m + l == Q);
+    if constexpr (rm || rml || rm0 || rml0 || zm || zml) {
+      const double cn = cf[16 * N];  // c-coefficient N of this nucleus
+      const double kk = (double)K * kee;
+      if constexpr (rm) een_role<m + l, false, false>(R, cn, gp, X, kk);
+      if constexpr (rml) een_role<m, true, false>(R, cn, gp, X, kk);
+      if constexpr (rm0) een_role<m + l, false, false>(R, cn, gp, XR, kk);
+      if constexpr (rml0) een_role<m, true, false>(R, cn, gp, XR, kk);
+      if constexpr (zm) een_role<m + l, false, true>(R, cn, gp, X0, 0.0);
+      if constexpr (zml) een_role<m, true, true>(R, cn, gp, X0, 0.0);
     }
-  }
-  if (CMP == 0) {
-    out[0] += s0;
-    out[1] += h[0] * s1;
-    out[2] += h[1] * s1;
-    out[3] += h[2] * s1;
-    out[4] += ka2 * s2 - kr * s1 + s3;
-  } else if (CMP <= 3) {
-    out[CMP] -= s0;
-    out[4] -= 2.0 * h[CMP - 1] * s1;
-  } else {
-    out[4] -= 2.0 * s0;
+    een_fold<C, K, Q, N + 1>(R, cf, gp, X, XR, X0, kee);
   }
 }
 
-template <int C, int G>
-__device__ __forceinline__ void een_epilogue(const d4 (&acc)[C], int cmp, int r, const double* rr,
-                                             const double* Sl_a, const double* gp, const double* h, double ka2,
-                                             double kr, const double* cf, double kee, double* out) {
-  using Gr = EenGroup<C, G>;
-  double Xa[C], Xb[C], X0[C];
-  Xa[0] = rr[(Gr::ka - 1) * 5 + cmp];
-#pragma unroll
-  for (int q = 1; q <= Gr::na; ++q) {
-    const d4 x = acc[q - 1];
-    Xa[q] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
-  }
-  if (Gr::has_b) {
-    Xb[0] = rr[(Gr::kb - 1) * 5 + cmp];
-#pragma unroll
-    for (int q = 1; q <= Gr::nb; ++q) {
-      const d4 x = acc[C - q];
-      Xb[q] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
+// Wave-uniform dispatch to the (k, q) specialisations.
+template <int C, int W = 0>
+__device__ __forceinline__ void een_dispatch_pass(int wave, d4 (&acc)[5], const double* pdG, const double* pdR,
+                                                  const double* xj, const double* gch, const double* xi, int JSP,
+                                                  int nsteps, int li, int lg) {
+  if constexpr (W < EenWaves<C>::n) {
+    if (wave == W) {
+      een_mfma_pass<C, een_wave_k(C, W), een_wave_q(C, W)>(acc, pdG, pdR, xj, gch, xi, JSP, nsteps, li, lg);
+      return;
     }
+    een_dispatch_pass<C, W + 1>(wave, acc, pdG, pdR, xj, gch, xi, JSP, nsteps, li, lg);
   }
-  X0[0] = 0.0;
-  if (G == 0 && cmp == 0)
+}
+
+template <int C, int W = 0>
+__device__ __forceinline__ void een_dispatch_fold(int wave, EenRow& R, const double* cf, const double* gp,
+                                                  const double* X, const double* r0row, const double* srow,
+                                                  double kee) {
+  if constexpr (W < EenWaves<C>::n) {
+    if (wave == W) {
+      constexpr int K = een_wave_k(C, W), Q = een_wave_q(C, W);
+      double XR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, X0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (Q == 1)
 #pragma unroll
-    for (int q = 1; q < C; ++q) X0[q] = Sl_a[q - 1] - gp[q];
-  switch (cmp) {
-    case 0: een_fold<C, G, 0>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
-    case 1: een_fold<C, G, 1>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
-    case 2: een_fold<C, G, 2>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
-    case 3: een_fold<C, G, 3>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
-    default: een_fold<C, G, 4>(Xa, Xb, X0, gp, h, ka2, kr, cf, kee, out); break;
+        for (int c = 0; c < 5; ++c) XR[c] = r0row[(K - 1) * 5 + c];  // X[K][0]
+      if (K == 1) X0[0] = srow[Q - 1] - gp[Q];                       // X[0][Q], value only
+      een_fold<C, K, Q>(R, cf, gp, X, XR, X0, kee);
+      return;
+    }
+    een_dispatch_fold<C, W + 1>(wave, R, cf, gp, X, r0row, srow, kee);
   }
 }
 
 // Dynamic LDS of k_jas_een_mfma, in doubles, for a j range of JS electrons per pass.
 struct EenMfmaLds {
-  int JS, C, dim, NW;
-  int xt, xj, pdG, pdR, gch, ep, r0t, st, cfl, red, jrow, total;
-  __host__ __device__ EenMfmaLds(int JS_, int C_, int dim_, int NW_) : JS(JS_), C(C_), dim(dim_), NW(NW_) {
+  int JS, JSP, C, dim, NW;
+  int xt, xj, pdG, pdR, gch, ep, r0t, st, cfl, red, tot, jrow, total;
+  __host__ __device__ EenMfmaLds(int JS_, int C_, int dim_, int NW_) : JS(JS_), JSP(JS_ + 4), C(C_), dim(dim_), NW(NW_) {
     int o = 0;
     xt = o;   o += EM * 3;               // tile electrons
-    xj = o;   o += 3 * JS;               // electrons j of the pass
-    pdG = o;  o += JS * EM;              // g_e(r_ij) = exp(-kappa r_ij), [j][i]; 0 for j == i
-    pdR = o;  o += JS * EM;              // 1 / r_ij, [j][i]; 0 for j == i
-    gch = o;  o += JS * 16;              // g_a(r_j) of the nucleus tile, [j][a]
+    xj = o;   o += 3 * JSP;              // electrons j of the pass, [3][j]
+    pdG = o;  o += JSP * EM;             // g_e(r_ij) = exp(-kappa r_ij), [j][i]; 0 for j == i
+    pdR = o;  o += JSP * EM;             // 1 / r_ij, [j][i]; 0 for j == i
+    gch = o;  o += JSP * 16;             // g_a(r_j) of the nucleus tile, [j][a]
     ep = o;   o += (C + 5) * EM * 16;    // per (i, a): g^1..g^C, h_x, h_y, h_z, 2 kappa/R, kappa^2
     r0t = o;  o += EM * (C - 1) * 5;     // q = 0 sums of the tile electrons
     st = o;   o += 16 * (C - 1);         // sum over all j of g_a(r_j)^q
-    cfl = o;  o += dim * 16;             // c-coefficients of the nucleus tile
+    cfl = o;  o += dim * 16;             // c-coefficients of the nucleus tile, [n][a]
     red = o;  o += NW * EM * 5;
+    tot = o;  o += EM * 5;               // running (J, gl) per row, summed over nuclei
     jrow = o; o += EM;
     total = o;
   }
 };
 
 template <int C>
-__global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDevView v, JasOut o,
-                                                                          const double* __restrict__ R0,
-                                                                          const double* __restrict__ gN,
-                                                                          const double* __restrict__ Sg,
-                                                                          double* __restrict__ Jt, int ntiles, int JS) {
+__global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView v, JasOut o,
+                                                                     const double* __restrict__ R0,
+                                                                     const double* __restrict__ gN,
+                                                                     const double* __restrict__ Sg,
+                                                                     double* __restrict__ Jt, int ntiles, int JS) {
   constexpr int K1 = C - 1;
-  constexpr int NW = EenGroups<C>::waves;
+  constexpr int NW = EenWaves<C>::n;
   constexpr int NT = 64 * NW;
   constexpr int DIM = Lkm<C>::dim;
   extern __shared__ double lds[];
@@ -658,13 +655,14 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
   double* st = lds + L.st;
   double* cfl = lds + L.cfl;
   double* red = lds + L.red;
+  double* tot = lds + L.tot;
   double* jrow = lds + L.jrow;
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
   const int64_t w = blockIdx.x / ntiles;
   const int tile = (int)(blockIdx.x - w * ntiles);
   const int i0 = tile * EM;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
-  const int grp = wave / ECMP, cmp = wave - ECMP * grp;
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (k, q) role
   const double kee = v.rescale_ee;
   const double* coord = v.coord;
   const int64_t cld = v.coord_ld;
@@ -677,11 +675,10 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
     const int il = t / (K1 * 5);
     r0t[t] = (i0 + il < Ne) ? R0[(w * Ne + i0) * K1 * 5 + t] : 0.0;
   }
+  for (int t = tid; t < EM * 5; t += NT) tot[t] = 0.0;
   __syncthreads();
-  // this lane's component of r_i - r_j needs x_i of row li
-  const double xi_c = (cmp >= 1 && cmp <= 3) ? xt[li * 3 + cmp - 1] : 0.0;
+  const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
 
-  double total = 0.0;  // thread t < 5 EM: (J, gl) output t % 5 of row t / 5, summed over nuclei
   for (int a0 = 0; a0 < Nn; a0 += 16) {
     // per-tile constants: c-coefficients, the k = 0 sums, and per (i, a) the powers of
     // g_a(r_i) and the factors of their derivatives
@@ -723,9 +720,9 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
       ep[(C + 4) * EM * 16 + t] = ka2;
     }
 
-    d4 acc[C];
+    d4 acc[5];
 #pragma unroll
-    for (int q = 0; q < C; ++q) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < 5; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
 
     for (int pass = 0; pass < npass; ++pass) {
       const int j0 = pass * JS;
@@ -735,7 +732,7 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
         if (npass > 1) __syncthreads();  // previous pass done with the pair data
         for (int t = tid; t < 3 * jn4; t += NT) {
           const int d = t / jn4, jj = t - d * jn4;
-          xj[d * JS + jj] = (jj < jn) ? coord[w * Ne + j0 + jj + d * cld] : 0.0;
+          xj[d * L.JSP + jj] = (jj < jn) ? coord[w * Ne + j0 + jj + d * cld] : 0.0;
         }
         for (int t = tid; t < EM * jn4; t += NT) {
           const int il = t & (EM - 1), jj = t / EM;
@@ -759,16 +756,7 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
         gch[jj * 16 + al] = (a < Nn && jj < jn) ? gN[(w * Nn + a) * Ne + j0 + jj] : 0.0;
       }
       __syncthreads();
-      const int nsteps = jn4 >> 2;
-      switch (grp) {
-        case 0: een_mfma_pass<C, 0>(acc, pdG, pdR, xj, gch, xi_c, JS, nsteps, cmp, li, lg); break;
-        case 1:
-          if constexpr (EenGroups<C>::n > 1) een_mfma_pass<C, 1>(acc, pdG, pdR, xj, gch, xi_c, JS, nsteps, cmp, li, lg);
-          break;
-        default:
-          if constexpr (EenGroups<C>::n > 2) een_mfma_pass<C, 2>(acc, pdG, pdR, xj, gch, xi_c, JS, nsteps, cmp, li, lg);
-          break;
-      }
+      een_dispatch_pass<C>(wave, acc, pdG, pdR, xj, gch, xi, L.JSP, jn4 >> 2, li, lg);
       if (pass + 1 < npass) __syncthreads();  // before gch of the next pass is written
     }
 
@@ -780,25 +768,26 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
       double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (va && i < Ne) {
         const int pidx = il * 16 + li;
-        double gp[C + 1], h[3];
+        double gp[C + 1];
         gp[0] = 1.0;
 #pragma unroll
         for (int p = 1; p <= C; ++p) gp[p] = ep[(p - 1) * EM * 16 + pidx];
-        h[0] = ep[(C + 0) * EM * 16 + pidx];
-        h[1] = ep[(C + 1) * EM * 16 + pidx];
-        h[2] = ep[(C + 2) * EM * 16 + pidx];
-        const double kr = ep[(C + 3) * EM * 16 + pidx], ka2 = ep[(C + 4) * EM * 16 + pidx];
-        const double* rr = r0t + il * K1 * 5;
-        const double* cf = cfl + li;
-        switch (grp) {
-          case 0: een_epilogue<C, 0>(acc, cmp, r, rr, st + li * K1, gp, h, ka2, kr, cf, kee, out); break;
-          case 1:
-            if constexpr (EenGroups<C>::n > 1) een_epilogue<C, 1>(acc, cmp, r, rr, st + li * K1, gp, h, ka2, kr, cf, kee, out);
-            break;
-          default:
-            if constexpr (EenGroups<C>::n > 2) een_epilogue<C, 2>(acc, cmp, r, rr, st + li * K1, gp, h, ka2, kr, cf, kee, out);
-            break;
+        double X[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          const d4 x = acc[c];
+          X[c] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
         }
+        EenRow R;
+        een_dispatch_fold<C>(wave, R, cfl + li, gp, X, r0t + il * K1 * 5, st + li * K1, kee);
+        const double h0 = ep[(C + 0) * EM * 16 + pidx], h1 = ep[(C + 1) * EM * 16 + pidx],
+                     h2 = ep[(C + 2) * EM * 16 + pidx];
+        const double kr = ep[(C + 3) * EM * 16 + pidx], ka2 = ep[(C + 4) * EM * 16 + pidx];
+        out[0] = R.sJ;
+        out[1] = h0 * R.s1 - R.t0[0];
+        out[2] = h1 * R.s1 - R.t0[1];
+        out[3] = h2 * R.s1 - R.t0[2];
+        out[4] = ka2 * R.s2 - kr * R.s1 + R.s3 - 2.0 * (h0 * R.t1[0] + h1 * R.t1[1] + h2 * R.t1[2]) - 2.0 * R.sL;
       }
       // sum over the 16 nuclei of the tile: lanes li = 0..15 of this row group
 #pragma unroll
@@ -815,19 +804,21 @@ __global__ void __launch_bounds__(64 * EenGroups<C>::waves) k_jas_een_mfma(JasDe
         for (int c = 0; c < 5; ++c) red[(wave * EM + il) * 5 + c] = out[c];
     }
     __syncthreads();
-    if (tid < 5 * EM) {
+    for (int t = tid; t < EM * 5; t += NT) {
+      double x = tot[t];
 #pragma unroll
-      for (int wv = 0; wv < NW; ++wv) total += red[wv * EM * 5 + tid];
+      for (int wv = 0; wv < NW; ++wv) x += red[wv * EM * 5 + t];
+      tot[t] = x;
     }
     // the next tile rewrites cfl, st, ep and gch only after this barrier
     __syncthreads();
   }
-  if (tid < 5 * EM) {
-    const int il = tid / 5, c = tid - 5 * il, i = i0 + il;
+  for (int t = tid; t < EM * 5; t += NT) {
+    const int il = t / 5, c = t - 5 * il, i = i0 + il;
     if (c == 0)
-      jrow[il] = (i < Ne) ? total : 0.0;
+      jrow[il] = (i < Ne) ? tot[t] : 0.0;
     else if (i < Ne)
-      o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = total;
+      o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = tot[t];
   }
   __syncthreads();
   if (tid == 0) {
@@ -887,8 +878,7 @@ EenShape een_shape(const JasDevView& v) {
     s.TI = EM;
     s.JC = std::min((Ne + 3) & ~3, 256);
     s.ntiles = (Ne + EM - 1) / EM;
-    const int ng = (C - 1) / 2 + (C % 2 == 0 ? 1 : 0);
-    s.lds = sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, ECMP * ng).total;
+    s.lds = sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, C * (C - 1) / 2).total;
     return s;
   }
   s.TI = kBlock / Nn;
@@ -1144,7 +1134,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       if (ea != hipSuccess) return ea;                                                                         \
       attr_set = true;                                                                                         \
     }                                                                                                          \
-    hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenGroups<CC>::waves), sh.lds, s, v, o, sv.R0, sv.gN, \
+    hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,      \
                        sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                         \
     break;                                                                                                     \
   }
