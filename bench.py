@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--mode", default="bf16x3", choices=["bf16x3", "fp32", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-mb", type=float, default=0.0,
+                    help="copy this many MiB device-to-device once before timing (a copy kernel of "
+                         "known size, the scale for HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: walkers are independent and this build measures one GPU's "
@@ -82,6 +85,14 @@ def main():
     mo = q.DeviceBuffer(8 * n_mo)
     val = q.DeviceBuffer(8 * W)
     gl = q.DeviceBuffer(8 * n_gl)
+    if args.copy_mb > 0:
+        nb = int(args.copy_mb * 2**20)
+        src, dst = q.DeviceBuffer(nb), q.DeviceBuffer(nb)
+        dst.copy_from(src)
+        q.device_synchronize()
+        log(f"copied {nb} bytes device-to-device")
+        src.free()
+        dst.free()
     q.device_synchronize()
 
     def step():
@@ -156,7 +167,7 @@ def main():
         roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm_GBs"], "unit": "GB/s"}
     roof["kernel"] = dom
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(args.workload, W, args.mode, dom)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -190,6 +201,22 @@ def main():
     }
     q.context_destroy(ctx)
     print(json.dumps(out))
+
+
+def measured_traffic(workload, W, mode, kernel):
+    """HBM bytes per launch of `kernel`, from the committed FETCH_SIZE/WRITE_SIZE counter
+    summary (profiles/hbm_traffic.json, each counter scaled by its reading for a copy of
+    known size), when it covers this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("walkers") != W:
+        return None
+    k = t.get("kernels", {}).get(f"{kernel}:{mode}") or t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
 
 
 def cpu_baseline(s, x, W, seconds):

@@ -234,6 +234,14 @@ class DeviceBuffer:
             raise RuntimeError(f"hipMemcpy D2H failed with {rc}")
         return out
 
+    def copy_from(self, other, nbytes=None):
+        """Device-to-device copy of `nbytes` (default: all of `other`)."""
+        n = other.nbytes if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= other.nbytes
+        rc = hip().hipMemcpy(self.ptr, other.ptr, n, self.D2D)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy D2D failed with {rc}")
+
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)

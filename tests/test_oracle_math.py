@@ -1,0 +1,201 @@
+"""Pins the oracle beyond the reference's golden values, with checks that do not restate the
+oracle's own formulas:
+
+* AO values against an independent numpy closed form (contracted Gaussian times the cartesian
+  monomial, in the reference's polynomial order), and AO gradients/Laplacians against
+  central finite differences of the oracle's own values;
+* MO VGL against a dense numpy product;
+* distances against numpy norms;
+* the e-e-n Jastrow value against a brute-force triple sum over (i, j, nucleus) of the
+  defining CHAMP expression, and every Jastrow gradient/Laplacian (e-e, e-n, e-e-n, total)
+  against central finite differences of the corresponding value;
+* value == exp(J) (0 below J = -100) and gl == e^J (grad J, lap J + |grad J|^2).
+
+Each check is chosen so that a plausible slip in the oracle (a wrong polynomial order or
+power, a dropped normalisation factor, a sign or index error in a derivative, a missing
+e-e-n term) fails it.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from qmckl_amd import system as S
+
+
+# ------------------------------------------------------------------------ AO closed form
+def _ao_closed_form(s, x):
+    """AO values [P][ao] from the definition: ao_factor * x^a y^b z^c * shell_factor *
+    sum_p coefficient_p prim_factor_p exp(-alpha_p r^2), cartesian order a = l..0,
+    b = l-a..0 for each shell (org/qmckl_ao.org, polynomial order of :6270-6300)."""
+    P = x.shape[1]
+    out = np.zeros((P, s.ao_num))
+    k = 0
+    for a in range(s.nucl_num):
+        R = s.nucl_coord[:, a][:, None]
+        d = x - R
+        r2 = (d * d).sum(0)
+        for sh in range(s.nucleus_index[a], s.nucleus_index[a] + s.nucleus_shell_num[a]):
+            l = int(s.shell_ang_mom[sh])
+            p0, np_ = int(s.shell_prim_index[sh]), int(s.shell_prim_num[sh])
+            rad = np.zeros(P)
+            for p in range(p0, p0 + np_):
+                rad += s.coefficient[p] * s.prim_factor[p] * np.exp(-s.exponent[p] * r2)
+            rad *= s.shell_factor[sh]
+            for pa in range(l, -1, -1):
+                for pb in range(l - pa, -1, -1):
+                    pc = l - pa - pb
+                    out[:, k] = s.ao_factor[k] * d[0] ** pa * d[1] ** pb * d[2] ** pc * rad
+                    k += 1
+    assert k == s.ao_num
+    return out
+
+
+def _points(s, n, seed):
+    rng = np.random.default_rng(seed)
+    c = s.nucl_coord[:, rng.integers(0, s.nucl_num, n)]
+    return np.ascontiguousarray(c + rng.normal(0.0, 0.8, (3, n)))
+
+
+@pytest.mark.parametrize("name", ["small", "medium"])
+def test_ao_values_match_closed_form(name):
+    s, _, _ = S.workload(name, walk_num=1)
+    s.precision = 53
+    x = _points(s, 24, 1)
+    got = oracle.ao_vgl(s, x)[:, 0, :]
+    ref = _ao_closed_form(s, x)
+    # the oracle drops primitives with alpha r^2 > -log(2^-52): below 1e-15 of each shell
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * scale)
+
+
+def test_ao_gradient_and_laplacian_match_finite_differences():
+    s, _, _ = S.workload("small", walk_num=1)
+    x = _points(s, 12, 2)
+    vgl = oracle.ao_vgl(s, x)
+    h = 1e-4
+    for d in range(3):
+        e = np.zeros((3, 1))
+        e[d] = h
+        fp = oracle.ao_vgl(s, x + e)[:, 0, :]
+        fm = oracle.ao_vgl(s, x - e)[:, 0, :]
+        fd = (fp - fm) / (2 * h)
+        scale = np.abs(vgl[:, 1 + d, :]).max()
+        assert np.abs(fd - vgl[:, 1 + d, :]).max() <= 1e-6 * scale, f"d/d{'xyz'[d]}"
+    h = 2e-3
+    lap = -6.0 * vgl[:, 0, :]
+    for d in range(3):
+        e = np.zeros((3, 1))
+        e[d] = h
+        lap = lap + oracle.ao_vgl(s, x + e)[:, 0, :] + oracle.ao_vgl(s, x - e)[:, 0, :]
+    lap /= h * h
+    scale = np.abs(vgl[:, 4, :]).max()
+    assert np.abs(lap - vgl[:, 4, :]).max() <= 2e-4 * scale
+
+
+def test_mo_vgl_is_the_dense_product():
+    s, x, W = S.workload("small", walk_num=4)
+    ao = oracle.ao_vgl(s, x)
+    coef = oracle.kept_mo_coef(s)
+    got = oracle.mo_vgl(coef, ao)
+    ref = np.einsum("pck,jk->pcj", ao, coef)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13 * np.abs(ref).max())
+
+
+def test_distances_are_norms():
+    s, x, W = S.workload("small", walk_num=3)
+    Ne = s.elec_num
+    ee = oracle.ee_distance(x, Ne, W)
+    en = oracle.en_distance(x, s.nucl_coord)
+    xw = x.reshape(3, W, Ne)
+    ref_ee = np.linalg.norm(xw[:, :, :, None] - xw[:, :, None, :], axis=0)  # [W][j][i]
+    np.testing.assert_allclose(ee, ref_ee.transpose(0, 2, 1), rtol=1e-14, atol=1e-14)
+    ref_en = np.linalg.norm(x[:, :, None] - s.nucl_coord[:, None, :], axis=0)
+    np.testing.assert_allclose(en, ref_en, rtol=1e-14)
+
+
+# ------------------------------------------------------------------------ Jastrow
+def _een_brute_force(s, x, W):
+    """J_eeN from its defining sum: over nuclei a, ordered pairs i != j and the terms
+    (l, k, m) of dimension dim_c, c_{n,type(a)} * g_e(r_ij)^k * g_a(r_j)^m * g_a(r_i)^(m+l),
+    with g_e = exp(-kappa_ee r), g_a = exp(-kappa_a R) (org/qmckl_jastrow_champ.org,
+    factor_een_naive :14019)."""
+    Ne, Nn, C = s.elec_num, s.nucl_num, s.cord
+    terms = []
+    for p in range(2, C + 1):
+        for k in range(p - 1, -1, -1):
+            lmax = p - k if k != 0 else p - k - 2
+            for l in range(lmax, -1, -1):
+                if (p - k - l) % 2 == 0:
+                    terms.append((l, k, (p - k - l) // 2))
+    xw = x.reshape(3, W, Ne)
+    J = np.zeros(W)
+    kap = s.rescale_en[s.type_nucl_vector]
+    for w in range(W):
+        r = np.linalg.norm(xw[:, w, :, None] - xw[:, w, None, :], axis=0)
+        G = np.exp(-s.rescale_ee * r)
+        off = ~np.eye(Ne, dtype=bool)
+        R = np.linalg.norm(xw[:, w, :, None] - s.nucl_coord[:, None, :], axis=0)  # [i][a]
+        g = np.exp(-kap[None, :] * R)
+        for n, (l, k, m) in enumerate(terms):
+            Gk = np.where(off, G ** k, 0.0)
+            c = s.c_vector[s.type_nucl_vector, n]  # [a]
+            # sum_{i,j} Gk[i,j] g[j,a]^m g[i,a]^(m+l)
+            J[w] += np.einsum("ij,ja,ia,a->", Gk, g ** m, g ** (m + l), c)
+    return J
+
+
+@pytest.mark.parametrize("cfg", [(3, 3, 2, 5, 5, 5), (4, 4, 3, 3, 4, 3), (2, 3, 3, 5, 5, 7)])
+def test_een_value_matches_brute_force(cfg):
+    s, x, W = S.jastrow_workload(*cfg, walk_num=3)
+    got = oracle.jastrow(s, x, W, want=("factor_een",))["factor_een"]
+    ref = _een_brute_force(s, x, W)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12)
+
+
+def _fd_gl(s, x, W, key):
+    """Central-difference gradient and Laplacian of the per-walker value `key` w.r.t. each
+    electron, [W][4][Ne]."""
+    Ne = s.elec_num
+    f0 = oracle.jastrow(s, x, W, want=(key,))[key]
+    out = np.zeros((W, 4, Ne))
+    h1, h2 = 1e-5, 1e-3
+    xw = x.reshape(3, W, Ne)
+    for i in range(Ne):
+        lap = -6.0 * f0
+        for d in range(3):
+            vals = {}
+            for h, sgn in ((h1, 1), (h1, -1), (h2, 1), (h2, -1)):
+                y = xw.copy()
+                y[d, :, i] += sgn * h
+                vals[(h, sgn)] = oracle.jastrow(s, np.ascontiguousarray(y.reshape(3, W * Ne)), W, want=(key,))[key]
+            out[:, d, i] = (vals[(h1, 1)] - vals[(h1, -1)]) / (2 * h1)
+            lap = lap + vals[(h2, 1)] + vals[(h2, -1)]
+        out[:, 3, i] = lap / (h2 * h2)
+    return out
+
+
+@pytest.mark.parametrize("key", ["factor_ee", "factor_en", "factor_een"])
+def test_jastrow_gl_matches_finite_differences(key):
+    s, x, W = S.jastrow_workload(3, 3, 2, 5, 5, 5, walk_num=2)
+    got = oracle.jastrow(s, x, W, want=(key + "_gl",))[key + "_gl"]
+    fd = _fd_gl(s, x, W, key)
+    for c, tol in ((0, 1e-6), (1, 1e-6), (2, 1e-6), (3, 1e-4)):
+        scale = max(np.abs(got[:, c]).max(), 1e-3)
+        assert np.abs(fd[:, c] - got[:, c]).max() <= tol * scale, f"{key} component {c}"
+
+
+def test_jastrow_value_and_gl_identities():
+    s, x, W = S.jastrow_workload(3, 3, 2, 5, 5, 5, walk_num=4)
+    r = oracle.jastrow(s, x, W)
+    J = r["factor_ee"] + r["factor_en"] + r["factor_een"]
+    np.testing.assert_allclose(r["value"], np.where(J < -100.0, 0.0, np.exp(J)), rtol=1e-14)
+    g = r["factor_ee_gl"] + r["factor_en_gl"] + r["factor_een_gl"]
+    ref = g.copy()
+    ref[:, 3] = g[:, 3] + (g[:, :3] ** 2).sum(1)
+    np.testing.assert_allclose(r["gl"], ref * r["value"][:, None, None], rtol=1e-13, atol=1e-300)
+    # and the total GL is the finite-difference GL of e^J
+    fd = _fd_gl(s, x, W, "value")
+    for c, tol in ((0, 1e-6), (1, 1e-6), (2, 1e-6), (3, 1e-4)):
+        scale = np.abs(r["gl"][:, c]).max()
+        assert np.abs(fd[:, c] - r["gl"][:, c]).max() <= tol * scale, f"gl component {c}"
