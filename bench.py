@@ -169,6 +169,29 @@ def main():
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = measured_traffic(args.workload, W, args.mode, dom)
 
+    # The drop-in boundary with host buffers: the same step through the reference's
+    # getters (coordinates from and results to host memory, over PCIe).  Reported beside
+    # `value`, never as it.
+    q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, modes[args.mode]))
+    host_mo = np.empty(n_mo)
+    host_val = np.empty(W)
+    host_gl = np.empty(n_gl)
+
+    def host_step():
+        q.check(ctx, L.qmckl_set_electron_coord(ctx, b"T", W, q.ptr(x), x.size))
+        q.check(ctx, L.qmckl_get_mo_basis_mo_vgl(ctx, q.ptr(host_mo), n_mo))
+        q.check(ctx, L.qmckl_get_jastrow_champ_value(ctx, q.ptr(host_val), W))
+        q.check(ctx, L.qmckl_get_jastrow_champ_gl(ctx, q.ptr(host_gl), n_gl))
+
+    host_step()
+    t0 = time.perf_counter()
+    nh = max(1, min(args.steps, 3))
+    for _ in range(nh):
+        host_step()
+    th = (time.perf_counter() - t0) / nh
+    log(f"host-buffer boundary ({args.mode}): {th * 1e3:.3f} ms/step")
+    del host_mo
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(s, x, W, args.cpu_seconds)
@@ -196,6 +219,8 @@ def main():
         "split_path": {"kernels": ksp, "speedup_vs_fp64": t64 / tsp},
         "mo_max_rel_err_vs_fp64": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
         "jastrow_max_abs_dJ_vs_fp64": dJ,
+        "host_buffer_path": {"value": W / th, "ms_per_step": th * 1e3,
+                             "note": "same step through the host-pointer getters (PCIe-inclusive)"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
