@@ -201,3 +201,84 @@ def asymp_jasb(s):
     b = _c(s.b_vector)
     lib().oracle_asymp_jasb(s.bord, _p(b), float(s.rescale_ee), int(s.spin_independent), _p(out))
     return out
+
+
+# ----------------------------------------------------------------------------------------
+# Parameter derivatives and nuclear forces of the Jastrow (qmckl_oracle_deriv.c)
+# ----------------------------------------------------------------------------------------
+DERIV_FIELDS = ["asymp_jasb_pderiv", "asymp_jasa_pderiv", "factor_ee_pderiv", "factor_ee_gl_pderiv",
+                "factor_en_pderiv", "factor_en_gl_pderiv", "factor_een_pderiv", "factor_een_gl_pderiv",
+                "forces_jastrow_en", "forces_jastrow_en_g", "forces_jastrow_en_l", "forces_tmp_c",
+                "forces_dtmp_c", "forces_een_n", "forces_jastrow_een", "forces_jastrow_een_g",
+                "forces_jastrow_een_l"]
+
+
+def c_full(s):
+    """c_vector_full [dim_c][Nn]: the c-coefficients of each nucleus's type."""
+    d = dim_c_vector(s.cord)
+    return np.ascontiguousarray(np.asarray(s.c_vector)[np.asarray(s.type_nucl_vector), :d].T)
+
+
+def jastrow_derivatives(s, coord_T, walk_num):
+    """Parameter derivatives and nuclear forces of the Jastrow, in the reference's C layouts
+    (see qmckl_oracle_deriv.c); returns a dict keyed by DERIV_FIELDS."""
+    L = lib()
+    W, Ne, Nn, C, T = walk_num, s.elec_num, s.nucl_num, s.cord, s.type_nucl_num
+    I = jastrow(s, coord_T, W, want=("ee_rescaled", "ee_rescaled_gl", "en_rescaled", "en_rescaled_gl", "een_e",
+                                      "een_e_gl", "een_n", "een_n_gl", "tmp_c", "dtmp_c"))
+    edist = en_distance(coord_T, s.nucl_coord)
+    typ = _c(s.type_nucl_vector, np.int64)
+    a, b, kap = _c(s.a_vector), _c(s.b_vector), _c(s.rescale_en)
+    d = dim_c_vector(C)
+    lk = np.ascontiguousarray(lkpm(C)) if C >= 2 else np.zeros((4, 1), np.int64)
+    cf = c_full(s) if C >= 2 else np.zeros((1, Nn))
+    o = {
+        "asymp_jasb_pderiv": np.zeros((2, s.bord + 1)), "asymp_jasa_pderiv": np.zeros((T, s.aord + 1)),
+        "factor_ee_pderiv": np.zeros(s.bord + 1), "factor_ee_gl_pderiv": np.zeros((s.bord + 1, Ne, 4)),
+        "factor_en_pderiv": np.zeros((T, s.aord + 1)), "factor_en_gl_pderiv": np.zeros((T, s.aord + 1, Ne, 4)),
+        "factor_een_pderiv": np.zeros((T, max(d, 1))), "factor_een_gl_pderiv": np.zeros((T, max(d, 1), Ne, 4)),
+        "forces_jastrow_en": np.zeros((W, Nn, 3)), "forces_jastrow_en_g": np.zeros((W, Nn, 3, Ne, 3)),
+        "forces_jastrow_en_l": np.zeros((W, Nn, 3)), "forces_tmp_c": np.zeros((W, C, C + 1, Nn, 4, Ne)),
+        "forces_dtmp_c": np.zeros((W, C, C + 1, 4, Nn, 4, Ne)), "forces_een_n": np.zeros((W, C + 1, 3, Nn, 4, Ne)),
+        "forces_jastrow_een": np.zeros((W, Nn, 3)), "forces_jastrow_een_g": np.zeros((W, 3, Nn, 3, Ne)),
+        "forces_jastrow_een_l": np.zeros((W, 3, Nn)),
+    }
+    def P(arr):  # explicit pointer: these functions have no declared argtypes
+        return ctypes.c_void_p(arr.ctypes.data)
+
+    L.oracle_asymp_jasb_pderiv(_I(s.bord), P(b), _D(s.rescale_ee), _I32(int(s.spin_independent)),
+                               P(o["asymp_jasb_pderiv"]))
+    L.oracle_asymp_jasa_pderiv(_I(s.aord), _I(T), P(a), P(kap), P(o["asymp_jasa_pderiv"]))
+    L.oracle_factor_ee_pderiv(_I(W), _I(Ne), _I(s.up), _I(s.bord), P(b), P(I["ee_rescaled"]),
+                              P(o["asymp_jasb_pderiv"]), _I32(int(s.spin_independent)), P(o["factor_ee_pderiv"]))
+    L.oracle_factor_ee_gl_pderiv(_I(W), _I(Ne), _I(s.up), _I(s.bord), P(b), P(I["ee_rescaled"]),
+                                 P(I["ee_rescaled_gl"]), _I32(int(s.spin_independent)), P(o["factor_ee_gl_pderiv"]))
+    L.oracle_factor_en_pderiv(_I(W), _I(Ne), _I(Nn), _I(T), P(typ), _I(s.aord), P(a), P(I["en_rescaled"]),
+                              P(o["asymp_jasa_pderiv"]), P(o["factor_en_pderiv"]))
+    L.oracle_factor_en_gl_pderiv(_I(W), _I(Ne), _I(Nn), _I(T), P(typ), _I(s.aord), P(a), P(I["en_rescaled"]),
+                                 P(I["en_rescaled_gl"]), P(o["factor_en_gl_pderiv"]))
+    L.oracle_forces_jastrow_en(_I(W), _I(Ne), _I(Nn), P(typ), _I(s.aord), P(a), P(I["en_rescaled"]),
+                               P(I["en_rescaled_gl"]), P(o["forces_jastrow_en"]))
+    L.oracle_forces_jastrow_en_g(_I(W), _I(Ne), _I(Nn), P(typ), _I(s.aord), P(a), P(kap), P(edist),
+                                 P(I["en_rescaled"]), P(I["en_rescaled_gl"]), P(o["forces_jastrow_en_g"]))
+    L.oracle_forces_jastrow_en_l(_I(W), _I(Ne), _I(Nn), P(typ), _I(s.aord), P(a), P(kap), P(edist),
+                                 P(I["en_rescaled"]), P(I["en_rescaled_gl"]), P(o["forces_jastrow_en_l"]))
+    if C >= 2:
+        L.oracle_factor_een_pderiv(_I(W), _I(Ne), _I(Nn), _I(T), P(typ), _I(C), _I(d), P(lk), P(I["tmp_c"]),
+                                   P(I["een_n"]), P(o["factor_een_pderiv"]))
+        L.oracle_factor_een_gl_pderiv(_I(W), _I(Ne), _I(Nn), _I(T), P(typ), _I(C), _I(d), P(lk), P(I["tmp_c"]),
+                                      P(I["dtmp_c"]), P(I["een_n"]), P(I["een_n_gl"]), P(o["factor_een_gl_pderiv"]))
+        L.oracle_forces_tmp_c(_I(W), _I(Ne), _I(Nn), _I(C), P(I["een_e"]), P(I["een_n_gl"]), P(o["forces_tmp_c"]))
+        L.oracle_forces_dtmp_c(_I(W), _I(Ne), _I(Nn), _I(C), P(I["een_e_gl"]), P(I["een_n_gl"]),
+                               P(o["forces_dtmp_c"]))
+        L.oracle_forces_een_n(_I(W), _I(Ne), _I(Nn), P(typ), _I(C), P(kap), P(edist), P(I["een_n"]),
+                              P(I["een_n_gl"]), P(o["forces_een_n"]))
+        L.oracle_forces_jastrow_een(_I(W), _I(Ne), _I(Nn), _I(C), _I(d), P(cf), P(lk), P(I["een_n"]),
+                                    P(I["een_n_gl"]), P(I["tmp_c"]), P(o["forces_tmp_c"]), P(o["forces_jastrow_een"]))
+        args = (P(cf), P(lk), P(I["tmp_c"]), P(I["dtmp_c"]), P(o["forces_tmp_c"]), P(o["forces_dtmp_c"]),
+                P(o["forces_een_n"]), P(I["een_n"]), P(I["een_n_gl"]))
+        L.oracle_forces_jastrow_een_g(_I(W), _I(Ne), _I(Nn), _I(C), _I(d), *args, P(o["forces_jastrow_een_g"]))
+        L.oracle_forces_jastrow_een_l(_I(W), _I(Ne), _I(Nn), _I(C), _I(d), *args, P(o["forces_jastrow_een_l"]))
+    o["factor_een_pderiv"] = o["factor_een_pderiv"][:, :d]
+    o["factor_een_gl_pderiv"] = o["factor_een_gl_pderiv"][:, :d]
+    return o
