@@ -315,6 +315,52 @@ qmckl_exit_code qmckl_get_jastrow_champ_value(qmckl_context context, double* con
 qmckl_exit_code qmckl_get_jastrow_champ_gl(qmckl_context context, double* const gl, const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_grad(qmckl_context context, double* const grad, const int64_t size_max);
 
+/* ---- CHAMP Jastrow parameter derivatives: org/qmckl_jastrow_champ.org:4807, 5031, 5332, 8041,
+ *      8269, 8573, 15751, 16055.  Summed over walkers and divided by walk_num for the e-e
+ *      and e-n terms, summed over walkers for the e-e-n term, as the reference.
+ *      asymp_jasb_pderiv [2][bord+1]      asymp_jasa_pderiv [type][aord+1]
+ *      factor_ee_pderiv  [bord+1]         factor_ee_gl_pderiv  [bord+1][elec][4]
+ *      factor_en_pderiv  [type][aord+1]   factor_en_gl_pderiv  [type][aord+1][elec][4]
+ *      factor_een_pderiv [type][dim_c]    factor_een_gl_pderiv [type][dim_c][elec][4] */
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasb_pderiv(qmckl_context context, double* const asymp_jasb_pderiv,
+                                                          const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasa_pderiv(qmckl_context context, double* const asymp_jasa_pderiv,
+                                                          const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee_pderiv(qmckl_context context, double* const factor_ee_pderiv,
+                                                         const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee_gl_pderiv(qmckl_context context, double* const factor_ee_gl_pderiv,
+                                                            const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en_pderiv(qmckl_context context, double* const factor_en_pderiv,
+                                                         const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en_gl_pderiv(qmckl_context context, double* const factor_en_gl_pderiv,
+                                                            const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_pderiv(qmckl_context context, double* const factor_een_pderiv,
+                                                          const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_gl_pderiv(qmckl_context context,
+                                                             double* const factor_een_gl_pderiv,
+                                                             const int64_t size_max);
+
+/* ---- Forces of the Jastrow (derivatives with respect to the nuclear coordinates):
+ *      org/qmckl_forces.org:566, 947, 1372, 2984, 3715, 4079.
+ *      forces_jastrow_en   [walk][nucl][3]              d J_en / d R_a
+ *      forces_jastrow_en_g [walk][nucl][3][elec][3]     d (grad_i J_en)_m / d R_a,l  as [w][a][l][i][m]
+ *      forces_jastrow_en_l [walk][nucl][3]              d (sum_i lap_i J_en) / d R_a
+ *      forces_jastrow_een  [walk][nucl][3]
+ *      forces_jastrow_een_g [walk][3][nucl][3][elec]    d (grad_i J_een)_m / d R_a,l as [w][l][a][m][i]
+ *      forces_jastrow_een_l [walk][3][nucl] */
+qmckl_exit_code qmckl_get_forces_jastrow_en(qmckl_context context, double* const forces_jastrow_en,
+                                            const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_jastrow_en_g(qmckl_context context, double* const forces_jastrow_en_g,
+                                              const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_jastrow_en_l(qmckl_context context, double* const forces_jastrow_en_l,
+                                              const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_jastrow_een(qmckl_context context, double* const forces_jastrow_een,
+                                             const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_jastrow_een_g(qmckl_context context, double* const forces_jastrow_een_g,
+                                               const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_jastrow_een_l(qmckl_context context, double* const forces_jastrow_een_l,
+                                               const int64_t size_max);
+
 /* ======================================================================================
  * Extensions without a counterpart in the reference (device residency, streams, MO GEMM
  * precision).  Device pointers are HIP device addresses on the context's device.

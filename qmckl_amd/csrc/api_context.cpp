@@ -134,6 +134,7 @@ static void release_device(Context* c) {
                     &c->d_feen_gl, &c->d_jvalue, &c->d_jgl, &c->d_jscratch, &c->d_jint,
                     &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c};
   for (DevBuf* b : bufs) b->release();
+  for (DevBuf& b : c->d_deriv) b.release();
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   c->own_stream = nullptr;
   c->stream = nullptr;

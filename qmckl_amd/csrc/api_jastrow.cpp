@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -173,6 +174,226 @@ qmckl_exit_code get_inter(qmckl_context context, int kind, double* out, int64_t 
     c->jint_kind = kind;
   }
   return copy_out(c, context, c->d_jint.ptr, out, n, fn);
+}
+
+// ---- parameter derivatives and nuclear forces ------------------------------------------
+// Buffers of Context::d_deriv: 0..9 the JasInter intermediates, then:
+enum DerivBuf { DB_EDIST = 10, DB_FTMP_C, DB_FDTMP_C, DB_FEEN_N, DB_PART, DB_OUT, DB_LKPM, DB_AJB, DB_AJA };
+
+// (l, k, p, m) of the c-coefficients in the order of
+// org/qmckl_jastrow_champ.org:12000-12040 (lkpm_combined_index), as [4][dim].
+std::vector<int64_t> lkpm_table(int64_t cord) {
+  const int64_t dim = dim_c_of(cord);
+  std::vector<int64_t> t(4 * std::max<int64_t>(dim, 1), 0);
+  int64_t n = 0;
+  for (int64_t p = 2; p <= cord; ++p)
+    for (int64_t k = p - 1; k >= 0; --k) {
+      const int64_t lmax = (k != 0) ? p - k : p - k - 2;
+      for (int64_t l = lmax; l >= 0; --l) {
+        if (((p - k - l) & 1) == 1) continue;
+        t[n] = l;
+        t[dim + n] = k;
+        t[2 * dim + n] = p;
+        t[3 * dim + n] = (p - k - l) / 2;
+        ++n;
+      }
+    }
+  return t;
+}
+
+// org/qmckl_jastrow_champ.org:4948-4994 (asymp_jasb_pderiv): [2][bord+1], entry [s][p] is
+// asymp_jasb_pderiv(p+1, s+1) of the reference.
+std::vector<double> asymp_jasb_pderiv_of(const Context* c) {
+  const int64_t n = c->bord + 1;
+  std::vector<double> o(2 * n, 0.0);
+  const double ki = 1.0 / c->rescale_ee, b0 = c->b_vector[0], b1 = c->bord >= 1 ? c->b_vector[1] : 0.0;
+  const double one = ki / (1.0 + b1 * ki), two = -b0 * ki * ki / ((1.0 + b1 * ki) * (1.0 + b1 * ki));
+  const bool si = c->spin_independent == 1;
+  o[0] = si ? one : 0.5 * one;
+  o[n] = one;
+  if (c->bord >= 1) {
+    o[1] = si ? two : 0.5 * two;
+    o[n + 1] = two;
+  }
+  double x = ki;
+  for (int64_t p = 2; p <= c->bord; ++p) {
+    x *= ki;
+    o[p] = x;
+    o[n + p] = x;
+  }
+  return o;
+}
+
+// org/qmckl_jastrow_champ.org:8192-8215 (asymp_jasa_pderiv): [type][aord+1].
+std::vector<double> asymp_jasa_pderiv_of(const Context* c) {
+  const int64_t n = c->aord + 1;
+  std::vector<double> o(c->type_nucl_num * n, 0.0);
+  for (int64_t t = 0; t < c->type_nucl_num; ++t) {
+    const double* av = c->a_vector.data() + t * n;
+    const double ki = 1.0 / c->rescale_en[t];
+    o[t * n] = ki / (1.0 + av[1] * ki);
+    o[t * n + 1] = -av[0] * ki * ki / ((1.0 + av[1] * ki) * (1.0 + av[1] * ki));
+    double x = ki;
+    for (int64_t p = 2; p <= c->aord; ++p) {
+      x *= ki;
+      o[t * n + p] = x;
+    }
+  }
+  return o;
+}
+
+hipError_t upload_table(Context* c, int slot, const void* src, size_t bytes) {
+  hipError_t e = c->d_deriv[slot].reserve(std::max<size_t>(bytes, 8));
+  if (e == hipSuccess) e = hipMemcpy(c->d_deriv[slot].ptr, src, bytes, hipMemcpyHostToDevice);
+  return e;
+}
+
+// A reference-layout intermediate, computed at most once per context date.
+hipError_t need_inter(Context* c, const JasDevView& v, int kind) {
+  if (c->deriv_date[kind] >= c->date && c->d_deriv[kind].ptr) return hipSuccess;
+  const int64_t n = jastrow_inter_size(v, kind);
+  hipError_t e = c->d_deriv[kind].reserve(sizeof(double) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess) e = launch_jastrow_inter(v, kind, c->d_deriv[kind].as<double>(), c->stream);
+  if (e == hipSuccess) c->deriv_date[kind] = c->date;
+  return e;
+}
+
+// A force intermediate (forces_tmp_c, forces_dtmp_c, forces_een_rescaled_n_gl, or the e-n
+// distances), computed at most once per context date.
+hipError_t need_force_inter(Context* c, const JasDevView& v, DerivArgs& d, int slot) {
+  if (c->deriv_date[slot] >= c->date && c->d_deriv[slot].ptr) return hipSuccess;
+  hipError_t e = hipSuccess;
+  int64_t n = 0;
+  if (slot == DB_EDIST) {
+    n = v.walk_num * v.elec_num * v.nucl_num;
+    e = c->d_deriv[slot].reserve(sizeof(double) * std::max<int64_t>(n, 1));
+    if (e == hipSuccess)
+      e = launch_en_distance(v.coord, v.walk_num * v.elec_num, v.nucl_coord, v.nucl_num, c->d_deriv[slot].as<double>(),
+                             c->stream);
+  } else {
+    const int what = slot == DB_FTMP_C ? DV_FORCES_TMP_C : slot == DB_FDTMP_C ? DV_FORCES_DTMP_C : DV_FORCES_EEN_N;
+    n = deriv_size(d, what);
+    e = c->d_deriv[slot].reserve(sizeof(double) * std::max<int64_t>(n, 1));
+    if (e == hipSuccess) e = launch_deriv(d, what, c->d_deriv[slot].as<double>(), c->stream);
+  }
+  if (e == hipSuccess) c->deriv_date[slot] = c->date;
+  return e;
+}
+
+// Fills `d` and provides the intermediates `what` reads.
+hipError_t prepare_deriv(Context* c, int what, DerivArgs& d) {
+  const JasDevView v = jas_view(c);
+  d = DerivArgs{};
+  d.W = c->walk_num;
+  d.Ne = c->elec_num;
+  d.Nn = c->nucl_num;
+  d.up = c->up_num;
+  d.C = c->cord;
+  d.dim = c->dim_c;
+  d.ntype = c->type_nucl_num;
+  d.aord = c->aord;
+  d.bord = c->bord;
+  d.spin_indep = c->spin_independent;
+  d.type = v.type_nucl;
+  d.a = v.a_vector;
+  d.b = v.b_vector;
+  d.kappa = v.kappa_en;
+  d.cfull = v.cfull;
+  hipError_t e = hipSuccess;
+  auto inter = [&](int kind) -> const double* {
+    if (e == hipSuccess) e = need_inter(c, v, kind);
+    return c->d_deriv[kind].as<double>();
+  };
+  auto force = [&](int slot) -> const double* {
+    if (e == hipSuccess) e = need_force_inter(c, v, d, slot);
+    return c->d_deriv[slot].as<double>();
+  };
+  if (what == DV_EE_PDERIV || what == DV_EE_GL_PDERIV) {
+    d.ee = inter(J_EE_RESCALED);
+    if (what == DV_EE_GL_PDERIV) d.eeg = inter(J_EE_RESCALED_GL);
+    const std::vector<double> ajb = asymp_jasb_pderiv_of(c);
+    if (e == hipSuccess) e = upload_table(c, DB_AJB, ajb.data(), sizeof(double) * ajb.size());
+    d.ajb = c->d_deriv[DB_AJB].as<double>();
+    return e;
+  }
+  if (what == DV_EN_PDERIV || what == DV_EN_GL_PDERIV || what == DV_FORCES_EN || what == DV_FORCES_EN_G ||
+      what == DV_FORCES_EN_L) {
+    d.en = inter(J_EN_RESCALED);
+    if (what != DV_EN_PDERIV) d.eng = inter(J_EN_RESCALED_GL);
+    if (what == DV_FORCES_EN_G || what == DV_FORCES_EN_L) d.edist = force(DB_EDIST);
+    if (what == DV_EN_PDERIV) {
+      const std::vector<double> aja = asymp_jasa_pderiv_of(c);
+      if (e == hipSuccess) e = upload_table(c, DB_AJA, aja.data(), sizeof(double) * aja.size());
+      d.aja = c->d_deriv[DB_AJA].as<double>();
+    }
+    return e;
+  }
+  // e-e-n quantities
+  const std::vector<int64_t> lk = lkpm_table(c->cord);
+  if (e == hipSuccess) e = upload_table(c, DB_LKPM, lk.data(), sizeof(int64_t) * lk.size());
+  d.lkpm = c->d_deriv[DB_LKPM].as<int64_t>();
+  d.tmp_c = inter(J_TMP_C);
+  d.een_n = inter(J_EEN_N);
+  if (what == DV_EEN_PDERIV) return e;
+  d.een_n_gl = inter(J_EEN_N_GL);
+  if (what == DV_EEN_GL_PDERIV) {
+    d.dtmp_c = inter(J_DTMP_C);
+    return e;
+  }
+  d.een_e = inter(J_EEN_E);
+  d.ftmp_c = force(DB_FTMP_C);
+  if (what == DV_FORCES_EEN) return e;
+  d.dtmp_c = inter(J_DTMP_C);
+  d.een_e_gl = inter(J_EEN_E_GL);
+  d.edist = force(DB_EDIST);
+  d.fdtmp_c = force(DB_FDTMP_C);
+  d.feen_n = force(DB_FEEN_N);
+  return e;
+}
+
+// Host getter of a derivative: `what` evaluated on the device, walker-summed quantities
+// reduced over walkers (mean for the e-e and e-n parameter derivatives, sum for e-e-n, as
+// the reference), then copied to the caller's buffer.
+qmckl_exit_code get_deriv(qmckl_context context, int what, double* out, int64_t size_max, const char* fn) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  qmckl_exit_code rc = jas_ready(c, context, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const bool summed = what <= DV_EEN_GL_PDERIV;
+  const bool een = what == DV_EEN_PDERIV || what == DV_EEN_GL_PDERIV || what >= DV_FORCES_TMP_C;
+  DerivArgs probe{};
+  probe.W = c->walk_num;
+  probe.Ne = c->elec_num;
+  probe.Nn = c->nucl_num;
+  probe.C = c->cord;
+  probe.dim = c->dim_c;
+  probe.ntype = c->type_nucl_num;
+  probe.aord = c->aord;
+  probe.bord = c->bord;
+  const int64_t n_all = deriv_size(probe, what);
+  const int64_t n = summed ? n_all / std::max<int64_t>(c->walk_num, 1) : n_all;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  if (een && (c->cord < 2 || c->dim_c == 0)) {  // no e-e-n term: its derivatives vanish
+    std::fill(out, out + n, 0.0);
+    return QMCKL_SUCCESS;
+  }
+  DerivArgs d;
+  hipError_t e = prepare_deriv(c, what, d);
+  if (e == hipSuccess) e = c->d_deriv[DB_OUT].reserve(sizeof(double) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess && summed) {
+    e = c->d_deriv[DB_PART].reserve(sizeof(double) * std::max<int64_t>(n_all, 1));
+    if (e == hipSuccess) e = launch_deriv(d, what, c->d_deriv[DB_PART].as<double>(), c->stream);
+    const double scale = (what == DV_EEN_PDERIV || what == DV_EEN_GL_PDERIV) ? 1.0 : 1.0 / (double)c->walk_num;
+    if (e == hipSuccess)
+      e = launch_sum_walkers(c->d_deriv[DB_PART].as<double>(), c->walk_num, n, scale, c->d_deriv[DB_OUT].as<double>(),
+                             c->stream);
+  } else if (e == hipSuccess) {
+    e = launch_deriv(d, what, c->d_deriv[DB_OUT].as<double>(), c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return copy_out(c, context, c->d_deriv[DB_OUT].ptr, out, n, fn);
 }
 
 }  // namespace
@@ -607,6 +828,84 @@ qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, dou
   hipError_t e = hipMemcpyAsync(d_gl, c->d_jgl.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream);
   if (e != hipSuccess) return hip_fail(context, e, fn);
   return QMCKL_SUCCESS;
+}
+
+
+/* ---- parameter derivatives (org/qmckl_jastrow_champ.org) and nuclear forces
+ *      (org/qmckl_forces.org) */
+// org/qmckl_jastrow_champ.org:4807-4850
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasb_pderiv(qmckl_context context, double* const out,
+                                                          const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  const char* fn = "qmckl_get_jastrow_champ_asymp_jasb_pderiv";
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const std::vector<double> v = asymp_jasb_pderiv_of(c);
+  if (size_max < (int64_t)v.size()) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected (bord_num+1)*2");
+  std::copy(v.begin(), v.end(), out);
+  return QMCKL_SUCCESS;
+}
+// org/qmckl_jastrow_champ.org:8041-8090
+qmckl_exit_code qmckl_get_jastrow_champ_asymp_jasa_pderiv(qmckl_context context, double* const out,
+                                                          const int64_t size_max) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  const char* fn = "qmckl_get_jastrow_champ_asymp_jasa_pderiv";
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const std::vector<double> v = asymp_jasa_pderiv_of(c);
+  if (size_max < (int64_t)v.size()) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
+  std::copy(v.begin(), v.end(), out);
+  return QMCKL_SUCCESS;
+}
+// org/qmckl_jastrow_champ.org:5031-5075
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee_pderiv(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_EE_PDERIV, out, n, "qmckl_get_jastrow_champ_factor_ee_pderiv");
+}
+// org/qmckl_jastrow_champ.org:5332-5376
+qmckl_exit_code qmckl_get_jastrow_champ_factor_ee_gl_pderiv(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_EE_GL_PDERIV, out, n, "qmckl_get_jastrow_champ_factor_ee_gl_pderiv");
+}
+// org/qmckl_jastrow_champ.org:8269-8316
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en_pderiv(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_EN_PDERIV, out, n, "qmckl_get_jastrow_champ_factor_en_pderiv");
+}
+// org/qmckl_jastrow_champ.org:8573-8617
+qmckl_exit_code qmckl_get_jastrow_champ_factor_en_gl_pderiv(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_EN_GL_PDERIV, out, n, "qmckl_get_jastrow_champ_factor_en_gl_pderiv");
+}
+// org/qmckl_jastrow_champ.org:15751-15795
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_pderiv(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_EEN_PDERIV, out, n, "qmckl_get_jastrow_champ_factor_een_pderiv");
+}
+// org/qmckl_jastrow_champ.org:16055-16100
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_gl_pderiv(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_EEN_GL_PDERIV, out, n, "qmckl_get_jastrow_champ_factor_een_gl_pderiv");
+}
+// org/qmckl_forces.org:566-612
+qmckl_exit_code qmckl_get_forces_jastrow_en(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_FORCES_EN, out, n, "qmckl_get_forces_jastrow_en");
+}
+// org/qmckl_forces.org:947-993
+qmckl_exit_code qmckl_get_forces_jastrow_en_g(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_FORCES_EN_G, out, n, "qmckl_get_forces_jastrow_en_g");
+}
+// org/qmckl_forces.org:1372-1418
+qmckl_exit_code qmckl_get_forces_jastrow_en_l(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_FORCES_EN_L, out, n, "qmckl_get_forces_jastrow_en_l");
+}
+// org/qmckl_forces.org:2984-3030
+qmckl_exit_code qmckl_get_forces_jastrow_een(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_FORCES_EEN, out, n, "qmckl_get_forces_jastrow_een");
+}
+// org/qmckl_forces.org:3715-3760
+qmckl_exit_code qmckl_get_forces_jastrow_een_g(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_FORCES_EEN_G, out, n, "qmckl_get_forces_jastrow_een_g");
+}
+// org/qmckl_forces.org:4079-4125
+qmckl_exit_code qmckl_get_forces_jastrow_een_l(qmckl_context ctx, double* const out, const int64_t n) {
+  return get_deriv(ctx, DV_FORCES_EEN_L, out, n, "qmckl_get_forces_jastrow_een_l");
 }
 
 }  // extern "C"

@@ -149,6 +149,11 @@ struct Context {
   uint64_t jastrow_date = 0;
   // Jastrow intermediates (materialised only when asked for)
   DevBuf d_jint;
+  // derivative getters: intermediates (JasInter kinds, then e-n distances and the force
+  // intermediates), per-walker values, outputs and small tables, each valid at deriv_date
+  static constexpr int N_DERIV_BUF = 20;
+  DevBuf d_deriv[N_DERIV_BUF];
+  uint64_t deriv_date[N_DERIV_BUF] = {};
   uint64_t jint_date = 0;
   int jint_kind = -1;
 
@@ -258,6 +263,28 @@ enum JasInter {
 };
 int64_t jastrow_inter_size(const JasDevView& v, int kind);
 hipError_t launch_jastrow_inter(const JasDevView& v, int kind, double* out, hipStream_t s);
+
+// Jastrow parameter derivatives and nuclear forces (kernels_deriv.hip).  Inputs are the
+// reference-layout intermediates of launch_jastrow_inter and the e-n distances [W*Ne][Nn].
+enum DerivKind {
+  DV_EE_PDERIV = 0, DV_EE_GL_PDERIV, DV_EN_PDERIV, DV_EN_GL_PDERIV, DV_EEN_PDERIV, DV_EEN_GL_PDERIV,
+  DV_FORCES_EN, DV_FORCES_EN_G, DV_FORCES_EN_L, DV_FORCES_TMP_C, DV_FORCES_DTMP_C, DV_FORCES_EEN_N,
+  DV_FORCES_EEN, DV_FORCES_EEN_G, DV_FORCES_EEN_L
+};
+struct DerivArgs {
+  int64_t W, Ne, Nn, up, C, dim, ntype, aord, bord;
+  int32_t spin_indep;
+  const int64_t* type;   // [Nn]
+  const double *a, *b, *kappa, *cfull;  // [type][aord+1], [bord+1], [type], [dim][Nn]
+  const int64_t* lkpm;   // [4][dim]
+  const double *ajb, *aja;  // asymp_jasb_pderiv [2][bord+1], asymp_jasa_pderiv [type][aord+1]
+  const double *ee, *eeg, *en, *eng, *edist, *een_e, *een_e_gl, *een_n, *een_n_gl, *tmp_c, *dtmp_c;
+  const double *ftmp_c, *fdtmp_c, *feen_n;
+};
+// Per-walker values of `what` (the walker-summed derivatives before the sum over walkers).
+hipError_t launch_deriv(const DerivArgs& d, int what, double* out, hipStream_t s);
+int64_t deriv_size(const DerivArgs& d, int what);
+hipError_t launch_sum_walkers(const double* in, int64_t W, int64_t n, double scale, double* out, hipStream_t s);
 
 hipError_t launch_ee_distance(const double* coord, int64_t W, int64_t Ne, double* out, hipStream_t s);
 hipError_t launch_en_distance(const double* coord, int64_t P, const double* nucl_coord, int64_t Nn,
