@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--mode", default="bf16x3", choices=["bf16x3", "fp32", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--derivatives", action="store_true",
+                    help="instead of the walker evaluation, time the Jastrow parameter derivatives and "
+                         "nuclear forces (all of their getters) on the 14-electron system, 4096 walkers")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="copy this many MiB device-to-device once before timing (a copy kernel of "
                          "known size, the scale for HBM traffic counters)")
@@ -72,6 +75,9 @@ def main():
     from qmckl_amd import system as S
 
     L = q.lib()
+    if args.derivatives:
+        bench_derivatives(args, q, S, L)
+        return
     log(f"building workload {args.workload}")
     s, x, W = S.workload(args.workload, walk_num=args.walkers)
     Ne, Nmo = s.elec_num, s.mo_num_kept
@@ -225,6 +231,75 @@ def main():
         "cpu_baseline": cpu,
     }
     q.context_destroy(ctx)
+    print(json.dumps(out))
+
+
+DERIV_GETTERS = [
+    ("factor_ee_pderiv", "qmckl_get_jastrow_champ_factor_ee_pderiv"),
+    ("factor_ee_gl_pderiv", "qmckl_get_jastrow_champ_factor_ee_gl_pderiv"),
+    ("factor_en_pderiv", "qmckl_get_jastrow_champ_factor_en_pderiv"),
+    ("factor_en_gl_pderiv", "qmckl_get_jastrow_champ_factor_en_gl_pderiv"),
+    ("factor_een_pderiv", "qmckl_get_jastrow_champ_factor_een_pderiv"),
+    ("factor_een_gl_pderiv", "qmckl_get_jastrow_champ_factor_een_gl_pderiv"),
+    ("forces_jastrow_en", "qmckl_get_forces_jastrow_en"),
+    ("forces_jastrow_en_g", "qmckl_get_forces_jastrow_en_g"),
+    ("forces_jastrow_en_l", "qmckl_get_forces_jastrow_en_l"),
+    ("forces_jastrow_een", "qmckl_get_forces_jastrow_een"),
+    ("forces_jastrow_een_g", "qmckl_get_forces_jastrow_een_g"),
+    ("forces_jastrow_een_l", "qmckl_get_forces_jastrow_een_l"),
+]
+
+
+def bench_derivatives(args, q, S, L):
+    """One step: new coordinates, then every parameter-derivative and force getter (results
+    to host buffers) for all walkers; the 14-electron system, 4096 walkers by default."""
+    W = args.walkers or 4096
+    s, x, W = S.workload("small", walk_num=W)
+    Ne, Nn = s.elec_num, s.nucl_num
+    ctx = q.context_create()
+    S.load(ctx, s, mo=False)
+    S.set_coords(ctx, s, x, W)
+    d = S.dim_c_vector(s.cord)
+    T = s.type_nucl_num
+    sizes = {
+        "factor_ee_pderiv": s.bord + 1, "factor_ee_gl_pderiv": (s.bord + 1) * Ne * 4,
+        "factor_en_pderiv": T * (s.aord + 1), "factor_en_gl_pderiv": T * (s.aord + 1) * Ne * 4,
+        "factor_een_pderiv": T * d, "factor_een_gl_pderiv": T * d * Ne * 4,
+        "forces_jastrow_en": W * Nn * 3, "forces_jastrow_en_g": W * Nn * 3 * Ne * 3,
+        "forces_jastrow_en_l": W * Nn * 3, "forces_jastrow_een": W * Nn * 3,
+        "forces_jastrow_een_g": W * 3 * Nn * 3 * Ne, "forces_jastrow_een_l": W * 3 * Nn,
+    }
+    bufs = {k: np.empty(n) for k, n in sizes.items()}
+    xs = [x, np.ascontiguousarray(x + 1e-3)]
+
+    def step(it):
+        xx = xs[it % 2]
+        q.check(ctx, L.qmckl_set_electron_coord(ctx, b"T", W, q.ptr(xx), xx.size))
+        per = {}
+        for key, fn in DERIV_GETTERS:
+            t0 = time.perf_counter()
+            q.check(ctx, getattr(L, fn)(ctx, q.ptr(bufs[key]), bufs[key].size))
+            per[key] = time.perf_counter() - t0
+        return per
+
+    for it in range(args.warmup):
+        step(it)
+    totals = {k: 0.0 for k, _ in DERIV_GETTERS}
+    t0 = time.perf_counter()
+    for it in range(args.steps):
+        for k, v in step(it).items():
+            totals[k] += v
+    t = (time.perf_counter() - t0) / args.steps
+    q.context_destroy(ctx)
+    out = {
+        "metric": "walker-derivative-evals/sec (Jastrow a/b/c parameter derivatives with GL, and forces with GL)",
+        "value": W / t, "unit": "walker-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"small: 2 nuclei, {Ne} electrons, Jastrow 5/5/5, {W} walkers, all 12 derivative "
+                               f"and force getters per step (host buffers)", "walkers": W},
+        "getter_ms_per_step": {k: v / args.steps * 1e3 for k, v in totals.items()},
+    }
     print(json.dumps(out))
 
 
