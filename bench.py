@@ -62,6 +62,10 @@ def main():
     ap.add_argument("--derivatives", action="store_true",
                     help="instead of the walker evaluation, time the Jastrow parameter derivatives and "
                          "nuclear forces (all of their getters) on the 14-electron system, 4096 walkers")
+    ap.add_argument("--batched", action="store_true",
+                    help="instead of one molecule, evaluate 64 molecules of 10..400 electrons (256 walkers "
+                         "each) from one recorded HIP graph, against the same steps launched eagerly")
+    ap.add_argument("--molecules", type=int, default=64)
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="copy this many MiB device-to-device once before timing (a copy kernel of "
                          "known size, the scale for HBM traffic counters)")
@@ -77,6 +81,9 @@ def main():
     L = q.lib()
     if args.derivatives:
         bench_derivatives(args, q, S, L)
+        return
+    if args.batched:
+        bench_batched(args, q, S, L)
         return
     log(f"building workload {args.workload}")
     s, x, W = S.workload(args.workload, walk_num=args.walkers)
@@ -299,6 +306,74 @@ def bench_derivatives(args, q, S, L):
         "config": {"workload": f"small: 2 nuclei, {Ne} electrons, Jastrow 5/5/5, {W} walkers, all 12 derivative "
                                f"and force getters per step (host buffers)", "walkers": W},
         "getter_ms_per_step": {k: v / args.steps * 1e3 for k, v in totals.items()},
+    }
+    print(json.dumps(out))
+
+
+def bench_batched(args, q, S, L):
+    """One step: the device step (coordinates in, MO VGL, Jastrow value and GL out) of every
+    molecule of the batch, as one graph replay; the same steps launched one call at a time
+    are timed beside it."""
+    from qmckl_amd.batch import Batch
+
+    W = args.walkers or 256
+    log(f"building {args.molecules} molecules")
+    mols = S.batched_workload(args.molecules, W)
+    mode = {"fp64": q.MO_FP64, "fp32": q.MO_FP32, "bf16x3": q.MO_BF16X3}[args.mode]
+    ctxs, coord, mo, val, gl = [], [], [], [], []
+    for s, x, _ in mols:
+        ctx = q.context_create()
+        S.load(ctx, s)
+        S.set_coords(ctx, s, x, W)
+        q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, mode))
+        ctxs.append(ctx)
+        coord.append(q.DeviceBuffer.from_array(x))
+        mo.append(q.DeviceBuffer(8 * W * s.elec_num * 5 * s.mo_num_kept))
+        val.append(q.DeviceBuffer(8 * W))
+        gl.append(q.DeviceBuffer(8 * W * 4 * s.elec_num))
+    q.device_synchronize()
+    n_walk = W * len(mols)
+    ne = [s.elec_num for s, _, _ in mols]
+    nao = [s.ao_num for s, _, _ in mols]
+
+    def eager_step():
+        for (s, _, _), ctx, cb, m, v, g in zip(mols, ctxs, coord, mo, val, gl):
+            Ne = s.elec_num
+            q.check(ctx, L.qmckl_amd_set_electron_coord_device(ctx, b"T", W, cb.ptr, 3 * W * Ne))
+            q.check(ctx, L.qmckl_amd_get_mo_basis_mo_vgl_device(ctx, m.ptr, W * Ne * 5 * s.mo_num_kept))
+            q.check(ctx, L.qmckl_amd_get_jastrow_champ_value_device(ctx, v.ptr, W))
+            q.check(ctx, L.qmckl_amd_get_jastrow_champ_gl_device(ctx, g.ptr, W * 4 * Ne))
+
+    def timed(fn, sync):
+        for _ in range(args.warmup):
+            fn()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        sync()
+        return (time.perf_counter() - t0) / args.steps
+
+    log("timing eager launches")
+    t_eager = timed(eager_step, q.device_synchronize)
+    log(f"eager: {t_eager * 1e3:.3f} ms/step")
+    b = Batch(ctxs, coord, mo, val, gl)
+    log("timing graph replay")
+    t_graph = timed(b.replay, b.synchronize)
+    log(f"graph: {t_graph * 1e3:.3f} ms/step")
+    b.destroy()
+    for c in ctxs:
+        q.context_destroy(c)
+    out = {
+        "metric": "walker-evals/sec (AO+MO+Jastrow VGL), batched uneven molecules in one HIP graph",
+        "value": n_walk / t_graph, "unit": "walker-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_graph * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": f"f64 (AO, Jastrow) + {args.mode} (MO product)", "data": "synthetic (seeded)",
+        "config": {"workload": f"{len(mols)} molecules, electrons {min(ne)}..{max(ne)} (mean {np.mean(ne):.0f}), "
+                               f"AOs {min(nao)}..{max(nao)}, {W} walkers each, Jastrow 5/5/5",
+                   "walkers": n_walk, "molecules": len(mols), "mo_precision": args.mode},
+        "eager_path": {"value": n_walk / t_eager, "ms_per_step": t_eager * 1e3},
+        "graph_speedup_vs_eager": t_eager / t_graph,
     }
     print(json.dumps(out))
 

@@ -398,6 +398,27 @@ qmckl_exit_code qmckl_amd_get_jastrow_champ_value_device(qmckl_context context, 
 qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, double* const d_gl,
                                                       const int64_t size_max);
 
+/* Batched evaluation of several contexts (e.g. molecules of different sizes) replayed from
+ * one HIP graph.  qmckl_amd_batch_record evaluates, for every context i, the step
+ *   qmckl_amd_set_electron_coord_device(contexts[i], 'T', walk_num_i, d_coord[i], ...)
+ *   qmckl_amd_get_mo_basis_mo_vgl_device(..., d_mo_vgl[i], ...)     (skipped if d_mo_vgl is NULL)
+ *   qmckl_amd_get_jastrow_champ_value_device(..., d_value[i], ...)  (skipped if d_value is NULL)
+ *   qmckl_amd_get_jastrow_champ_gl_device(..., d_gl[i], ...)        (skipped if d_gl is NULL)
+ * once eagerly (walk_num_i is the context's current walker count; its coordinates must have
+ * been set), then records the same steps into a graph that forks from and joins back to
+ * `stream` (a hipStream_t; NULL: a stream of the batch's own); qmckl_amd_batch_replay
+ * launches that graph on it.  Each replay reads the
+ * current contents of the d_coord[i] buffers.  The contexts' streams are set to the batch's
+ * branch streams until qmckl_amd_batch_destroy; the contexts must outlive the batch, must not
+ * change sizes, and kernel profiling must be off. */
+qmckl_exit_code qmckl_amd_batch_record(const qmckl_context* contexts, const int64_t n, void* stream,
+                                       const double* const* d_coord, double* const* d_mo_vgl,
+                                       double* const* d_value, double* const* d_gl, int64_t* const batch);
+qmckl_exit_code qmckl_amd_batch_replay(const int64_t batch);
+/* Waits for the batch's replays (its stream; with a NULL `stream` the batch owns one). */
+qmckl_exit_code qmckl_amd_batch_synchronize(const int64_t batch);
+qmckl_exit_code qmckl_amd_batch_destroy(const int64_t batch);
+
 /* Kernel timing: when profiling is on, every kernel launch of the context is bracketed by
  * HIP events on the context's stream.  qmckl_amd_get_kernel_times waits for the stream,
  * returns the accumulated milliseconds and launch counts per slot and resets them.

@@ -343,3 +343,31 @@ def jastrow_workload(nucl_num, up, dn, aord=5, bord=5, cord=5, walk_num=16, seed
                **_jastrow(rng, types, 2, aord, bord, cord))
     x = electron_coords(rng, s, walk_num)
     return s, x, walk_num
+
+
+def molecule_workload(elec_num, walk_num=256, seed=11):
+    """A molecule of the batched workload: ceil(Ne/7) nitrogen-like nuclei (Z = 7) with
+    4s3p2d1f cartesian shells (35 AOs per nucleus, 5 AOs per electron) at random positions
+    1.8 bohr apart or more in a cube of side 2.2 (number of nuclei)^(1/3) bohr; Ne electrons
+    (up-spin first, up = ceil(Ne/2)); max(up, dn) occupied MOs as random orthonormal rows;
+    Jastrow orders 5/5/5; screening precision 41.  Returns (System, coord_T, walk_num)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nn = max(1, -(-elec_num // 7))
+    coord = _place_nuclei(rng, nn, 2.2 * nn ** (1.0 / 3.0) + 1.0, 1.8)
+    elements = ["N"] * nn
+    basis = _basis(rng, elements, {"N": (4, 3, 2, 1)})
+    up = -(-elec_num // 2)
+    dn = elec_num - up
+    nao = basis["ao_factor"].size
+    mo = _orthonormal_rows(rng, max(up, dn), nao)
+    s = System(nucl_charge=np.full(nn, 7.0), nucl_coord=coord, up=up, dn=dn, mo_coef=mo, precision=41,
+               name=f"molecule-{elec_num}e", **basis, **_jastrow(rng, [0] * nn, 1))
+    x = electron_coords(rng, s, walk_num)
+    return s, x, walk_num
+
+
+def batched_workload(n_mol=64, walk_num=256, seed=20261207):
+    """The batched workload: n_mol molecules with electron counts uniform in [10, 400]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = rng.integers(10, 401, n_mol)
+    return [molecule_workload(int(ne), walk_num, seed=int(seed + 1 + i)) for i, ne in enumerate(counts)]
