@@ -40,16 +40,21 @@ constexpr int RT = ROWS / 16; // 16-row MFMA tiles
 constexpr int SH = 8;         // doubles per staged (point, shell): 5 radial VGL + dx, dy, dz
 constexpr int CT = 2;         // column tiles per wave (8 per workgroup)
 constexpr int KP16 = KC + 8;  // padded bf16 row (80 bytes, 16-byte aligned)
+// Row strides of the FP64 / FP32 operand tiles, chosen so that the MFMA operand reads
+// A[16 r + li][4 ks + lg] of a wave hit distinct LDS banks: rows 34 doubles apart
+// (2 mod 32) and 36 floats apart (4 mod 64).
+constexpr int LD64 = KC + 2;
+constexpr int LD32 = KC + 4;
 
 template <int MODE>
 struct Tile;
 template <>
 struct Tile<0> {
-  static constexpr size_t bytes = sizeof(double) * ROWS * (KC + 1);
+  static constexpr size_t bytes = sizeof(double) * ROWS * LD64;
 };
 template <>
 struct Tile<1> {
-  static constexpr size_t bytes = sizeof(float) * ROWS * (KC + 1);
+  static constexpr size_t bytes = sizeof(float) * ROWS * LD32;
 };
 template <>
 struct Tile<2> {
@@ -125,11 +130,11 @@ __global__ void __launch_bounds__(256) k_ao_mo(AoDevView v, const double* __rest
       if (MODE == 0) {
         double* A = static_cast<double*>(at);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) A[(5 * p + c) * (KC + 1) + kk] = r[c];
+        for (int c = 0; c < 5; ++c) A[(5 * p + c) * LD64 + kk] = r[c];
       } else if (MODE == 1) {
         float* A = static_cast<float*>(at);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) A[(5 * p + c) * (KC + 1) + kk] = (float)r[c];
+        for (int c = 0; c < 5; ++c) A[(5 * p + c) * LD32 + kk] = (float)r[c];
       } else {
         uint16_t* A = static_cast<uint16_t*>(at);
 #pragma unroll
@@ -157,7 +162,7 @@ __global__ void __launch_bounds__(256) k_ao_mo(AoDevView v, const double* __rest
           const double b = B[(int64_t)(k0 + 4 * ks + lg) * Npad + n];
 #pragma unroll
           for (int r = 0; r < RT; ++r) {
-            const double a = A[(16 * r + li) * (KC + 1) + 4 * ks + lg];
+            const double a = A[(16 * r + li) * LD64 + 4 * ks + lg];
             acc[j][r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j][r], 0, 0, 0);
           }
         }
@@ -172,7 +177,7 @@ __global__ void __launch_bounds__(256) k_ao_mo(AoDevView v, const double* __rest
           const float b = B[(int64_t)(k0 + 4 * ks + lg) * Npad + n];
 #pragma unroll
           for (int r = 0; r < RT; ++r) {
-            const float a = A[(16 * r + li) * (KC + 1) + 4 * ks + lg];
+            const float a = A[(16 * r + li) * LD32 + 4 * ks + lg];
             part[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, part[r], 0, 0, 0);
           }
         }

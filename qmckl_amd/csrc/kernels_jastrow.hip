@@ -556,7 +556,7 @@ __device__ __forceinline__ void een_role(EenRow& R, double cn, const double* gp,
 // Applies every term of wave (K, Q): X = X[K][Q] (from the products), XR = X[K][0] (Q = 1),
 // X0 = X[0][Q] value only (K = 1).  Terms are visited at compile time.
 template <int C, int K, int Q, int N = 0>
-__device__ __forceinline__ void een_fold(EenRow& R, const double* cf, const double* gp, const double* X,
+__device__ __forceinline__ void een_fold(EenRow& R, const double* cf, int cfs, const double* gp, const double* X,
                                          const double* XR, const double* X0, double kee) {
   constexpr Lkm<C> T{};
   if constexpr (N < T.dim) {
@@ -565,7 +565,7 @@ __device__ __forceinline__ void een_fold(EenRow& R, const double* cf, const doub
     constexpr bool rm0 = (Q == 1 && k == K && m == 0), rml0 = (Q == 1 && k == K && m + l == 0);
     constexpr bool zm = (K == 1 && k == 0 && m == Q), zml = (K == 1 && k == 0 && m + l == Q);
     if constexpr (rm || rml || rm0 || rml0 || zm || zml) {
-      const double cn = cf[16 * N];  // c-coefficient N of this nucleus
+      const double cn = cf[cfs * N];  // c-coefficient N of this nucleus
       const double kk = (double)K * kee;
       if constexpr (rm) een_role<m + l, false, false>(R, cn, gp, X, kk);
       if constexpr (rml) een_role<m, true, false>(R, cn, gp, X, kk);
@@ -574,7 +574,7 @@ __device__ __forceinline__ void een_fold(EenRow& R, const double* cf, const doub
       if constexpr (zm) een_role<m + l, false, true>(R, cn, gp, X0, 0.0);
       if constexpr (zml) een_role<m, true, true>(R, cn, gp, X0, 0.0);
     }
-    een_fold<C, K, Q, N + 1>(R, cf, gp, X, XR, X0, kee);
+    een_fold<C, K, Q, N + 1>(R, cf, cfs, gp, X, XR, X0, kee);
   }
 }
 
@@ -593,7 +593,7 @@ __device__ __forceinline__ void een_dispatch_pass(int wave, d4 (&acc)[5], const 
 }
 
 template <int C, int W = 0>
-__device__ __forceinline__ void een_dispatch_fold(int wave, EenRow& R, const double* cf, const double* gp,
+__device__ __forceinline__ void een_dispatch_fold(int wave, EenRow& R, const double* cf, int cfs, const double* gp,
                                                   const double* X, const double* r0row, const double* srow,
                                                   double kee) {
   if constexpr (W < EenWaves<C>::n) {
@@ -603,11 +603,13 @@ __device__ __forceinline__ void een_dispatch_fold(int wave, EenRow& R, const dou
       if (Q == 1)
 #pragma unroll
         for (int c = 0; c < 5; ++c) XR[c] = r0row[(K - 1) * 5 + c];  // X[K][0]
-      if (K == 1) X0[0] = srow[Q - 1] - gp[Q];                       // X[0][Q], value only
-      een_fold<C, K, Q>(R, cf, gp, X, XR, X0, kee);
+      // X[0][Q], value only.  The subtrahend is g^Q rounded (fma with 0), kept out of any
+      // contraction so that S - g^Q is exactly 0 when electron i is the only one.
+      if (K == 1) X0[0] = srow[Q - 1] - __builtin_fma(gp[Q - 1], gp[1], 0.0);
+      een_fold<C, K, Q>(R, cf, cfs, gp, X, XR, X0, kee);
       return;
     }
-    een_dispatch_fold<C, W + 1>(wave, R, cf, gp, X, r0row, srow, kee);
+    een_dispatch_fold<C, W + 1>(wave, R, cf, cfs, gp, X, r0row, srow, kee);
   }
 }
 
@@ -779,7 +781,7 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
           X[c] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
         }
         EenRow R;
-        een_dispatch_fold<C>(wave, R, cfl + li, gp, X, r0t + il * K1 * 5, st + li * K1, kee);
+        een_dispatch_fold<C>(wave, R, cfl + li, 16, gp, X, r0t + il * K1 * 5, st + li * K1, kee);
         const double h0 = ep[(C + 0) * EM * 16 + pidx], h1 = ep[(C + 1) * EM * 16 + pidx],
                      h2 = ep[(C + 2) * EM * 16 + pidx];
         const double kr = ep[(C + 3) * EM * 16 + pidx], ka2 = ep[(C + 4) * EM * 16 + pidx];
@@ -828,6 +830,282 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
   }
 }
 
+// ------------------------------------------------------------------------------------
+// e-e-n on FP64 matrix cores, resident variant: used when the pair data of a tile of 16
+// electrons i against every electron j, and g_a(r_j) for every nucleus, fit in LDS together.
+// The products and the epilogue are those of k_jas_een_mfma; the work is cut into units
+// (k, q, 16-nucleus tile), each a full contraction over j followed by its fold, and the
+// units are split into EQ_WAVES equal contiguous ranges, one per wave (two waves per SIMD),
+// so every SIMD gets the same number of units and no barrier separates them.  On gfx950 the
+// FP64 matrix instructions and all other vector instructions share one issue budget, so the
+// loop keeps its vector work small: the data of electron j is one LDS record (g_e(r_ij) and
+// 1/r_ij of the 16 electrons i, then x_j, y_j, z_j) addressed from one register.  Each wave
+// keeps its (i, component) partial sums in registers across its units; the sums over
+// nuclei (lanes) and waves are done once at the end, in a fixed order.
+// ------------------------------------------------------------------------------------
+constexpr int EQ_WAVES = 8;
+constexpr int JREC = 36;  // doubles per electron-j record: G[16], R[16], x, y, z, pad
+
+// Dynamic LDS of k_jas_een_res, in doubles.  JP = Ne rounded up to 4, NA16 = 16 x number of
+// nucleus tiles.
+struct EenResLds {
+  int JP, NA16;
+  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, total;
+  __host__ __device__ EenResLds(int JP_, int NA16_, int C, int dim) : JP(JP_), NA16(NA16_) {
+    int o = 0;
+    xt = o;   o += EM * 3;              // tile electrons i
+    jr = o;   o += JP * JREC;           // electron-j records; G = R = 0 for j == i and padding
+    gch = o;  o += NA16 * JP;           // g_a(r_j), [tile][j][16]
+    nuc = o;  o += 4 * NA16;            // nucleus x, y, z, kappa_a
+    enr = o;  o += EM * NA16;           // 1 / |r_i - R_a|, [i][a]
+    r0t = o;  o += EM * (C - 1) * 5;    // q = 0 sums of the tile electrons
+    st = o;   o += NA16 * (C - 1);      // sum over all j of g_a(r_j)^q, [a][q-1]
+    cfl = o;  o += dim * NA16;          // c-coefficients, [n][a]
+    red = o;  o += EQ_WAVES * EM * 5;   // per-wave row sums
+    total = o;
+  }
+};
+
+// The contraction of unit (K, Q) over nsteps x 4 electrons j; lane (li, lg) feeds
+// A(i = li, j = 4 s + lg) and B(j = 4 s + lg, a = li) of v_mfma_f64_16x16x4_f64.
+template <int K, int Q>
+__device__ __forceinline__ void een_res_pass(d4 (&acc)[5], const double* jr, const double* gt, const double (&xi)[3],
+                                             int nsteps, int li, int lg) {
+  const double* pj = jr + lg * JREC;
+  const double* pi = pj + li;
+  const double* pg = gt + lg * 16 + li;
+#pragma unroll 2
+  for (int st = 0; st < nsteps; ++st) {
+    const double Ge = pi[0], ri = pi[16], g = pg[0];
+    const double ux = (xi[0] - pj[32]) * ri, uy = (xi[1] - pj[33]) * ri, uz = (xi[2] - pj[34]) * ri;
+    pj += 4 * JREC;
+    pi += 4 * JREC;
+    pg += 64;
+    double Gk = Ge, gq = g;
+#pragma unroll
+    for (int k = 2; k <= K; ++k) Gk *= Ge;
+#pragma unroll
+    for (int q = 2; q <= Q; ++q) gq *= g;
+    // w = (1, u_x, u_y, u_z, 1/r); u and 1/r are 0 for j == i (1/r is stored 0 there)
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk, gq, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ux, gq, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uy, gq, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uz, gq, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ri, gq, acc[4], 0, 0, 0);
+  }
+}
+
+// One unit: contraction, then the fold of its four rows into ra.  Lane (li, lg) holds rows
+// lg + 4 r (register r of each accumulator), column a = 16 at + li.
+struct EenResCtx {
+  const double *jr, *gt, *xt, *cfl, *r0t, *st, *enr;
+  double nx, ny, nz, ka, kee;
+  int nsteps, li, lg, i0, Ne, a, NA16;
+  bool va;
+};
+
+template <int C, int K, int Q>
+__device__ __forceinline__ void een_res_unit(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
+  constexpr int K1 = C - 1;
+  d4 acc[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+  een_res_pass<K, Q>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int il = x.lg + 4 * r, i = x.i0 + il;
+    if (x.va && i < x.Ne) {
+      double gp[C + 1];
+      gp[0] = 1.0;
+      gp[1] = x.gt[i * 16 + x.li];  // g_a(r_i)
+#pragma unroll
+      for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * gp[1];
+      const double X[5] = {acc[0][r], acc[1][r], acc[2][r], acc[3][r], acc[4][r]};
+      double XR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, X0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (Q == 1)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) XR[c] = x.r0t[il * K1 * 5 + (K - 1) * 5 + c];  // X[K][0]
+      // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
+      if (K == 1) X0[0] = x.st[x.a * K1 + Q - 1] - __builtin_fma(gp[Q - 1], gp[1], 0.0);
+      EenRow R;
+      een_fold<C, K, Q>(R, x.cfl + x.a, x.NA16, gp, X, XR, X0, x.kee);
+      const double rinv = x.enr[il * x.NA16 + x.a];
+      const double h0 = -x.ka * ((x.xt[il * 3] - x.nx) * rinv), h1 = -x.ka * ((x.xt[il * 3 + 1] - x.ny) * rinv),
+                   h2 = -x.ka * ((x.xt[il * 3 + 2] - x.nz) * rinv);
+      const double kr = 2.0 * x.ka * rinv, ka2 = x.ka * x.ka;
+      ra[r][0] += R.sJ;
+      ra[r][1] += h0 * R.s1 - R.t0[0];
+      ra[r][2] += h1 * R.s1 - R.t0[1];
+      ra[r][3] += h2 * R.s1 - R.t0[2];
+      ra[r][4] += ka2 * R.s2 - kr * R.s1 + R.s3 - 2.0 * (h0 * R.t1[0] + h1 * R.t1[1] + h2 * R.t1[2]) - 2.0 * R.sL;
+    }
+  }
+}
+
+template <int C, int W = 0>
+__device__ __forceinline__ void een_res_dispatch(int kq, const EenResCtx& x, const double (&xi)[3],
+                                                 double (&ra)[4][5]) {
+  if constexpr (W < EenWaves<C>::n) {
+    if (kq == W) {
+      een_res_unit<C, een_wave_k(C, W), een_wave_q(C, W)>(x, xi, ra);
+      return;
+    }
+    een_res_dispatch<C, W + 1>(kq, x, xi, ra);
+  }
+}
+
+template <int C>
+__global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, const double* __restrict__ R0,
+                                                              const double* __restrict__ gN,
+                                                              const double* __restrict__ Sg,
+                                                              double* __restrict__ Jt, int ntiles) {
+  constexpr int K1 = C - 1;
+  constexpr int NP = EenWaves<C>::n;  // (k, q) pairs
+  constexpr int NT = 64 * EQ_WAVES;
+  constexpr int DIM = Lkm<C>::dim;
+  extern __shared__ double lds[];
+  const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
+  const int JP = (Ne + 3) & ~3, NA = (Nn + 15) >> 4, NA16 = 16 * NA;
+  const EenResLds L(JP, NA16, C, DIM);
+  double* xt = lds + L.xt;
+  double* jr = lds + L.jr;
+  double* gch = lds + L.gch;
+  double* nuc = lds + L.nuc;
+  double* enr = lds + L.enr;
+  double* r0t = lds + L.r0t;
+  double* st = lds + L.st;
+  double* cfl = lds + L.cfl;
+  double* red = lds + L.red;
+  const int64_t w = blockIdx.x / ntiles;
+  const int tile = (int)(blockIdx.x - w * ntiles);
+  const int i0 = tile * EM;
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double* coord = v.coord;
+  const int64_t cld = v.coord_ld;
+
+  // 1. inputs of the walker: coordinates, nuclei, q = 0 and k = 0 sums, coefficients, g_a(r_j)
+  for (int t = tid; t < EM * 3; t += NT) {
+    const int il = t / 3, d = t - 3 * il;
+    xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
+  }
+  for (int t = tid; t < 4 * JP; t += NT) {
+    const int d = t / JP, jj = t - d * JP;  // d = 3: the pad slot
+    jr[jj * JREC + 32 + d] = (d < 3 && jj < Ne) ? coord[w * Ne + jj + d * cld] : 0.0;
+  }
+  for (int t = tid; t < NA16; t += NT) {
+    const bool va = t < Nn;
+    nuc[t] = va ? v.nucl_coord[t] : 0.0;
+    nuc[NA16 + t] = va ? v.nucl_coord[t + Nn] : 0.0;
+    nuc[2 * NA16 + t] = va ? v.nucl_coord[t + 2 * Nn] : 0.0;
+    nuc[3 * NA16 + t] = va ? v.kappa_en[v.type_nucl[t]] : 0.0;
+  }
+  for (int t = tid; t < EM * K1 * 5; t += NT) {
+    const int il = t / (K1 * 5);
+    r0t[t] = (i0 + il < Ne) ? R0[(w * Ne + i0) * K1 * 5 + t] : 0.0;
+  }
+  for (int t = tid; t < NA16 * K1; t += NT) st[t] = (t / K1 < Nn) ? Sg[w * Nn * K1 + t] : 0.0;
+  for (int t = tid; t < DIM * NA16; t += NT) {
+    const int n = t / NA16, a = t - n * NA16;
+    cfl[t] = (a < Nn) ? v.cfull[(int64_t)n * Nn + a] : 0.0;
+  }
+  for (int t = tid; t < NA16 * JP; t += NT) {
+    const int al = t & 15, r = t >> 4, at = r / JP, j = r - at * JP, a = 16 * at + al;
+    gch[t] = (a < Nn && j < Ne) ? gN[(w * Nn + a) * Ne + j] : 0.0;  // t == (at JP + j) 16 + al
+  }
+  __syncthreads();
+  // 2. pair data of the tile and the e-n inverse distances
+  for (int t = tid; t < EM * JP; t += NT) {
+    const int il = t & (EM - 1), jj = t / EM;
+    const int i = i0 + il;
+    double Ge = 0.0, ri = 0.0;
+    if (i < Ne && jj < Ne && i != jj) {
+      const double* pj = jr + jj * JREC + 32;
+      const double dx = xt[il * 3] - pj[0], dy = xt[il * 3 + 1] - pj[1], dz = xt[il * 3 + 2] - pj[2];
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      ri = 1.0 / d;
+      Ge = exp(-v.rescale_ee * d);
+    }
+    jr[jj * JREC + il] = Ge;
+    jr[jj * JREC + 16 + il] = ri;
+  }
+  for (int t = tid; t < EM * NA16; t += NT) {
+    const int il = t / NA16, a = t - il * NA16;
+    double rinv = 0.0;
+    if (i0 + il < Ne && a < Nn) {
+      const double dx = xt[il * 3] - nuc[a], dy = xt[il * 3 + 1] - nuc[NA16 + a], dz = xt[il * 3 + 2] - nuc[2 * NA16 + a];
+      rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    enr[t] = rinv;
+  }
+  __syncthreads();
+
+  // 3. units (k, q, nucleus tile): wave w takes the contiguous range [w U / 8, (w + 1) U / 8)
+  const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
+  double ra[4][5];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
+  const int nunits = NP * NA;
+  const int u0 = wave * nunits / EQ_WAVES, u1 = (wave + 1) * nunits / EQ_WAVES;
+  EenResCtx x;
+  x.jr = jr;
+  x.xt = xt;
+  x.r0t = r0t;
+  x.enr = enr;
+  x.cfl = cfl;
+  x.st = st;
+  x.kee = v.rescale_ee;
+  x.nsteps = JP >> 2;
+  x.li = li;
+  x.lg = lg;
+  x.i0 = i0;
+  x.Ne = Ne;
+  x.NA16 = NA16;
+  for (int u = u0; u < u1; ++u) {
+    const int at = u / NP, kq = u - at * NP;
+    x.gt = gch + at * JP * 16;
+    x.a = 16 * at + li;
+    x.va = x.a < Nn;
+    x.nx = nuc[x.a];
+    x.ny = nuc[NA16 + x.a];
+    x.nz = nuc[2 * NA16 + x.a];
+    x.ka = nuc[3 * NA16 + x.a];
+    een_res_dispatch<C>(kq, x, xi, ra);
+  }
+  // 4. sums over the nuclei (lanes li) and the waves
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double y = ra[r][c];
+      y += __shfl_xor(y, 1, 64);
+      y += __shfl_xor(y, 2, 64);
+      y += __shfl_xor(y, 4, 64);
+      y += __shfl_xor(y, 8, 64);
+      if (li == 0) red[(wave * EM + lg + 4 * r) * 5 + c] = y;
+    }
+  __syncthreads();
+  double* jrow = xt;  // free after the units
+  for (int t = tid; t < EM * 5; t += NT) {
+    double y = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < EQ_WAVES; ++wv) y += red[wv * EM * 5 + t];
+    const int il = t / 5, c = t - 5 * il, i = i0 + il;
+    if (c == 0)
+      jrow[il] = (i < Ne) ? y : 0.0;
+    else if (i < Ne)
+      o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = y;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sj = 0.0;
+    for (int il = 0; il < EM; ++il) sj += jrow[il];
+    Jt[w * ntiles + tile] = sj;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, const double* __restrict__ Jt,
                                                       int ntiles, int have_een) {
   __shared__ double sval;
@@ -864,6 +1142,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, co
 
 struct EenShape {
   bool mfma;  // matrix-core kernel (cord <= EMFMA_MAX_C)
+  bool res;   // its resident variant (everything of a walker tile in LDS)
   int TI, JC, ntiles;
   size_t lds;
 };
@@ -878,9 +1157,12 @@ EenShape een_shape(const JasDevView& v) {
     s.TI = EM;
     s.JC = std::min((Ne + 3) & ~3, 256);
     s.ntiles = (Ne + EM - 1) / EM;
-    s.lds = sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, C * (C - 1) / 2).total;
+    const size_t res_lds = sizeof(double) * (size_t)EenResLds((Ne + 3) & ~3, 16 * ((Nn + 15) / 16), C, (int)v.dim_c).total;
+    s.res = res_lds <= 160 * 1024;
+    s.lds = s.res ? res_lds : sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, C * (C - 1) / 2).total;
     return s;
   }
+  s.res = false;
   s.TI = kBlock / Nn;
   if (s.TI < 1) s.TI = 1;
   if (s.TI > Ne) s.TI = Ne;
@@ -1129,13 +1411,20 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
   case CC: {                                                                                                   \
     static bool attr_set = false;                                                                              \
     if (!attr_set) {                                                                                           \
-      const hipError_t ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_mfma<CC>),            \
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);       \
+      hipError_t ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_mfma<CC>),                  \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);             \
+      if (ea == hipSuccess)                                                                                    \
+        ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_res<CC>),                            \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
       if (ea != hipSuccess) return ea;                                                                         \
       attr_set = true;                                                                                         \
     }                                                                                                          \
-    hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,      \
-                       sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                         \
+    if (sh.res)                                                                                                \
+      hipLaunchKernelGGL(k_jas_een_res<CC>, grid, dim3(64 * EQ_WAVES), sh.lds, s, v, o, sv.R0, sv.gN, sv.Sg,    \
+                         sv.Jt, sh.ntiles);                                                                    \
+    else                                                                                                       \
+      hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,    \
+                         sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                       \
     break;                                                                                                     \
   }
     switch (C) {
