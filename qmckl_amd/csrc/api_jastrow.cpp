@@ -88,6 +88,18 @@ qmckl_exit_code upload_jastrow(Context* c, qmckl_context handle, const char* fn)
   up(d.jas_a, c->a_vector.data(), 8 * c->a_vector.size());
   up(d.jas_b, c->b_vector.data(), 8 * c->b_vector.size());
   up(d.jas_cfull, cfull.data(), 8 * cfull.size());
+  // the same with the nuclei padded to whole 16-nucleus tiles (zeros), and the nucleus
+  // coordinates and e-n rescaling factors in one table: the e-e-n kernel copies them to LDS
+  const int64_t NA16 = 16 * ((Nn + 15) / 16);
+  std::vector<double> cpad((size_t)std::max<int64_t>(dim_c, 1) * NA16, 0.0), npad(4 * (size_t)NA16, 0.0);
+  for (int64_t n = 0; n < dim_c; ++n)
+    for (int64_t a = 0; a < Nn; ++a) cpad[n * NA16 + a] = cfull[n * Nn + a];
+  for (int64_t a = 0; a < Nn; ++a) {
+    for (int d3 = 0; d3 < 3; ++d3) npad[d3 * NA16 + a] = c->nucl_coord[d3 * Nn + a];
+    npad[3 * NA16 + a] = c->rescale_en[c->type_nucl_vector[a]];
+  }
+  up(d.jas_cfull_pad, cpad.data(), 8 * cpad.size());
+  up(d.jas_nuc_pad, npad.data(), 8 * npad.size());
   up(d.jas_type, c->type_nucl_vector.data(), 8 * c->type_nucl_vector.size());
   up(d.jas_kappa_en, c->rescale_en.data(), 8 * c->rescale_en.size());
   up(d.jas_asymp_a, c->asymp_jasa.data(), 8 * c->asymp_jasa.size());
@@ -117,6 +129,8 @@ JasDevView jas_view(Context* c) {
   v.a_vector = c->dc.jas_a.as<double>();
   v.b_vector = c->dc.jas_b.as<double>();
   v.cfull = c->dc.jas_cfull.as<double>();
+  v.cfull_pad = c->dc.jas_cfull_pad.as<double>();
+  v.nuc_pad = c->dc.jas_nuc_pad.as<double>();
   v.asymp_jasb0 = c->asymp_jasb[0];
   v.asymp_jasb1 = c->asymp_jasb[1];
   v.asymp_jasa = c->dc.jas_asymp_a.as<double>();

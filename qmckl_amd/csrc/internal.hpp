@@ -54,6 +54,7 @@ struct DevConst {
   DevBuf coef_t_bf;   // three bf16 planes [3][mo_pad][k_pad]
   int64_t mo_pad = 0, k_pad = 0;
   DevBuf jas_a, jas_b, jas_cfull, jas_type, jas_kappa_en, jas_asymp_a;
+  DevBuf jas_cfull_pad, jas_nuc_pad;  // nucleus tables zero-padded to whole 16-nucleus tiles
   DevBuf prim_nucl_c, prim_expo_c;  // caller-order primitives (primitive VGL getter)
   uint64_t date = 0;      // context date of the basis upload
   uint64_t jas_date = 0;  // context date of the Jastrow upload
@@ -242,6 +243,8 @@ struct JasDevView {
   const double* a_vector;    // [type][aord+1]
   const double* b_vector;    // [bord+1]
   const double* cfull;       // [dim_c][Nn]
+  const double* cfull_pad;   // [dim_c][NA16], NA16 = 16 ceil(Nn / 16); zero for a >= Nn
+  const double* nuc_pad;     // [4][NA16]: x, y, z, kappa_a of nucleus a; zero for a >= Nn
   double asymp_jasb0, asymp_jasb1;
   const double* asymp_jasa;  // [type]
 };

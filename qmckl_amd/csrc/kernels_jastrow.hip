@@ -77,14 +77,23 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
   return r;
 }
 
+// g_a(r_j) = exp(-kappa_a |r_j - R_a|) scratch, laid out as the e-e-n kernels stage it in
+// LDS: [W][nucleus tile of 16][j, Ne rounded up to 4][16], zero in the padding.
+__host__ __device__ __forceinline__ int64_t gn_index(int64_t w, int a, int j, int NA, int JP) {
+  return ((w * NA + (a >> 4)) * JP + j) * 16 + (a & 15);
+}
+__host__ __device__ __forceinline__ int gn_na(int64_t Nn) { return (int)((Nn + 15) >> 4); }
+__host__ __device__ __forceinline__ int gn_jp(int64_t Ne) { return (int)((Ne + 3) & ~3); }
+
 // ------------------------------------------------------------------------------------
 // e-e and e-n terms, and the q = 0 sums of the e-e-n tensor.
-// R0 scratch layout: [W][Ne][cord-1][5]; gN: [W][Nn][Ne] = exp(-kappa_a |r_i - R_a|).
+// R0 scratch layout: [W][Ne][cord-1][5]; gN: see gn_index.
 // ------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
                                                      double* __restrict__ gN, double* __restrict__ Sg) {
   extern __shared__ double lds[];
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
+  const int NAg = gn_na(Nn), JPg = gn_jp(Ne);
   const int64_t w = blockIdx.x;
   double* xe = lds;            // [Ne][3]
   double* red = lds + 3 * Ne;  // [kBlock]
@@ -195,7 +204,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
       const double dx = xi - v.nucl_coord[a], dy = yi - v.nucl_coord[a + Nn], dz = zi - v.nucl_coord[a + 2 * Nn];
       const double d = sqrt(dx * dx + dy * dy + dz * dz);
       const double e = exp(-ka * d);
-      gN[(w * Nn + a) * Ne + i] = e;  // g_a(r_i), reused by the e-e-n kernel
+      gN[gn_index(w, a, i, NAg, JPg)] = e;  // g_a(r_i), reused by the e-e-n kernels
       const double x = (1.0 - e) * (1.0 / ka);
       fen += av[0] * x / (1.0 + av[1] * x) - v.asymp_jasa[t];
       double xk = x;
@@ -244,10 +253,10 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   if (Sg != nullptr && C >= 2) {
     for (int a = threadIdx.x; a < Nn; a += kBlock) {
       double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      const double* ga = gN + (w * Nn + a) * Ne;
+      const double* ga = gN + gn_index(w, a, 0, NAg, JPg);
 #pragma unroll 4
       for (int i = 0; i < Ne; ++i) {
-        const double g = ga[i];
+        const double g = ga[16 * i];
         double p = g;
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
@@ -257,6 +266,16 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
       }
       for (int q = 0; q < C - 1 && q < 7; ++q) Sg[(w * Nn + a) * (C - 1) + q] = acc[q];
     }
+  }
+  // zero padding of gN: nuclei Nn .. 16 NA - 1 for every j, electrons Ne .. JP - 1 for a < Nn
+  const int npad_a = 16 * NAg - Nn;
+  for (int t = threadIdx.x; t < npad_a * JPg; t += kBlock) {
+    const int j = t / npad_a, a = Nn + t - j * npad_a;
+    gN[gn_index(w, a, j, NAg, JPg)] = 0.0;
+  }
+  for (int t = threadIdx.x; t < (JPg - Ne) * Nn; t += kBlock) {
+    const int j = Ne + t / Nn, a = t % Nn;
+    gN[gn_index(w, a, j, NAg, JPg)] = 0.0;
   }
 }
 
@@ -327,7 +346,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, cons
       }
       for (int t = threadIdx.x; t < Nn * jc; t += kBlock) {
         const int aa = t / jc, jj = t - aa * jc;
-        gn[aa * JC + jj] = gN[(w * Nn + aa) * Ne + j0 + jj];
+        gn[aa * JC + jj] = gN[gn_index(w, aa, j0 + jj, gn_na(Nn), gn_jp(Ne))];
       }
       __syncthreads();
       if (active) {
@@ -703,7 +722,7 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
                      dz = xt[il * 3 + 2] - v.nucl_coord[a + 2 * Nn];
         const double R = sqrt(dx * dx + dy * dy + dz * dz);
         const double rinv = 1.0 / R;
-        const double g1 = gN[(w * Nn + a) * Ne + i];  // exp(-kappa_a R)
+        const double g1 = gN[gn_index(w, a, i, gn_na(Nn), gn_jp(Ne))];  // exp(-kappa_a R)
         gp[1] = g1;
 #pragma unroll
         for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * g1;
@@ -755,7 +774,7 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
       for (int t = tid; t < 16 * jn4; t += NT) {
         const int jj = t % jn4, al = t / jn4;
         const int a = a0 + al;
-        gch[jj * 16 + al] = (a < Nn && jj < jn) ? gN[(w * Nn + a) * Ne + j0 + jj] : 0.0;
+        gch[jj * 16 + al] = (a < Nn && jj < jn) ? gN[gn_index(w, a, j0 + jj, gn_na(Nn), gn_jp(Ne))] : 0.0;
       }
       __syncthreads();
       een_dispatch_pass<C>(wave, acc, pdG, pdR, xj, gch, xi, L.JSP, jn4 >> 2, li, lg);
@@ -834,47 +853,177 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
 // e-e-n on FP64 matrix cores, resident variant: used when the pair data of a tile of 16
 // electrons i against every electron j, and g_a(r_j) for every nucleus, fit in LDS together.
 // The products and the epilogue are those of k_jas_een_mfma; the work is cut into units
-// (k, q, 16-nucleus tile), each a full contraction over j followed by its fold, and the
-// units are split into EQ_WAVES equal contiguous ranges, one per wave (two waves per SIMD),
-// so every SIMD gets the same number of units and no barrier separates them.  On gfx950 the
-// FP64 matrix instructions and all other vector instructions share one issue budget, so the
-// loop keeps its vector work small: the data of electron j is one LDS record (g_e(r_ij) and
-// 1/r_ij of the 16 electrons i, then x_j, y_j, z_j) addressed from one register.  Each wave
-// keeps its (i, component) partial sums in registers across its units; the sums over
-// nuclei (lanes) and waves are done once at the end, in a fixed order.
+// (k, q, 16-nucleus tile), each a full contraction over j followed by its fold.  Units of the
+// same k and tile go in groups of two (q, q + 1) that share the A operand; the groups are
+// dealt to EQ_WAVES waves (two per SIMD) so that every wave gets the same number of units,
+// and no barrier separates them.  On gfx950 the FP64 matrix instructions and all other
+// vector instructions share one issue budget, so the loop keeps its vector work small: the
+// data of electron j is one LDS record (g_e(r_ij) and 1/r_ij of the 16 electrons i, then
+// x_j, y_j, z_j) addressed from one register, and a group builds its A operand once for
+// ten matrix instructions.  Each wave keeps its (i, component) partial sums in registers
+// across its groups; the sums over nuclei (lanes) and waves are done once at the end, in a
+// fixed order.
 // ------------------------------------------------------------------------------------
 constexpr int EQ_WAVES = 8;
 constexpr int JREC = 36;  // doubles per electron-j record: G[16], R[16], x, y, z, pad
 
-// Dynamic LDS of k_jas_een_res, in doubles.  JP = Ne rounded up to 4, NA16 = 16 x number of
-// nucleus tiles.
+// Dynamic LDS of k_jas_een_res, in doubles, every array on a 16-byte boundary.  JP = Ne
+// rounded up to 4, NA16 = 16 x number of nucleus tiles.
 struct EenResLds {
-  int JP, NA16;
   int xt, jr, gch, nuc, enr, r0t, st, cfl, red, total;
-  __host__ __device__ EenResLds(int JP_, int NA16_, int C, int dim) : JP(JP_), NA16(NA16_) {
+  __host__ __device__ EenResLds(int JP, int NA16, int C, int dim) {
     int o = 0;
-    xt = o;   o += EM * 3;              // tile electrons i
-    jr = o;   o += JP * JREC;           // electron-j records; G = R = 0 for j == i and padding
-    gch = o;  o += NA16 * JP;           // g_a(r_j), [tile][j][16]
-    nuc = o;  o += 4 * NA16;            // nucleus x, y, z, kappa_a
-    enr = o;  o += EM * NA16;           // 1 / |r_i - R_a|, [i][a]
-    r0t = o;  o += EM * (C - 1) * 5;    // q = 0 sums of the tile electrons
-    st = o;   o += NA16 * (C - 1);      // sum over all j of g_a(r_j)^q, [a][q-1]
-    cfl = o;  o += dim * NA16;          // c-coefficients, [n][a]
-    red = o;  o += EQ_WAVES * EM * 5;   // per-wave row sums
+    auto take = [&o](int n) { const int at = o; o = (o + n + 1) & ~1; return at; };
+    xt = take(EM * 3);                 // tile electrons i
+    jr = take(JP * JREC);              // electron-j records; G = R = 0 for j == i and padding
+    gch = take(NA16 * JP);             // g_a(r_j), [tile][j][16] (as gn_index)
+    nuc = take(4 * NA16);              // nucleus x, y, z, kappa_a, [4][a]
+    enr = take(EM * NA16);             // 1 / |r_i - R_a|, [i][a]
+    r0t = take(EM * (C - 1) * 5);      // q = 0 sums of the tile electrons
+    st = take(NA16 * (C - 1));         // sum over all j of g_a(r_j)^q, [a][q-1]
+    cfl = take(dim * NA16);            // c-coefficients, [n][a]
+    red = take(EQ_WAVES * EM * 5);     // per-wave row sums
     total = o;
   }
 };
 
-// The contraction of unit (K, Q) over nsteps x 4 electrons j; lane (li, lg) feeds
-// A(i = li, j = 4 s + lg) and B(j = 4 s + lg, a = li) of v_mfma_f64_16x16x4_f64.
-template <int K, int Q>
-__device__ __forceinline__ void een_res_pass(d4 (&acc)[5], const double* jr, const double* gt, const double (&xi)[3],
-                                             int nsteps, int li, int lg) {
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Copies n doubles from global memory to LDS with the gfx950 global-to-LDS loads, 16 bytes
+// per lane (src and dst 16-byte aligned, n even); the waves of the block take 64-lane slices
+// in turn.  The copy is asynchronous: wait (s_waitcnt) and synchronise before reading dst.
+__device__ __forceinline__ void lds_copy16(double* dst, const double* src, int n, int wave, int lane) {
+  for (int base = wave * 128; base < n; base += EQ_WAVES * 128) {
+    const int t = base + 2 * lane;
+    if (t < n) __builtin_amdgcn_global_load_lds(src + t, (lds_void*)(dst + base), 16, 0, 0);
+  }
+}
+// The same, 4 bytes per lane, for any 8-byte aligned src and dst.
+__device__ __forceinline__ void lds_copy4(double* dst, const double* src, int n, int wave, int lane) {
+  const float* s = reinterpret_cast<const float*>(src);
+  float* d = reinterpret_cast<float*>(dst);
+  for (int base = wave * 64; base < 2 * n; base += EQ_WAVES * 64) {
+    const int t = base + lane;
+    if (t < 2 * n) __builtin_amdgcn_global_load_lds(s + t, (lds_void*)(d + base), 4, 0, 0);
+  }
+}
+
+// The fold of one unit, regrouped: every role of a term that multiplies the same X (the
+// unit's products X[K][Q], or X[K][0] when Q = 1, or X[0][Q] when K = 1) differs only in
+// t = c_n g^PG, so the roles are first summed into T = sum t, TJ = sum of t over the roles
+// that carry J, T1 = sum PG t, T2 = sum PG^2 t, and X is applied once (een_apply).
+struct EenTs {
+  double T = 0.0, TJ = 0.0, T1 = 0.0, T2 = 0.0;
+};
+
+template <int PG, bool JROLE>
+__device__ __forceinline__ void een_tadd(EenTs& s, double cn, const double* gp) {
+  const double t = cn * gp[PG];
+  s.T += t;
+  if (JROLE) s.TJ += t;
+  if (PG != 0) {
+    s.T1 += (double)PG * t;
+    s.T2 += (double)(PG * PG) * t;
+  }
+}
+
+// Role sums of unit (K, Q): m with X[K][Q], r with X[K][0] (Q = 1), z with X[0][Q] (K = 1).
+template <int C, int K, int Q, int N = 0>
+__device__ __forceinline__ void een_tsums(EenTs& m, EenTs& r, EenTs& z, const double* cf, int cfs, const double* gp) {
+  constexpr Lkm<C> T{};
+  if constexpr (N < T.dim) {
+    constexpr int l = T.l[N], k = T.k[N], mm = T.m[N];
+    constexpr bool rm = (k == K && mm == Q), rml = (k == K && mm + l == Q);
+    constexpr bool rm0 = (Q == 1 && k == K && mm == 0), rml0 = (Q == 1 && k == K && mm + l == 0);
+    constexpr bool zm = (K == 1 && k == 0 && mm == Q), zml = (K == 1 && k == 0 && mm + l == Q);
+    if constexpr (rm || rml || rm0 || rml0 || zm || zml) {
+      const double cn = cf[cfs * N];  // c-coefficient N of this nucleus
+      if constexpr (rm) een_tadd<mm + l, false>(m, cn, gp);
+      if constexpr (rml) een_tadd<mm, true>(m, cn, gp);
+      if constexpr (rm0) een_tadd<mm + l, false>(r, cn, gp);
+      if constexpr (rml0) een_tadd<mm, true>(r, cn, gp);
+      if constexpr (zm) een_tadd<mm + l, false>(z, cn, gp);
+      if constexpr (zml) een_tadd<mm, true>(z, cn, gp);
+    }
+    een_tsums<C, K, Q, N + 1>(m, r, z, cf, cfs, gp);
+  }
+}
+
+// Per-(i, a) factors of the epilogue: h = -kappa_a (r_i - R_a)/R, kr = 2 kappa_a / R, ka2 = kappa_a^2.
+struct EenEp {
+  double h0, h1, h2, kr, ka2;
+};
+
+// Adds to the row outputs o = (J, grad, lap) the epilogue of role sums s over X, which is
+// linear in the partial sums of k_jas_een_mfma (EenRow):
+//   o0 += TJ X0,   o_c += h_c T1 X0 - kk T X_c,
+//   o4 += ka2 T2 X0 - kr T1 X0 + kk^2 T X0 - 2 kk T1 (h . X_123) - 2 kk T X4;
+// VALUE_ONLY (the X[0][Q] roles, k = 0): the terms without kk.
+template <bool VALUE_ONLY>
+__device__ __forceinline__ void een_apply(double (&o)[5], const EenTs& s, const double* X, double kk, const EenEp& e) {
+  const double t1x = s.T1 * X[0];
+  o[0] += s.TJ * X[0];
+  o[1] += e.h0 * t1x;
+  o[2] += e.h1 * t1x;
+  o[3] += e.h2 * t1x;
+  o[4] += e.ka2 * s.T2 * X[0] - e.kr * t1x;
+  if constexpr (!VALUE_ONLY) {
+    const double kt = kk * s.T, kt1 = kk * s.T1;
+    o[1] -= kt * X[1];
+    o[2] -= kt * X[2];
+    o[3] -= kt * X[3];
+    o[4] += kk * kt * X[0] - 2.0 * kt1 * (e.h0 * X[1] + e.h1 * X[2] + e.h2 * X[3]) - 2.0 * kt * X[4];
+  }
+}
+
+// Groups of units that share k: (k, q) and (k, q + 1) for q = 1, 3, 5, ... (one unit when
+// q + 1 > C - k).  A group shares the A operand G^k w of its units; it is the unit of work
+// dealt to the waves.  Groups of two come first in this enumeration order: the "pairs"
+// (k-major), then the single units.
+constexpr int een_npair(int C) {
+  int n = 0;
+  for (int k = 1; k < C; ++k) n += (C - k) / 2;
+  return n;
+}
+constexpr int een_nsingle(int C) {
+  int n = 0;
+  for (int k = 1; k < C; ++k) n += (C - k) & 1;
+  return n;
+}
+// (k, first q) of pair p / of single unit p
+constexpr int een_pair_k(int C, int p) {
+  for (int k = 1; k < C; ++k) {
+    if (p < (C - k) / 2) return k;
+    p -= (C - k) / 2;
+  }
+  return 0;
+}
+constexpr int een_pair_q(int C, int p) {
+  for (int k = 1; k < C; ++k) {
+    if (p < (C - k) / 2) return 1 + 2 * p;
+    p -= (C - k) / 2;
+  }
+  return 0;
+}
+constexpr int een_single_k(int C, int p) {
+  for (int k = 1; k < C; ++k) {
+    if ((C - k) & 1) {
+      if (p == 0) return k;
+      --p;
+    }
+  }
+  return 0;
+}
+constexpr int een_single_q(int C, int p) { return C - een_single_k(C, p); }
+
+// The contraction of group (K, Q .. Q + NQ - 1) over nsteps x 4 electrons j; lane (li, lg)
+// feeds A(i = li, j = 4 s + lg) and B(j = 4 s + lg, a = li) of v_mfma_f64_16x16x4_f64.
+template <int K, int Q, int NQ>
+__device__ __forceinline__ void een_res_pass(d4 (&acc)[NQ][5], const double* jr, const double* gt,
+                                             const double (&xi)[3], int nsteps, int li, int lg) {
   const double* pj = jr + lg * JREC;
   const double* pi = pj + li;
   const double* pg = gt + lg * 16 + li;
-#pragma unroll 2
   for (int st = 0; st < nsteps; ++st) {
     const double Ge = pi[0], ri = pi[16], g = pg[0];
     const double ux = (xi[0] - pj[32]) * ri, uy = (xi[1] - pj[33]) * ri, uz = (xi[2] - pj[34]) * ri;
@@ -887,70 +1036,89 @@ __device__ __forceinline__ void een_res_pass(d4 (&acc)[5], const double* jr, con
 #pragma unroll
     for (int q = 2; q <= Q; ++q) gq *= g;
     // w = (1, u_x, u_y, u_z, 1/r); u and 1/r are 0 for j == i (1/r is stored 0 there)
-    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk, gq, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ux, gq, acc[1], 0, 0, 0);
-    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uy, gq, acc[2], 0, 0, 0);
-    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uz, gq, acc[3], 0, 0, 0);
-    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ri, gq, acc[4], 0, 0, 0);
+    const double A[5] = {Gk, Gk * ux, Gk * uy, Gk * uz, Gk * ri};
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) {
+      if (n > 0) gq *= g;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[n][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, acc[n][c], 0, 0, 0);
+    }
   }
 }
 
-// One unit: contraction, then the fold of its four rows into ra.  Lane (li, lg) holds rows
+// One group: contraction, then the fold of its four rows into ra.  Lane (li, lg) holds rows
 // lg + 4 r (register r of each accumulator), column a = 16 at + li.
 struct EenResCtx {
   const double *jr, *gt, *xt, *cfl, *r0t, *st, *enr;
   double nx, ny, nz, ka, kee;
   int nsteps, li, lg, i0, Ne, a, NA16;
-  bool va;
 };
 
-template <int C, int K, int Q>
-__device__ __forceinline__ void een_res_unit(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
+template <int C, int K, int Q, int NQ>
+__device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
   constexpr int K1 = C - 1;
-  d4 acc[5];
+  d4 acc[NQ][5];
 #pragma unroll
-  for (int c = 0; c < 5; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
-  een_res_pass<K, Q>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+  for (int n = 0; n < NQ; ++n)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc[n][c] = d4{0.0, 0.0, 0.0, 0.0};
+  een_res_pass<K, Q, NQ>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+  // No branch on the row or the nucleus, so the four rows interleave: a padding nucleus has
+  // zero c-coefficients, kappa, 1/R, products and k = 0 sums and adds exactly 0; a row past
+  // the last electron is discarded by the caller.
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int il = x.lg + 4 * r, i = x.i0 + il;
-    if (x.va && i < x.Ne) {
+    const int il = x.lg + 4 * r, i = min(x.i0 + il, x.Ne - 1);
+    {
       double gp[C + 1];
       gp[0] = 1.0;
       gp[1] = x.gt[i * 16 + x.li];  // g_a(r_i)
 #pragma unroll
       for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * gp[1];
-      const double X[5] = {acc[0][r], acc[1][r], acc[2][r], acc[3][r], acc[4][r]};
-      double XR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, X0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (Q == 1)
-#pragma unroll
-        for (int c = 0; c < 5; ++c) XR[c] = x.r0t[il * K1 * 5 + (K - 1) * 5 + c];  // X[K][0]
-      // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
-      if (K == 1) X0[0] = x.st[x.a * K1 + Q - 1] - __builtin_fma(gp[Q - 1], gp[1], 0.0);
-      EenRow R;
-      een_fold<C, K, Q>(R, x.cfl + x.a, x.NA16, gp, X, XR, X0, x.kee);
       const double rinv = x.enr[il * x.NA16 + x.a];
-      const double h0 = -x.ka * ((x.xt[il * 3] - x.nx) * rinv), h1 = -x.ka * ((x.xt[il * 3 + 1] - x.ny) * rinv),
-                   h2 = -x.ka * ((x.xt[il * 3 + 2] - x.nz) * rinv);
-      const double kr = 2.0 * x.ka * rinv, ka2 = x.ka * x.ka;
-      ra[r][0] += R.sJ;
-      ra[r][1] += h0 * R.s1 - R.t0[0];
-      ra[r][2] += h1 * R.s1 - R.t0[1];
-      ra[r][3] += h2 * R.s1 - R.t0[2];
-      ra[r][4] += ka2 * R.s2 - kr * R.s1 + R.s3 - 2.0 * (h0 * R.t1[0] + h1 * R.t1[1] + h2 * R.t1[2]) - 2.0 * R.sL;
+      EenEp e;
+      e.h0 = -x.ka * ((x.xt[il * 3] - x.nx) * rinv);
+      e.h1 = -x.ka * ((x.xt[il * 3 + 1] - x.ny) * rinv);
+      e.h2 = -x.ka * ((x.xt[il * 3 + 2] - x.nz) * rinv);
+      e.kr = 2.0 * x.ka * rinv;
+      e.ka2 = x.ka * x.ka;
+      const double kk = (double)K * x.kee;
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) {
+        const double X[5] = {acc[n][0][r], acc[n][1][r], acc[n][2][r], acc[n][3][r], acc[n][4][r]};
+        EenTs tm, tr, tz;
+        if (n == 0)
+          een_tsums<C, K, Q>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
+        else
+          een_tsums<C, K, (NQ > 1 ? Q + 1 : Q)>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
+        een_apply<false>(ra[r], tm, X, kk, e);
+        if (Q + n == 1) {  // X[K][0], the q = 0 sums of the tile electrons
+          const double* XR = x.r0t + il * K1 * 5 + (K - 1) * 5;
+          een_apply<false>(ra[r], tr, XR, kk, e);
+        }
+        if (K == 1) {
+          // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
+          const double X0[1] = {x.st[x.a * K1 + Q + n - 1] - __builtin_fma(gp[Q + n - 1], gp[1], 0.0)};
+          een_apply<true>(ra[r], tz, X0, 0.0, e);
+        }
+      }
     }
   }
 }
 
-template <int C, int W = 0>
-__device__ __forceinline__ void een_res_dispatch(int kq, const EenResCtx& x, const double (&xi)[3],
+template <int C, int G = 0>
+__device__ __forceinline__ void een_res_dispatch(int g, const EenResCtx& x, const double (&xi)[3],
                                                  double (&ra)[4][5]) {
-  if constexpr (W < EenWaves<C>::n) {
-    if (kq == W) {
-      een_res_unit<C, een_wave_k(C, W), een_wave_q(C, W)>(x, xi, ra);
+  constexpr int NPR = een_npair(C);
+  if constexpr (G < NPR + een_nsingle(C)) {
+    if (g == G) {
+      if constexpr (G < NPR)
+        een_res_group<C, een_pair_k(C, G), een_pair_q(C, G), 2>(x, xi, ra);
+      else
+        een_res_group<C, een_single_k(C, G - NPR), een_single_q(C, G - NPR), 1>(x, xi, ra);
       return;
     }
-    een_res_dispatch<C, W + 1>(kq, x, xi, ra);
+    een_res_dispatch<C, G + 1>(g, x, xi, ra);
   }
 }
 
@@ -960,12 +1128,11 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
                                                               const double* __restrict__ Sg,
                                                               double* __restrict__ Jt, int ntiles) {
   constexpr int K1 = C - 1;
-  constexpr int NP = EenWaves<C>::n;  // (k, q) pairs
   constexpr int NT = 64 * EQ_WAVES;
   constexpr int DIM = Lkm<C>::dim;
   extern __shared__ double lds[];
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
-  const int JP = (Ne + 3) & ~3, NA = (Nn + 15) >> 4, NA16 = 16 * NA;
+  const int JP = gn_jp(Ne), NA = gn_na(Nn), NA16 = 16 * NA;
   const EenResLds L(JP, NA16, C, DIM);
   double* xt = lds + L.xt;
   double* jr = lds + L.jr;
@@ -984,7 +1151,16 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   const double* coord = v.coord;
   const int64_t cld = v.coord_ld;
 
-  // 1. inputs of the walker: coordinates, nuclei, q = 0 and k = 0 sums, coefficients, g_a(r_j)
+  // 1. inputs of the walker.  The contiguous ones (g_a(r_j) of every nucleus tile, the
+  // nucleus table, c-coefficients, k = 0 and q = 0 sums) go straight from global memory to
+  // LDS, all in flight together; the scattered ones through registers; one wait for both.
+  // The rows of r0t past the last electron keep stale data: they only reach discarded rows.
+  lds_copy16(gch, gN + gn_index(w, 0, 0, NA, JP), NA16 * JP, wave, lane);
+  lds_copy4(nuc, v.nuc_pad, 4 * NA16, wave, lane);
+  lds_copy4(cfl, v.cfull_pad, DIM * NA16, wave, lane);
+  lds_copy4(st, Sg + w * Nn * K1, Nn * K1, wave, lane);
+  lds_copy4(r0t, R0 + (w * Ne + i0) * K1 * 5, min(EM, Ne - i0) * K1 * 5, wave, lane);
+  for (int t = Nn * K1 + tid; t < NA16 * K1; t += NT) st[t] = 0.0;  // padding nuclei
   for (int t = tid; t < EM * 3; t += NT) {
     const int il = t / 3, d = t - 3 * il;
     xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
@@ -993,26 +1169,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     const int d = t / JP, jj = t - d * JP;  // d = 3: the pad slot
     jr[jj * JREC + 32 + d] = (d < 3 && jj < Ne) ? coord[w * Ne + jj + d * cld] : 0.0;
   }
-  for (int t = tid; t < NA16; t += NT) {
-    const bool va = t < Nn;
-    nuc[t] = va ? v.nucl_coord[t] : 0.0;
-    nuc[NA16 + t] = va ? v.nucl_coord[t + Nn] : 0.0;
-    nuc[2 * NA16 + t] = va ? v.nucl_coord[t + 2 * Nn] : 0.0;
-    nuc[3 * NA16 + t] = va ? v.kappa_en[v.type_nucl[t]] : 0.0;
-  }
-  for (int t = tid; t < EM * K1 * 5; t += NT) {
-    const int il = t / (K1 * 5);
-    r0t[t] = (i0 + il < Ne) ? R0[(w * Ne + i0) * K1 * 5 + t] : 0.0;
-  }
-  for (int t = tid; t < NA16 * K1; t += NT) st[t] = (t / K1 < Nn) ? Sg[w * Nn * K1 + t] : 0.0;
-  for (int t = tid; t < DIM * NA16; t += NT) {
-    const int n = t / NA16, a = t - n * NA16;
-    cfl[t] = (a < Nn) ? v.cfull[(int64_t)n * Nn + a] : 0.0;
-  }
-  for (int t = tid; t < NA16 * JP; t += NT) {
-    const int al = t & 15, r = t >> 4, at = r / JP, j = r - at * JP, a = 16 * at + al;
-    gch[t] = (a < Nn && j < Ne) ? gN[(w * Nn + a) * Ne + j] : 0.0;  // t == (at JP + j) 16 + al
-  }
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   // 2. pair data of the tile and the e-n inverse distances
   for (int t = tid; t < EM * JP; t += NT) {
@@ -1040,15 +1197,15 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   }
   __syncthreads();
 
-  // 3. units (k, q, nucleus tile): wave w takes the contiguous range [w U / 8, (w + 1) U / 8)
+  // 3. groups of units, (group, nucleus tile) items dealt longest first: every pair item,
+  // then every single item, each to the least loaded wave (lowest index on ties).  Every
+  // wave runs the same scalar schedule and processes the items it owns.
   const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
   double ra[4][5];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
-  const int nunits = NP * NA;
-  const int u0 = wave * nunits / EQ_WAVES, u1 = (wave + 1) * nunits / EQ_WAVES;
   EenResCtx x;
   x.jr = jr;
   x.xt = xt;
@@ -1063,16 +1220,29 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   x.i0 = i0;
   x.Ne = Ne;
   x.NA16 = NA16;
-  for (int u = u0; u < u1; ++u) {
-    const int at = u / NP, kq = u - at * NP;
+  constexpr int NPR = een_npair(C), NSG = een_nsingle(C);
+  int load[EQ_WAVES];
+#pragma unroll
+  for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
+  const int nitems = (NPR + NSG) * NA;
+  for (int it = 0; it < nitems; ++it) {
+    const bool pair = it < NPR * NA;
+    const int r = pair ? it : it - NPR * NA, per = pair ? NPR : NSG;
+    const int at = r / per, g = (pair ? 0 : NPR) + r - at * per;
+    int owner = 0;
+#pragma unroll
+    for (int wv = 1; wv < EQ_WAVES; ++wv)
+      if (load[wv] < load[owner]) owner = wv;
+#pragma unroll
+    for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] += (wv == owner) ? (pair ? 2 : 1) : 0;
+    if (owner != wave) continue;
     x.gt = gch + at * JP * 16;
     x.a = 16 * at + li;
-    x.va = x.a < Nn;
     x.nx = nuc[x.a];
     x.ny = nuc[NA16 + x.a];
     x.nz = nuc[2 * NA16 + x.a];
     x.ka = nuc[3 * NA16 + x.a];
-    een_res_dispatch<C>(kq, x, xi, ra);
+    een_res_dispatch<C>(g, x, xi, ra);
   }
   // 4. sums over the nuclei (lanes li) and the waves
 #pragma unroll
@@ -1157,7 +1327,7 @@ EenShape een_shape(const JasDevView& v) {
     s.TI = EM;
     s.JC = std::min((Ne + 3) & ~3, 256);
     s.ntiles = (Ne + EM - 1) / EM;
-    const size_t res_lds = sizeof(double) * (size_t)EenResLds((Ne + 3) & ~3, 16 * ((Nn + 15) / 16), C, (int)v.dim_c).total;
+    const size_t res_lds = sizeof(double) * (size_t)EenResLds(gn_jp(Ne), 16 * gn_na(Nn), C, (int)v.dim_c).total;
     s.res = res_lds <= 160 * 1024;
     s.lds = s.res ? res_lds : sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, C * (C - 1) / 2).total;
     return s;
@@ -1367,26 +1537,34 @@ __global__ void k_en_distance(const double* __restrict__ coord, int64_t P, const
 
 }  // namespace
 
-int64_t jastrow_scratch_doubles(const JasDevView& v) {
-  const int64_t C = v.cord;
-  const int64_t r0 = C >= 2 ? v.walk_num * v.elec_num * (C - 1) * 5 : 0;
-  const int64_t sg = C >= 2 ? v.walk_num * v.nucl_num * (C - 1) : 0;
-  const EenShape s = een_shape(v);
-  return r0 + v.walk_num * v.nucl_num * v.elec_num + v.walk_num * s.ntiles + sg;
-}
+// Scratch of the Jastrow kernels: gN, R0, Jt and Sg, each starting on a 256-byte boundary
+// (the e-e-n kernel copies gN into LDS 16 bytes per lane).
+struct ScratchLayout {
+  int64_t gN, R0, Jt, Sg, total;
+  explicit ScratchLayout(const JasDevView& v) {
+    auto up = [](int64_t n) { return (n + 31) & ~int64_t(31); };
+    const int64_t C = v.cord, W = v.walk_num;
+    gN = 0;
+    R0 = up(gN + W * 16 * gn_na(v.nucl_num) * gn_jp(v.elec_num));
+    Jt = up(R0 + (C >= 2 ? W * v.elec_num * (C - 1) * 5 : 0));
+    Sg = up(Jt + W * een_shape(v).ntiles);
+    total = up(Sg + (C >= 2 ? W * v.nucl_num * (C - 1) : 0));
+  }
+};
+
+int64_t jastrow_scratch_doubles(const JasDevView& v) { return ScratchLayout(v).total; }
 
 struct ScratchView {
   double *R0, *gN, *Jt, *Sg;
 };
 
 ScratchView scratch_view(const JasDevView& v, const JasOut& o) {
-  const int64_t C = v.cord;
-  const int64_t r0n = C >= 2 ? v.walk_num * v.elec_num * (C - 1) * 5 : 0;
+  const ScratchLayout l(v);
   ScratchView sv;
-  sv.R0 = o.scratch;
-  sv.gN = o.scratch + r0n;
-  sv.Jt = sv.gN + v.walk_num * v.nucl_num * v.elec_num;
-  sv.Sg = sv.Jt + v.walk_num * een_shape(v).ntiles;
+  sv.gN = o.scratch + l.gN;
+  sv.R0 = o.scratch + l.R0;
+  sv.Jt = o.scratch + l.Jt;
+  sv.Sg = o.scratch + l.Sg;
   return sv;
 }
 
