@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 
 #include "internal.hpp"
 
@@ -119,6 +120,28 @@ qmckl_exit_code get_array(Context* c, qmckl_context ctx, uint32_t mask, T* dst, 
 
 namespace qmckl_amd {
 
+// Host copy of split3 (ao_device.hpp): x ~ hi + mid + lo, three bfloat16 terms, each the
+// round-to-nearest-even of the float of the remainder, so host and device splits agree.
+static uint16_t bf16_rne_host(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static double bf16_value_host(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return (double)f;
+}
+static void split3_host(double x, uint16_t h[3]) {
+  h[0] = bf16_rne_host((float)x);
+  const double r1 = x - bf16_value_host(h[0]);
+  h[1] = bf16_rne_host((float)r1);
+  const double r2 = r1 - bf16_value_host(h[1]);
+  h[2] = bf16_rne_host((float)r2);
+}
+
 // Uploads the AO tables and nuclear coordinates (once per context date change).
 static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* fn) {
   qmckl_exit_code rc = device_ready(c, handle, fn);
@@ -145,6 +168,42 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
   up(d.nucl_range, t.nucl_range.data(), 8 * t.nucl_range.size());
   up(d.nucl_coord, c->nucl_coord.data(), 8 * c->nucl_coord.size());
   {
+    // Descriptors of the 32-AO chunks for the fused AO->MO kernel (layout: AoMoArgs).
+    const int64_t Nao = t.ao_num, nchunk = (Nao + AO_MO_KC - 1) / AO_MO_KC, Nn = t.nucl_num;
+    std::vector<double> desc;
+    std::vector<int64_t> off(1, 0);
+    int dmax = 0;
+    for (int64_t ch = 0; ch < nchunk; ++ch) {
+      const int64_t k0 = ch * AO_MO_KC, k1 = std::min<int64_t>(k0 + AO_MO_KC, Nao);
+      const int s_lo = t.ao_shell[k0], s_hi = t.ao_shell[k1 - 1];
+      const int a_lo = t.shell_nucl[s_lo], a_hi = t.shell_nucl[s_hi];
+      const int p_lo = t.shell_prim_start[s_lo], p_hi = t.shell_prim_start[s_hi] + t.shell_prim_count[s_hi];
+      const int ns = s_hi - s_lo + 1, na = a_hi - a_lo + 1, np = p_hi - p_lo;
+      const size_t at = desc.size();
+      desc.insert(desc.end(), {(double)ns, (double)na, (double)np, 0.0});
+      for (int a = a_lo; a <= a_hi; ++a)
+        desc.insert(desc.end(), {c->nucl_coord[a], c->nucl_coord[Nn + a], c->nucl_coord[2 * Nn + a],
+                                 t.nucl_range[a]});
+      for (int s = s_lo; s <= s_hi; ++s)
+        desc.insert(desc.end(), {(double)(t.shell_nucl[s] - a_lo + 256 * t.shell_prim_count[s]),
+                                 (double)(t.shell_prim_start[s] - p_lo)});
+      for (int q = p_lo; q < p_hi; ++q) desc.insert(desc.end(), {t.prim_expo[q], t.prim_coef[q]});
+      for (int64_t k = k0; k < k0 + AO_MO_KC; ++k) {
+        if (k < Nao)
+          desc.insert(desc.end(), {(double)(t.ao_shell[k] - s_lo) + 256.0 * (double)t.ao_pow[k], t.ao_factor[k]});
+        else
+          desc.insert(desc.end(), {0.0, 0.0});
+      }
+      if (desc.size() % 2) desc.push_back(0.0);
+      dmax = std::max<int>(dmax, (int)(desc.size() - at));
+      off.push_back((int64_t)desc.size());
+    }
+    d.ao_nchunk = (int)nchunk;
+    d.ao_desc_max = dmax;
+    up(d.ao_desc, desc.data(), 8 * desc.size());
+    up(d.ao_desc_off, off.data(), 8 * off.size());
+  }
+  {
     // Nucleus of each primitive in the caller's order, org/qmckl_ao.org:3000-3015
     // (nucleus_prim_index from the first shell of each nucleus).
     std::vector<int32_t> pn(c->prim_num, 0);
@@ -161,22 +220,39 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     // and mo_pad = 16*ceil(mo/16) columns, so the product kernels need no guards on it.
     const int64_t Nao = c->ao_num, Nmo = c->mo_num;
     const int64_t pad = ((Nmo + 15) / 16) * 16, kpad = ((Nao + 31) / 32) * 32;
-    std::vector<double> ct((size_t)kpad * pad, 0.0), nk((size_t)pad * kpad, 0.0);
+    std::vector<double> ct((size_t)kpad * pad, 0.0);
     for (int64_t k = 0; k < Nao; ++k)
-      for (int64_t j = 0; j < Nmo; ++j) {
-        ct[k * pad + j] = c->mo_coef[j * Nao + k];
-        nk[j * kpad + k] = c->mo_coef[j * Nao + k];
-      }
+      for (int64_t j = 0; j < Nmo; ++j) ct[k * pad + j] = c->mo_coef[j * Nao + k];
     d.mo_pad = pad;
     d.k_pad = kpad;
     up(d.coef_t, ct.data(), 8 * ct.size());
-    up(d.coef_nk, nk.data(), 8 * nk.size());
-    if (e == hipSuccess) e = d.coef_t32.reserve(4 * ct.size());
-    if (e == hipSuccess) e = launch_to_f32(d.coef_t.as<double>(), (int64_t)ct.size(), d.coef_t32.as<float>(), c->stream);
-    if (e == hipSuccess) e = d.coef_t_bf.reserve(2 * 3 * nk.size());
-    if (e == hipSuccess)
-      e = launch_split_bf16x3(d.coef_nk.as<double>(), (int64_t)nk.size(), d.coef_t_bf.as<uint16_t>(),
-                              (int64_t)nk.size(), c->stream);
+    // the operand layouts of the fused kernel: per (group of G column tiles, 32-AO chunk)
+    const int64_t nct = pad / 16, nchunk = kpad / AO_MO_KC;
+    const int G = (int)std::min<int64_t>(AO_MO_MAX_G, nct);
+    const int64_t ngroup = (nct + G - 1) / G, NG16 = 16 * G;
+    d.ao_mo_g = G;
+    d.ao_mo_ngroup = (int)ngroup;
+    const size_t nblk = (size_t)(ngroup * nchunk);
+    std::vector<double> b64(nblk * AO_MO_KC * NG16, 0.0);
+    std::vector<float> b32(b64.size(), 0.0f);
+    std::vector<uint16_t> bbf(nblk * 3 * NG16 * AO_MO_BF_LD, 0);
+    for (int64_t g = 0; g < ngroup; ++g)
+      for (int64_t ch = 0; ch < nchunk; ++ch) {
+        const size_t blk = (size_t)(g * nchunk + ch);
+        for (int kk = 0; kk < AO_MO_KC; ++kk)
+          for (int64_t n = 0; n < NG16; ++n) {
+            const int64_t col = g * NG16 + n, k = ch * AO_MO_KC + kk;
+            const double x = col < pad ? ct[k * pad + col] : 0.0;
+            b64[(blk * AO_MO_KC + kk) * NG16 + n] = x;
+            b32[(blk * AO_MO_KC + kk) * NG16 + n] = (float)x;
+            uint16_t h3[3];
+            split3_host(x, h3);
+            for (int pl = 0; pl < 3; ++pl) bbf[((blk * 3 + pl) * NG16 + n) * AO_MO_BF_LD + kk] = h3[pl];
+          }
+      }
+    up(d.coef_ch64, b64.data(), 8 * b64.size());
+    up(d.coef_ch32, b32.data(), 4 * b32.size());
+    up(d.coef_chbf, bbf.data(), 2 * bbf.size());
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   d.date = c->const_date;
@@ -247,18 +323,35 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     out = c->d_mo_vgl.as<double>();
   }
   const int mode = c->mo_precision;
-  const int64_t pad = c->dc.mo_pad, kpad = c->dc.k_pad;
+  const int64_t pad = c->dc.mo_pad;
   ProfScope ps(c, SLOT_MO);
   if (mode == QMCKL_AMD_MO_FP64 && c->ao_vgl_date >= c->date) {
     // The FP64 AO VGL of these points is resident: plain GEMM on it.
     e = launch_gemm_f64(c->d_ao_vgl.as<double>(), Nao, c->dc.coef_t.as<double>(), pad, out, Nmo, 5 * P, Nmo,
                         Nao, c->stream);
   } else {
-    const void* B = mode == QMCKL_AMD_MO_FP64 ? c->dc.coef_t.ptr
-                    : mode == QMCKL_AMD_MO_FP32 ? c->dc.coef_t32.ptr
-                                                : c->dc.coef_t_bf.ptr;
-    e = launch_ao_mo(ao_view(c), c->d_coord.as<double>(), P, 0, P, mode, B, kpad, pad, out, Nmo, Nmo,
-                     c->tables.chunk_shells_max, c->stream);
+    const DevConst& d = c->dc;
+    AoMoArgs a;
+    a.coord = c->d_coord.as<double>();
+    a.coord_ld = P;
+    a.p0 = 0;
+    a.np = P;
+    a.desc = d.ao_desc.as<double>();
+    a.desc_off = d.ao_desc_off.as<int64_t>();
+    a.desc_max = d.ao_desc_max;
+    a.ns_max = c->tables.chunk_shells_max;
+    a.nchunk = d.ao_nchunk;
+    a.G = d.ao_mo_g;
+    a.ngroup = d.ao_mo_ngroup;
+    a.B = mode == QMCKL_AMD_MO_FP64 ? d.coef_ch64.ptr : mode == QMCKL_AMD_MO_FP32 ? d.coef_ch32.ptr : d.coef_chbf.ptr;
+    a.b_chunk_bytes = mode == QMCKL_AMD_MO_FP64   ? AO_MO_KC * 16 * a.G * 8
+                      : mode == QMCKL_AMD_MO_FP32 ? AO_MO_KC * 16 * a.G * 4
+                                                  : 3 * 16 * a.G * AO_MO_BF_LD * 2;
+    a.C = out;
+    a.ldc = Nmo;
+    a.N = Nmo;
+    a.cutoff = ao_view(c).cutoff;
+    e = launch_ao_mo(a, mode, c->stream);
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   if (d_out == nullptr) c->mo_vgl_date = c->date;

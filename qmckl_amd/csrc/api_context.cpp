@@ -127,8 +127,9 @@ static void release_device(Context* c) {
   DevConst& d = c->dc;
   DevBuf* bufs[] = {&d.shell_nucl, &d.shell_l, &d.shell_prim_start, &d.shell_prim_count,
                     &d.shell_ao_start, &d.prim_expo, &d.prim_coef, &d.ao_shell, &d.ao_pow,
-                    &d.ao_factor, &d.nucl_range, &d.nucl_coord, &d.coef_t, &d.coef_t32,
-                    &d.coef_t_bf, &d.coef_nk, &d.jas_a, &d.jas_b, &d.jas_cfull, &d.jas_cfull_pad, &d.jas_nuc_pad, &d.jas_type, &d.jas_kappa_en,
+                    &d.ao_factor, &d.nucl_range, &d.nucl_coord, &d.coef_t, &d.coef_ch64,
+                    &d.coef_ch32, &d.coef_chbf, &d.ao_desc, &d.ao_desc_off, &d.jas_a, &d.jas_b,
+                    &d.jas_cfull, &d.jas_cfull_pad, &d.jas_nuc_pad, &d.jas_type, &d.jas_kappa_en,
                     &c->d_coord, &c->d_ao_vgl, &c->d_ao_work, &c->d_mo_vgl, &c->d_ee_dist,
                     &c->d_en_dist, &c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl,
                     &c->d_feen_gl, &c->d_jvalue, &c->d_jgl, &c->d_jscratch, &c->d_jint,

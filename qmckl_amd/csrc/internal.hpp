@@ -49,10 +49,14 @@ struct DevConst {
   DevBuf prim_expo, prim_coef, ao_shell, ao_pow, ao_factor, nucl_range;
   DevBuf nucl_coord;  // [3][nucl]
   DevBuf coef_t;      // MO coefficients transposed [k_pad][mo_pad] (FP64, zero-padded)
-  DevBuf coef_t32;    // FP32 copy
-  DevBuf coef_nk;     // FP64 [mo_pad][k_pad] (source of the bf16 planes)
-  DevBuf coef_t_bf;   // three bf16 planes [3][mo_pad][k_pad]
+  // C_t in the per-chunk operand layouts of the fused AO->MO kernel, [group][chunk][...]:
+  // FP64 / FP32 [32][16 G], BF16X3 [3][16 G][AO_MO_BF_LD] (zero-padded)
+  DevBuf coef_ch64, coef_ch32, coef_chbf;
   int64_t mo_pad = 0, k_pad = 0;
+  int ao_mo_g = 1, ao_mo_ngroup = 1;
+  // 32-AO chunk descriptors of the fused kernel (AoMoArgs), offsets in doubles
+  DevBuf ao_desc, ao_desc_off;
+  int ao_desc_max = 0, ao_nchunk = 0;
   DevBuf jas_a, jas_b, jas_cfull, jas_type, jas_kappa_en, jas_asymp_a;
   DevBuf jas_cfull_pad, jas_nuc_pad;  // nucleus tables zero-padded to whole 16-nucleus tiles
   DevBuf prim_nucl_c, prim_expo_c;  // caller-order primitives (primitive VGL getter)
@@ -224,11 +228,30 @@ hipError_t launch_gemm_bf16x3(const uint16_t* A, int64_t lda, int64_t a_plane, c
                               int64_t N, int64_t K, hipStream_t s);
 hipError_t launch_split_bf16x3(const double* src, int64_t n, uint16_t* dst, int64_t plane, hipStream_t s);
 // Fused AO VGL -> MO VGL for points p0 .. p0+np-1 (mode 0 FP64, 1 FP32, 2 BF16X3).
-// B: [k_pad][mo_pad] (FP64/FP32) or [3][mo_pad][k_pad] bf16 planes; C: [points][5][ldc].
-// ns_max: largest number of shells among the AOs of one 32-AO chunk.
-hipError_t launch_ao_mo(const AoDevView& v, const double* coord, int64_t coord_ld, int64_t p0, int64_t np, int mode,
-                        const void* B, int64_t Kpad, int64_t Npad, double* C, int64_t ldc, int64_t N, int ns_max,
-                        hipStream_t s);
+// The AOs are taken in chunks of AO_MO_KC; chunk c is described by desc[desc_off[c] ..
+// desc_off[c+1]) (doubles, an even count):
+//   [ns, na, np, 0], na nuclei [x, y, z, range], ns shells [nucleus + 256 nprim, first prim],
+//   np primitives [alpha, coef], AO_MO_KC AOs [shell + 256 (a | b << 8 | c << 16), factor]
+// (indices local to the chunk; AOs past ao_num have factor 0).  B holds C_t chunk by chunk
+// in the operand layout of the mode (see DevConst), b_chunk_bytes per (group, chunk); the MO
+// columns are taken G tiles of 16 at a time, ngroup groups.  C: [points][5][ldc], columns < N.
+constexpr int AO_MO_KC = 32;
+constexpr int AO_MO_MAX_G = 5;
+constexpr int AO_MO_BF_LD = 40;  // bf16 row of a BF16X3 C_t chunk: 32 AOs + 8 padding
+struct AoMoArgs {
+  const double* coord;
+  int64_t coord_ld, p0, np;
+  const double* desc;
+  const int64_t* desc_off;
+  int desc_max, ns_max, nchunk;
+  const void* B;
+  int b_chunk_bytes, G, ngroup;
+  double* C;
+  int64_t ldc, N;
+  double cutoff;
+};
+size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode);
+hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s);
 hipError_t launch_to_f32(const double* src, int64_t n, float* dst, hipStream_t s);
 
 struct JasDevView {

@@ -1,24 +1,36 @@
 // Fused AO VGL -> MO VGL on gfx950 (org/qmckl_ao.org:8438-8572 followed by
 // org/qmckl_mo.org:1905-1940): the AO matrix never leaves the chip.
 //
-// A workgroup owns TP = 16 points, i.e. the 80 rows (point, component) of the product
-// MO[p][c][:] = sum_k AO[p][c][k] C_t[k][:], and one 64-lane wave per 16-column tile of
-// MOs.  The AOs are walked in chunks of KC = 32:
-//   1. the radial VGL of every shell touching the chunk is evaluated for the 16 points
-//      (FP64, screened exactly as the reference) into LDS;
-//   2. the 16 x 32 AO VGL values of the chunk (5 components each) are assembled in FP64
-//      and stored into an LDS tile in the operand precision of the product;
-//   3. each wave multiplies the 80 x 32 tile by its 32 x 16 slice of C_t on matrix cores.
-// Operand precisions:
-//   FP64   v_mfma_f64_16x16x4_f64, FP64 accumulation.
-//   FP32   v_mfma_f32_16x16x4_f32; FP32 partial sums of each chunk added into FP64.
-//   BF16X3 both operands as three bfloat16 terms (the AO tile by an exact truncation split
-//          of float(x), C_t by a rounded split of the FP64 value); per chunk the six
-//          leading cross products (smallest first) through v_mfma_f32_16x16x32_bf16 into
-//          FP32, then added into FP64.
-// A workgroup has 256 lanes; wave w handles the column tiles w, w+4 of its block.
-// C_t is read from global memory (it stays in L2): [Kpad][Npad] for FP64/FP32 and three
-// transposed bfloat16 planes [3][Npad][Kpad] for BF16X3, zero-padded, so no guards.
+// The product MO[p][c][:] = sum_k AO[p][c][k] C_t[k][:] runs on matrix cores with the rows
+// in component-major order: row tile c holds component c (value, d/dx, d/dy, d/dz,
+// Laplacian) of 16 points.  A 64-lane wave owns 16 points (lane li = point) and G column
+// tiles of MOs, i.e. 5 G accumulator tiles.  With that order the A operand a lane feeds is
+// exactly the VGL of its own point for the AOs of its k slots, so every lane evaluates the
+// AO values it needs straight into registers: no AO tile in LDS, and every wave does the
+// same work.  The AOs are walked in chunks of KC = 32:
+//   1. the radial VGL of every shell of the chunk at the wave's 16 points (FP64, screened
+//      exactly as the reference) into the wave's LDS staging area;
+//   2. per k slot, the AO VGL (monomial x radial x normalisation) of the lane's point, then
+//      the products of the 5 components with G column tiles of C_t.
+// The per-chunk data (nuclei, shells, primitives and AOs of the chunk: the chunk
+// descriptor) and the chunk of C_t in the operand layout are copied to LDS one chunk ahead
+// with the gfx950 global-to-LDS loads, so the loop issues no dependent global loads.
+// Operand precisions of this kernel (k_ao_mo_cm):
+//   FP64   v_mfma_f64_16x16x4_f64, lane (li, lg) feeds AO k0 + 4 s + lg at step s.
+//   FP32   v_mfma_f32_16x16x4_f32, same slots; FP32 sums of each chunk added into FP64.
+// A workgroup is 4 waves (64 points); the accumulators live in AGPRs (one wave per SIMD).
+// On gfx950 FP64 matrix and vector instructions share one issue budget, and this layout
+// keeps both at their minimum (no operand-tile traffic, balanced tiles).
+//
+// BF16X3 (k_ao_mo_tile) is bound by the vector work of the AO evaluation instead, not by
+// its matrix instructions, and needs more waves per SIMD to hide latency than 25 FP64
+// accumulator tiles per wave allow.  It keeps the cooperative layout: a workgroup of 4 waves
+// owns 16 points (rows 5 p + c), evaluates their AO VGL for the chunk once into an LDS
+// operand tile, and each wave multiplies it by its column tiles; both operands are three
+// bfloat16 terms (the AO values by an exact truncation split of float(x), C_t by a rounded
+// split of the FP64 value), the six leading cross products (smallest first) go through
+// v_mfma_f32_16x16x32_bf16 into FP32 per chunk and are added into FP64.  It reads the same
+// chunk descriptors (async, LDS) and the same chunked C_t (registers).
 #include <hip/hip_runtime.h>
 
 #include "ao_device.hpp"
@@ -32,225 +44,412 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int TP = 16;        // points per workgroup
-constexpr int KC = 32;        // AOs per chunk
-constexpr int ROWS = 5 * TP;  // product rows per workgroup
-constexpr int RT = ROWS / 16; // 16-row MFMA tiles
-constexpr int SH = 8;         // doubles per staged (point, shell): 5 radial VGL + dx, dy, dz
-constexpr int CT = 2;         // column tiles per wave (8 per workgroup)
-constexpr int KP16 = KC + 8;  // padded bf16 row (80 bytes, 16-byte aligned)
-// Row strides of the FP64 / FP32 operand tiles, chosen so that the MFMA operand reads
-// A[16 r + li][4 ks + lg] of a wave hit distinct LDS banks: rows 34 doubles apart
-// (2 mod 32) and 36 floats apart (4 mod 64).
-constexpr int LD64 = KC + 2;
-constexpr int LD32 = KC + 4;
+constexpr int KC = AO_MO_KC;  // AOs per chunk
+constexpr int NWAVE = 4;      // waves (16 points each) per workgroup
+constexpr int RAD = 6;        // doubles per staged (point, shell): 5 radial VGL + pad
 
-template <int MODE>
-struct Tile;
-template <>
-struct Tile<0> {
-  static constexpr size_t bytes = sizeof(double) * ROWS * LD64;
-};
-template <>
-struct Tile<1> {
-  static constexpr size_t bytes = sizeof(float) * ROWS * LD32;
-};
-template <>
-struct Tile<2> {
-  static constexpr size_t bytes = sizeof(uint16_t) * 3 * ROWS * KP16;
-};
+// Row stride (doubles) of a point's radial staging, 2 mod 32 so that the 16 points of a
+// wave start in different LDS banks.
+__host__ __device__ constexpr int rad_stride(int ns_max) {
+  const int n = RAD * ns_max;
+  return n + ((2 - n) % 32 + 32) % 32;
+}
 
-template <int MODE>
-__global__ void __launch_bounds__(256) k_ao_mo(AoDevView v, const double* __restrict__ coord, int64_t coord_ld,
-                                               int64_t p0, int64_t np, const void* __restrict__ Bv, int64_t Kpad,
-                                               int64_t Npad, double* __restrict__ C, int64_t ldc, int64_t N,
-                                               int ns_max) {
-  extern __shared__ double lds[];
-  double* xt = lds;                 // [TP][3]
-  double* sh = xt + 3 * TP;         // [TP][ns_max][SH]
-  void* at = sh + TP * ns_max * SH;  // operand tile
-  const int Nao = (int)v.ao_num;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthreads = blockDim.x;
-  const int li = lane & 15, lg = lane >> 4;
-  const int64_t tp0 = p0 + (int64_t)blockIdx.x * TP;  // first point of the tile
-  const int ntp = (int)min((int64_t)TP, p0 + np - tp0);
-  int ctile[CT];
-  bool has[CT];
-#pragma unroll
-  for (int j = 0; j < CT; ++j) {
-    ctile[j] = blockIdx.y * 8 + wave + 4 * j;  // MO column tiles of this wave
-    has[j] = ctile[j] * 16 < Npad;
+// Copies n bytes (a multiple of 16, both addresses 16-byte aligned) from global memory to
+// LDS, 16 bytes per lane, the waves of the block taking 1 KiB slices in turn.  Asynchronous:
+// the caller waits (s_waitcnt) and synchronises before reading dst.
+__device__ __forceinline__ void lds_copy(void* dst, const void* src, int nbytes, int wave, int lane) {
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  for (int base = wave * 1024; base < nbytes; base += NWAVE * 1024) {
+    const int t = base + 16 * lane;
+    if (t < nbytes) __builtin_amdgcn_global_load_lds(s + t, (lds_void*)(d + base), 16, 0, 0);
   }
-  const bool has_cols = has[0];
+}
 
-  for (int t = tid; t < TP * 3; t += nthreads) {
-    const int p = t / 3, d = t - 3 * p;
-    xt[t] = p < ntp ? coord[tp0 + p + d * coord_ld] : 0.0;
+struct Chunk {  // views of one chunk descriptor in LDS (layout: AoMoDesc)
+  const double *nuc, *shl, *prm, *ao;
+  int ns, na;
+};
+__device__ __forceinline__ Chunk chunk_view(const double* d) {
+  Chunk c;
+  c.ns = (int)d[0];
+  c.na = (int)d[1];
+  const int np = (int)d[2];
+  c.nuc = d + 4;
+  c.shl = c.nuc + 4 * c.na;
+  c.prm = c.shl + 2 * c.ns;
+  c.ao = c.prm + 2 * np;
+  return c;
+}
+
+// Radial VGL of shell s of the chunk at displacement (x, y, z) from its nucleus, screened as
+// shell_radial (org/qmckl_ao.org:4769-4868).
+__device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, double y, double z, double cutoff,
+                                             double r[5]) {
+  const double code = c.shl[2 * s];
+  const int nl = (int)code & 255, np = (int)code >> 8, p0 = (int)c.shl[2 * s + 1];
+  const double r2 = x * x + y * y + z * z;
+  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+  if (!(r2 > cutoff * c.nuc[4 * nl + 3])) {
+    for (int i = 0; i < np; ++i) {
+      const double al = c.prm[2 * (p0 + i)];
+      const double ar2 = al * r2;
+      if (ar2 > cutoff) continue;
+      const double e = c.prm[2 * (p0 + i) + 1] * exp(-ar2);
+      const double two_a = -2.0 * al * e;
+      v0 += e;
+      v1 += two_a * x;
+      v2 += two_a * y;
+      v3 += two_a * z;
+      v4 += two_a * (3.0 - 2.0 * ar2);
+    }
   }
-  __syncthreads();
+  r[0] = v0;
+  r[1] = v1;
+  r[2] = v2;
+  r[3] = v3;
+  r[4] = v4;
+}
 
-  d4 acc[CT][RT];
-#pragma unroll
-  for (int j = 0; j < CT; ++j)
-#pragma unroll
-    for (int r = 0; r < RT; ++r) acc[j][r] = d4{0.0, 0.0, 0.0, 0.0};
+// AO VGL of AO kk of the chunk at point (px, py, pz), whose shells' radial VGL are staged at
+// rad, `stride` doubles per shell; org/qmckl_ao.org:8530-8565.
+__device__ __forceinline__ void chunk_ao_pt(const Chunk& c, int kk, double px, double py, double pz,
+                                            const double* rad, int stride, double r[5]) {
+  const double code = c.ao[2 * kk];
+  const double f = c.ao[2 * kk + 1];
+  const uint32_t u = (uint32_t)code;
+  const int s = u & 255, pw = u >> 8;
+  const int nl = (int)c.shl[2 * s] & 255;
+  const double x = px - c.nuc[4 * nl], y = py - c.nuc[4 * nl + 1], z = pz - c.nuc[4 * nl + 2];
+  double pv[5];
+  monomial_vgl(x + 1.e-20, y + 1.e-20, z + 1.e-20, pw & 0xff, (pw >> 8) & 0xff, (pw >> 16) & 0xff, pv);
+  const double* sh = rad + stride * s;
+  const double s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3], s4 = sh[4];
+  r[0] = pv[0] * s0 * f;
+  r[1] = (pv[1] * s0 + pv[0] * s1) * f;
+  r[2] = (pv[2] * s0 + pv[0] * s2) * f;
+  r[3] = (pv[3] * s0 + pv[0] * s3) * f;
+  r[4] = (pv[4] * s0 + pv[0] * s4 + 2.0 * (pv[1] * s1 + pv[2] * s2 + pv[3] * s3)) * f;
+}
+__device__ __forceinline__ void chunk_ao(const Chunk& c, int kk, double px, double py, double pz, const double* rad,
+                                         double r[5]) {
+  chunk_ao_pt(c, kk, px, py, pz, rad, RAD, r);
+}
 
-  for (int k0 = 0; k0 < Kpad; k0 += KC) {
-    // 1. radial parts of the shells of AOs k0 .. k0+KC-1
-    const int kend = min(k0 + KC, Nao);
-    const int s_lo = v.ao_shell[k0];
-    const int ns = v.ao_shell[kend - 1] - s_lo + 1;
-    for (int t = tid; t < TP * ns; t += nthreads) {
-      const int p = t / ns, sl = t - p * ns;
-      const int s = s_lo + sl;
-      const int a = v.shell_nucl[s];
-      const double x = xt[3 * p] - v.nucl_coord[a];
-      const double y = xt[3 * p + 1] - v.nucl_coord[a + v.nucl_num];
-      const double z = xt[3 * p + 2] - v.nucl_coord[a + 2 * v.nucl_num];
+template <int MODE, int G>
+__global__ void __launch_bounds__(64 * NWAVE) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_ao_mo_cm(AoMoArgs A) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  constexpr int NG16 = 16 * G;
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int RS = rad_stride(A.ns_max);
+  double* dbuf[2] = {lds, lds + A.desc_max};
+  char* bbuf[2] = {reinterpret_cast<char*>(lds + 2 * A.desc_max),
+                   reinterpret_cast<char*>(lds + 2 * A.desc_max) + A.b_chunk_bytes};
+  double* rad = lds + 2 * A.desc_max + A.b_chunk_bytes / 4 + wave * 16 * RS;  // 2 b buffers = b/4 doubles
+  const int group = blockIdx.y;
+  const int64_t p = A.p0 + (int64_t)blockIdx.x * (16 * NWAVE) + 16 * wave + li;  // the lane's point
+  const bool pv = p < A.p0 + A.np;
+  const double px = pv ? A.coord[p] : 0.0, py = pv ? A.coord[p + A.coord_ld] : 0.0,
+               pz = pv ? A.coord[p + 2 * A.coord_ld] : 0.0;
+  const char* bsrc = static_cast<const char*>(A.B) + (int64_t)group * A.nchunk * A.b_chunk_bytes;
+  const double cutoff = A.cutoff;
+
+  d4 acc[5][G];
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+#pragma unroll
+    for (int j = 0; j < G; ++j) acc[r][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  lds_copy(dbuf[0], A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
+  lds_copy(bbuf[0], bsrc, A.b_chunk_bytes, wave, lane);
+  for (int ch = 0; ch < A.nchunk; ++ch) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();  // chunk ch in LDS; every wave done with chunk ch - 1
+    if (ch + 1 < A.nchunk) {
+      const int nb = (ch + 1) & 1;
+      lds_copy(dbuf[nb], A.desc + A.desc_off[ch + 1], 8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave, lane);
+      lds_copy(bbuf[nb], bsrc + (int64_t)(ch + 1) * A.b_chunk_bytes, A.b_chunk_bytes, wave, lane);
+    }
+    const Chunk c = chunk_view(dbuf[ch & 1]);
+    const char* bl = bbuf[ch & 1];
+    // 1. radial VGL of the chunk's shells at the wave's points: lane (li, lg) takes shells
+    //    lg, lg + 4, ... at point li
+    for (int s = lg; s < c.ns; s += 4) {
+      const int nl = (int)c.shl[2 * s] & 255;
       double r[5];
-      shell_radial(v, s, a, x, y, z, x * x + y * y + z * z, r);
-      double* o = sh + (p * ns_max + sl) * SH;
+      chunk_radial(c, s, px - c.nuc[4 * nl], py - c.nuc[4 * nl + 1], pz - c.nuc[4 * nl + 2], cutoff, r);
+      double* o = rad + li * RS + RAD * s;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) o[c] = r[c];
-      o[5] = x;
-      o[6] = y;
-      o[7] = z;
+      for (int q = 0; q < 5; ++q) o[q] = r[q];
     }
-    __syncthreads();
-    // 2. AO VGL of the chunk into the operand tile, row = 5 p + c
-    for (int t = tid; t < TP * KC; t += nthreads) {
-      const int p = t / KC, kk = t - p * KC;
-      const int k = k0 + kk;
-      double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (p < ntp && k < Nao) {
-        const double* o = sh + (p * ns_max + (v.ao_shell[k] - s_lo)) * SH;
-        ao_from_shell(v, k, o[5], o[6], o[7], o, r);
-      }
-      if (MODE == 0) {
-        double* A = static_cast<double*>(at);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double* myrad = rad + li * RS;
+    // 2. AO VGL of the lane's slots and the products
+    if constexpr (MODE == 0) {
+      const double* B = reinterpret_cast<const double*>(bl);  // [32][16 G]
+#pragma unroll 2
+      for (int ks = 0; ks < KC / 4; ++ks) {
+        double a[5];
+        chunk_ao(c, 4 * ks + lg, px, py, pz, myrad, a);
+        const double* brow = B + (4 * ks + lg) * NG16 + li;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) A[(5 * p + c) * LD64 + kk] = r[c];
-      } else if (MODE == 1) {
-        float* A = static_cast<float*>(at);
+        for (int j = 0; j < G; ++j) {
+          const double b = brow[16 * j];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) A[(5 * p + c) * LD32 + kk] = (float)r[c];
-      } else {
-        uint16_t* A = static_cast<uint16_t*>(at);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          uint16_t h, m, l;
-          split3_trunc(r[c], h, m, l);
-          const int idx = (5 * p + c) * KP16 + kk;
-          A[idx] = h;
-          A[ROWS * KP16 + idx] = m;
-          A[2 * ROWS * KP16 + idx] = l;
+          for (int r = 0; r < 5; ++r) acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][j], 0, 0, 0);
         }
       }
+    } else {
+      const float* B = reinterpret_cast<const float*>(bl);  // [32][16 G]
+      f4 part[5][G];
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int j = 0; j < G; ++j) part[r][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+      for (int ks = 0; ks < KC / 4; ++ks) {
+        double a[5];
+        chunk_ao(c, 4 * ks + lg, px, py, pz, myrad, a);
+        const float* brow = B + (4 * ks + lg) * NG16 + li;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const float b = brow[16 * j];
+#pragma unroll
+          for (int r = 0; r < 5; ++r)
+            part[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a[r], b, part[r][j], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[r][j][q] += (double)part[r][j][q];
+    }
+  }
+  // rows of the result tiles: f64 MFMA register q holds point lg + 4 q, f32 4 lg + q
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const int64_t col = (int64_t)(group * G + j) * 16 + li;
+    if (col >= A.N) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int prow = MODE == 0 ? lg + 4 * q : 4 * lg + q;
+      const int64_t pt = A.p0 + (int64_t)blockIdx.x * (16 * NWAVE) + 16 * wave + prow;
+      if (pt >= A.p0 + A.np) continue;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) A.C[(pt * 5 + r) * A.ldc + col] = acc[r][j][q];
+    }
+  }
+}
+
+// BF16X3 fused kernel: see the header.  TP = 16 points per workgroup, KC = 32 AOs per chunk.
+// The 5 x G products of a chunk (row tile r, column tile j) go to the waves as: wave w takes
+// column tile w (all 5 row tiles) when w < min(G, 4), and with G = 5 the tiles (r, 4) are
+// spread over the waves by r mod 4, so no wave holds more than 7 accumulator tiles.
+constexpr int TP = 16;
+constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row = 5 p + c)
+constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
+constexpr int SH = 8;          // doubles per staged (point, shell): 5 radial VGL + dx, dy, dz
+constexpr int KP16 = AO_MO_BF_LD;
+
+// six leading cross products of a (3 planes) x b (3 planes), smallest first, into FP32
+__device__ __forceinline__ f4 bf16x3_product(const bf8 (&a)[3], const bf8 (&b)[3]) {
+  f4 q = f4{0.f, 0.f, 0.f, 0.f};
+  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], q, 0, 0, 0);
+  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], q, 0, 0, 0);
+  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], q, 0, 0, 0);
+  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], q, 0, 0, 0);
+  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], q, 0, 0, 0);
+  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], q, 0, 0, 0);
+  return q;
+}
+
+template <int G>
+__global__ void __launch_bounds__(64 * NWAVE) __attribute__((amdgpu_waves_per_eu(3))) k_ao_mo_tile(AoMoArgs A) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  constexpr int NG16 = 16 * G;
+  constexpr int NMAIN = G < NWAVE ? G : NWAVE;  // waves with a main column tile
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  double* dbuf[2] = {lds, lds + A.desc_max};
+  double* xt = lds + 2 * A.desc_max;  // [TP][3]
+  double* sh = xt + 3 * TP;           // [TP][ns_max][SH]
+  uint16_t* at = reinterpret_cast<uint16_t*>(sh + TP * A.ns_max * SH);  // [3][ROWS][KP16]
+  const int group = blockIdx.y;
+  const int64_t tp0 = A.p0 + (int64_t)blockIdx.x * TP;
+  const int ntp = (int)min((int64_t)TP, A.p0 + A.np - tp0);
+  const uint16_t* bsrc = reinterpret_cast<const uint16_t*>(static_cast<const char*>(A.B) +
+                                                          (int64_t)group * A.nchunk * A.b_chunk_bytes);
+  const int bchunk = A.b_chunk_bytes / 2;  // bf16 elements per chunk
+  const bool has_main = wave < NMAIN;
+  for (int t = tid; t < TP * 3; t += 64 * NWAVE) {
+    const int q = t / 3, d = t - 3 * q;
+    xt[t] = q < ntp ? A.coord[tp0 + q + d * A.coord_ld] : 0.0;
+  }
+  d4 acc[RT], accx[2];  // main column tile; with G = 5, tiles (wave, 4) and (wave + 4, 4)
+#pragma unroll
+  for (int r = 0; r < RT; ++r) acc[r] = d4{0.0, 0.0, 0.0, 0.0};
+  accx[0] = accx[1] = d4{0.0, 0.0, 0.0, 0.0};
+  auto load_b = [&](int ch, int j, bf8 (&b)[3]) {
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+      b[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(bsrc + (int64_t)ch * bchunk +
+                                                                   ((pl * NG16) + 16 * j + li) * KP16 + 8 * lg));
+  };
+  auto load_a = [&](int r, bf8 (&a)[3]) {
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+      a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(at + pl * ROWS * KP16 + (16 * r + li) * KP16 + 8 * lg));
+  };
+  lds_copy(dbuf[0], A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
+  for (int ch = 0; ch < A.nchunk; ++ch) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();  // descriptor ch in LDS; every wave done with chunk ch - 1
+    if (ch + 1 < A.nchunk)
+      lds_copy(dbuf[(ch + 1) & 1], A.desc + A.desc_off[ch + 1], 8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave,
+               lane);
+    const Chunk c = chunk_view(dbuf[ch & 1]);
+    // 1. radial VGL of the chunk's shells at the 16 points
+    for (int t = tid; t < TP * c.ns; t += 64 * NWAVE) {
+      const int q = t / c.ns, s = t - q * c.ns;
+      const int nl = (int)c.shl[2 * s] & 255;
+      const double x = xt[3 * q] - c.nuc[4 * nl], y = xt[3 * q + 1] - c.nuc[4 * nl + 1],
+                   z = xt[3 * q + 2] - c.nuc[4 * nl + 2];
+      double r[5];
+      chunk_radial(c, s, x, y, z, A.cutoff, r);
+      double* o = sh + (q * A.ns_max + s) * SH;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) o[k] = r[k];
     }
     __syncthreads();
-    // 3. product of the chunk on matrix cores
+    // 2. AO VGL of the chunk into the operand tile (three bf16 planes), row = 5 p + c
+    for (int t = tid; t < TP * KC; t += 64 * NWAVE) {
+      const int q = t / KC, kk = t - q * KC;
+      double r[5];
+      chunk_ao_pt(c, kk, xt[3 * q], xt[3 * q + 1], xt[3 * q + 2], sh + q * A.ns_max * SH, SH, r);
 #pragma unroll
-    for (int j = 0; j < CT; ++j) {
-      if (!has[j]) continue;
-      const int n = ctile[j] * 16 + li;
-      if (MODE == 0) {
-        const double* B = static_cast<const double*>(Bv);
-        const double* A = static_cast<const double*>(at);
+      for (int k = 0; k < 5; ++k) {
+        uint16_t h, m, l;
+        split3_trunc(r[k], h, m, l);
+        const int idx = (5 * q + k) * KP16 + kk;
+        at[idx] = h;
+        at[ROWS * KP16 + idx] = m;
+        at[2 * ROWS * KP16 + idx] = l;
+      }
+    }
+    __syncthreads();
+    // 3. products of the chunk on matrix cores
+    if (has_main) {
+      bf8 b[3];
+      load_b(ch, wave, b);
 #pragma unroll
-        for (int ks = 0; ks < KC / 4; ++ks) {
-          const double b = B[(int64_t)(k0 + 4 * ks + lg) * Npad + n];
+      for (int r = 0; r < RT; ++r) {
+        bf8 a[3];
+        load_a(r, a);
+        const f4 q4 = bf16x3_product(a, b);
 #pragma unroll
-          for (int r = 0; r < RT; ++r) {
-            const double a = A[(16 * r + li) * LD64 + 4 * ks + lg];
-            acc[j][r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j][r], 0, 0, 0);
-          }
-        }
-      } else if (MODE == 1) {
-        const float* B = static_cast<const float*>(Bv);
-        const float* A = static_cast<const float*>(at);
-        f4 part[RT];
+        for (int k = 0; k < 4; ++k) acc[r][k] += (double)q4[k];
+      }
+    }
+    if constexpr (G == 5) {
+      bf8 b[3];
+      load_b(ch, 4, b);
 #pragma unroll
-        for (int r = 0; r < RT; ++r) part[r] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int x = 0; x < 2; ++x) {
+        const int r = wave + 4 * x;
+        if (r >= RT) continue;
+        bf8 a[3];
+        load_a(r, a);
+        const f4 q4 = bf16x3_product(a, b);
 #pragma unroll
-        for (int ks = 0; ks < KC / 4; ++ks) {
-          const float b = B[(int64_t)(k0 + 4 * ks + lg) * Npad + n];
-#pragma unroll
-          for (int r = 0; r < RT; ++r) {
-            const float a = A[(16 * r + li) * LD32 + 4 * ks + lg];
-            part[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, part[r], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < RT; ++r)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[j][r][q] += (double)part[r][q];
-      } else {
-        const uint16_t* B = static_cast<const uint16_t*>(Bv);
-        const uint16_t* A = static_cast<const uint16_t*>(at);
-        const int64_t plane = Npad * Kpad;
-        bf8 b[3];
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl)
-          b[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(B + pl * plane + (int64_t)n * Kpad + k0 + 8 * lg));
-#pragma unroll
-        for (int r = 0; r < RT; ++r) {
-          bf8 a[3];
-#pragma unroll
-          for (int pl = 0; pl < 3; ++pl)
-            a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(A + pl * ROWS * KP16 + (16 * r + li) * KP16 + 8 * lg));
-          f4 p = f4{0.f, 0.f, 0.f, 0.f};
-          p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], p, 0, 0, 0);
-          p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], p, 0, 0, 0);
-          p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], p, 0, 0, 0);
-          p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], p, 0, 0, 0);
-          p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], p, 0, 0, 0);
-          p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], p, 0, 0, 0);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[j][r][q] += (double)p[q];
-        }
+        for (int k = 0; k < 4; ++k) accx[x][k] += (double)q4[k];
       }
     }
   }
-  if (!has_cols) return;
+  // result rows: f32 (bf16) MFMA register k holds row 4 lg + k of the tile
+  auto store = [&](const d4& v, int r, int j) {
+    const int64_t col = (int64_t)(group * G + j) * 16 + li;
+    if (col >= A.N) return;
 #pragma unroll
-  for (int j = 0; j < CT; ++j) {
-    const int64_t col = (int64_t)ctile[j] * 16 + li;
-    if (!has[j] || col >= N) continue;
+    for (int k = 0; k < 4; ++k) {
+      const int row = 16 * r + 4 * lg + k;
+      if (row < 5 * ntp) A.C[(tp0 * 5 + row) * A.ldc + col] = v[k];
+    }
+  };
+  if (has_main)
 #pragma unroll
-    for (int r = 0; r < RT; ++r)
+    for (int r = 0; r < RT; ++r) store(acc[r], r, wave);
+  if constexpr (G == 5)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        // result rows: f64 MFMA register q holds row g + 4q, f32 (and bf16) row 4g + q
-        const int row = 16 * r + (MODE == 0 ? lg + 4 * q : 4 * lg + q);
-        if (row < 5 * ntp) C[(tp0 * 5 + row) * ldc + col] = acc[j][r][q];
-      }
+    for (int x = 0; x < 2; ++x)
+      if (wave + 4 * x < RT) store(accx[x], wave + 4 * x, 4);
+}
+
+template <class K>
+hipError_t launch_kernel(K kernel, const AoMoArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
+  return hipGetLastError();
+}
+
+template <int MODE, int G>
+hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
+  static bool attr = false;
+  const void* fn = MODE == 2 ? reinterpret_cast<const void*>(&k_ao_mo_tile<G>)
+                             : reinterpret_cast<const void*>(&k_ao_mo_cm<MODE == 2 ? 0 : MODE, G>);
+  if (!attr) {
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  if constexpr (MODE == 2) {
+    const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
+    return launch_kernel(k_ao_mo_tile<G>, a, grid, dim3(64 * NWAVE), lds, s);
+  } else {
+    const dim3 grid((unsigned)((a.np + 16 * NWAVE - 1) / (16 * NWAVE)), (unsigned)a.ngroup);
+    return launch_kernel(k_ao_mo_cm<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
+  }
+}
+
+template <int MODE>
+hipError_t launch_mode(const AoMoArgs& a, size_t lds, hipStream_t s) {
+  switch (a.G) {
+    case 1: return launch_g<MODE, 1>(a, lds, s);
+    case 2: return launch_g<MODE, 2>(a, lds, s);
+    case 3: return launch_g<MODE, 3>(a, lds, s);
+    case 4: return launch_g<MODE, 4>(a, lds, s);
+    case 5: return launch_g<MODE, 5>(a, lds, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
 
-size_t ao_mo_lds_bytes(int mode, int ns_max) {
-  const size_t base = sizeof(double) * (3 * TP + TP * ns_max * SH);
-  return base + (mode == 0 ? Tile<0>::bytes : mode == 1 ? Tile<1>::bytes : Tile<2>::bytes);
+size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
+  if (mode == 2)
+    return sizeof(double) * (2 * (size_t)a.desc_max + 3 * TP + (size_t)TP * a.ns_max * SH) +
+           sizeof(uint16_t) * 3 * ROWS * KP16;
+  return sizeof(double) * (2 * (size_t)a.desc_max + (size_t)NWAVE * 16 * rad_stride(a.ns_max)) +
+         2 * (size_t)a.b_chunk_bytes;
 }
 
-hipError_t launch_ao_mo(const AoDevView& v, const double* coord, int64_t coord_ld, int64_t p0, int64_t np, int mode,
-                        const void* B, int64_t Kpad, int64_t Npad, double* C, int64_t ldc, int64_t N, int ns_max,
-                        hipStream_t s) {
-  if (np <= 0) return hipSuccess;
-  if (ns_max < 1 || ns_max > KC) return hipErrorInvalidValue;
-  const int nct = (int)(Npad / 16);
-  const dim3 grid((unsigned)((np + TP - 1) / TP), (unsigned)((nct + 7) / 8));
-  const dim3 block(256);
-  const size_t lds = ao_mo_lds_bytes(mode, ns_max);
-  if (mode == 0)
-    hipLaunchKernelGGL(k_ao_mo<0>, grid, block, lds, s, v, coord, coord_ld, p0, np, B, Kpad, Npad, C, ldc, N, ns_max);
-  else if (mode == 1)
-    hipLaunchKernelGGL(k_ao_mo<1>, grid, block, lds, s, v, coord, coord_ld, p0, np, B, Kpad, Npad, C, ldc, N, ns_max);
-  else
-    hipLaunchKernelGGL(k_ao_mo<2>, grid, block, lds, s, v, coord, coord_ld, p0, np, B, Kpad, Npad, C, ldc, N, ns_max);
-  return hipGetLastError();
+hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s) {
+  if (a.np <= 0) return hipSuccess;
+  if (a.G < 1 || a.G > AO_MO_MAX_G || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
+      a.b_chunk_bytes % 16)
+    return hipErrorInvalidValue;
+  const size_t lds = ao_mo_lds_bytes(a, mode);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  if (mode == 0) return launch_mode<0>(a, lds, s);
+  if (mode == 1) return launch_mode<1>(a, lds, s);
+  return launch_mode<2>(a, lds, s);
 }
 
 }  // namespace qmckl_amd
