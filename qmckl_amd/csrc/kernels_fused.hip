@@ -22,15 +22,18 @@
 // On gfx950 FP64 matrix and vector instructions share one issue budget, and this layout
 // keeps both at their minimum (no operand-tile traffic, balanced tiles).
 //
-// BF16X3 (k_ao_mo_tile) is bound by the vector work of the AO evaluation instead, not by
-// its matrix instructions, and needs more waves per SIMD to hide latency than 25 FP64
+// With more than AO_MO_CM_MAX_G column tiles that layout would evaluate the AOs once per
+// group of 5 column tiles; the tile kernel (k_ao_mo_tile) then serves FP64 and FP32.
+//
+// BF16X3 always runs the tile kernel: it is bound by the vector work of the AO evaluation,
+// not by its matrix instructions, and needs more waves per SIMD to hide latency than 25 FP64
 // accumulator tiles per wave allow.  It keeps the cooperative layout: a workgroup of 4 waves
 // owns 16 points (rows 5 p + c), evaluates their AO VGL for the chunk once into an LDS
-// operand tile, and each wave multiplies it by its column tiles; both operands are three
-// bfloat16 terms (the AO values by an exact truncation split of float(x), C_t by a rounded
-// split of the FP64 value), the six leading cross products (smallest first) go through
-// v_mfma_f32_16x16x32_bf16 into FP32 per chunk and are added into FP64.  It reads the same
-// chunk descriptors (async, LDS) and the same chunked C_t (registers).
+// operand tile, and each wave multiplies it by its column tiles; both BF16X3 operands are
+// three bfloat16 terms (the AO values by an exact truncation split of float(x), C_t by a
+// rounded split of the FP64 value), the six leading cross products (smallest first) go
+// through v_mfma_f32_16x16x32_bf16 into FP32 per chunk and are added into FP64.  It reads
+// the same chunk descriptors (async, LDS) and the chunked C_t of its mode (global, L2).
 #include <hip/hip_runtime.h>
 
 #include "ao_device.hpp"
@@ -250,15 +253,37 @@ k_ao_mo_cm(AoMoArgs A) {
   }
 }
 
-// BF16X3 fused kernel: see the header.  TP = 16 points per workgroup, KC = 32 AOs per chunk.
-// The 5 x G products of a chunk (row tile r, column tile j) go to the waves as: wave w takes
-// column tile w (all 5 row tiles) when w < min(G, 4), and with G = 5 the tiles (r, 4) are
-// spread over the waves by r mod 4, so no wave holds more than 7 accumulator tiles.
+// Tile kernel (BF16X3 always; FP64 and FP32 when the MOs need more than AO_MO_CM_MAX_G
+// column tiles, where one evaluation of the AO tile feeds up to 8 column tiles).  TP = 16
+// points per workgroup (5 row tiles, point-major rows 5 p + c), KC = 32 AOs per chunk.  The
+// 5 x G products of a chunk (row tile r, column tile j) go to the waves as: wave w takes
+// column tiles w and w + 4 (all row tiles) when they exist, except that with G = 5 the tiles
+// (r, 4) are spread over the waves by r mod 4 (at most 7 accumulator tiles per wave).
 constexpr int TP = 16;
 constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row = 5 p + c)
 constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
 constexpr int SH = 8;          // doubles per staged (point, shell): 5 radial VGL + dx, dy, dz
 constexpr int KP16 = AO_MO_BF_LD;
+// Row strides of the FP64 / FP32 operand tiles: the operand reads A[16 r + li][4 s + lg] of
+// a wave then cover the 64 four-byte LDS banks evenly (2 * 36 = 8 mod 64 for doubles;
+// 36 li mod 64 distinct multiples of 4 for floats).
+constexpr int LDA64 = KC + 4;
+constexpr int LDA32 = KC + 4;
+
+template <int MODE>
+struct TileA;
+template <>
+struct TileA<0> {
+  static constexpr size_t bytes = sizeof(double) * ROWS * LDA64;
+};
+template <>
+struct TileA<1> {
+  static constexpr size_t bytes = sizeof(float) * ROWS * LDA32;
+};
+template <>
+struct TileA<2> {
+  static constexpr size_t bytes = sizeof(uint16_t) * 3 * ROWS * KP16;
+};
 
 // six leading cross products of a (3 planes) x b (3 planes), smallest first, into FP32
 __device__ __forceinline__ f4 bf16x3_product(const bf8 (&a)[3], const bf8 (&b)[3]) {
@@ -272,43 +297,70 @@ __device__ __forceinline__ f4 bf16x3_product(const bf8 (&a)[3], const bf8 (&b)[3
   return q;
 }
 
-template <int G>
-__global__ void __launch_bounds__(64 * NWAVE) __attribute__((amdgpu_waves_per_eu(3))) k_ao_mo_tile(AoMoArgs A) {
-  extern __shared__ __attribute__((aligned(16))) double lds[];
+// Product of the operand tile's row tile r with column tile j of the chunk's C_t, added
+// into acc (FP64 for every mode).
+template <int MODE, int G>
+__device__ __forceinline__ void tile_product(d4& acc, const void* at, const void* bchunk, int r, int j, int li,
+                                             int lg) {
   constexpr int NG16 = 16 * G;
-  constexpr int NMAIN = G < NWAVE ? G : NWAVE;  // waves with a main column tile
+  if constexpr (MODE == 0) {
+    const double* A = static_cast<const double*>(at);
+    const double* B = static_cast<const double*>(bchunk);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(16 * r + li) * LDA64 + 4 * ks + lg],
+                                                 B[(4 * ks + lg) * NG16 + 16 * j + li], acc, 0, 0, 0);
+  } else if constexpr (MODE == 1) {
+    const float* A = static_cast<const float*>(at);
+    const float* B = static_cast<const float*>(bchunk);
+    f4 q = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks)
+      q = __builtin_amdgcn_mfma_f32_16x16x4f32(A[(16 * r + li) * LDA32 + 4 * ks + lg],
+                                               B[(4 * ks + lg) * NG16 + 16 * j + li], q, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += (double)q[k];
+  } else {
+    const uint16_t* A = static_cast<const uint16_t*>(at);
+    const uint16_t* B = static_cast<const uint16_t*>(bchunk);
+    bf8 a[3], b[3];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(A + pl * ROWS * KP16 + (16 * r + li) * KP16 + 8 * lg));
+      b[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(B + ((pl * NG16) + 16 * j + li) * KP16 + 8 * lg));
+    }
+    const f4 q = bf16x3_product(a, b);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += (double)q[k];
+  }
+}
+
+template <int MODE, int G>
+__global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : 3) k_ao_mo_tile(AoMoArgs A) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  constexpr int NMAIN = G < NWAVE ? G : NWAVE;  // waves with a first column tile
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   double* dbuf[2] = {lds, lds + A.desc_max};
   double* xt = lds + 2 * A.desc_max;  // [TP][3]
   double* sh = xt + 3 * TP;           // [TP][ns_max][SH]
-  uint16_t* at = reinterpret_cast<uint16_t*>(sh + TP * A.ns_max * SH);  // [3][ROWS][KP16]
+  void* at = sh + TP * A.ns_max * SH;  // operand tile (TileA<MODE>)
   const int group = blockIdx.y;
   const int64_t tp0 = A.p0 + (int64_t)blockIdx.x * TP;
   const int ntp = (int)min((int64_t)TP, A.p0 + A.np - tp0);
-  const uint16_t* bsrc = reinterpret_cast<const uint16_t*>(static_cast<const char*>(A.B) +
-                                                          (int64_t)group * A.nchunk * A.b_chunk_bytes);
-  const int bchunk = A.b_chunk_bytes / 2;  // bf16 elements per chunk
-  const bool has_main = wave < NMAIN;
+  const char* bsrc = static_cast<const char*>(A.B) + (int64_t)group * A.nchunk * A.b_chunk_bytes;
+  const bool has1 = wave < NMAIN, has2 = G > 5 && wave + 4 < G;
   for (int t = tid; t < TP * 3; t += 64 * NWAVE) {
     const int q = t / 3, d = t - 3 * q;
     xt[t] = q < ntp ? A.coord[tp0 + q + d * A.coord_ld] : 0.0;
   }
-  d4 acc[RT], accx[2];  // main column tile; with G = 5, tiles (wave, 4) and (wave + 4, 4)
+  // column tile `wave` (all row tiles); then column tile wave + 4 (G > 5) or the tiles
+  // (wave, 4), (wave + 4, 4) (G = 5)
+  d4 acc1[RT], acc2[G > 5 ? RT : 2];
 #pragma unroll
-  for (int r = 0; r < RT; ++r) acc[r] = d4{0.0, 0.0, 0.0, 0.0};
-  accx[0] = accx[1] = d4{0.0, 0.0, 0.0, 0.0};
-  auto load_b = [&](int ch, int j, bf8 (&b)[3]) {
+  for (int r = 0; r < RT; ++r) acc1[r] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl)
-      b[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(bsrc + (int64_t)ch * bchunk +
-                                                                   ((pl * NG16) + 16 * j + li) * KP16 + 8 * lg));
-  };
-  auto load_a = [&](int r, bf8 (&a)[3]) {
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl)
-      a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(at + pl * ROWS * KP16 + (16 * r + li) * KP16 + 8 * lg));
-  };
+  for (int r = 0; r < (G > 5 ? RT : 2); ++r) acc2[r] = d4{0.0, 0.0, 0.0, 0.0};
   lds_copy(dbuf[0], A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
   for (int ch = 0; ch < A.nchunk; ++ch) {
     __builtin_amdgcn_s_waitcnt(0);
@@ -330,67 +382,67 @@ __global__ void __launch_bounds__(64 * NWAVE) __attribute__((amdgpu_waves_per_eu
       for (int k = 0; k < 5; ++k) o[k] = r[k];
     }
     __syncthreads();
-    // 2. AO VGL of the chunk into the operand tile (three bf16 planes), row = 5 p + c
+    // 2. AO VGL of the chunk into the operand tile, row = 5 p + c
     for (int t = tid; t < TP * KC; t += 64 * NWAVE) {
       const int q = t / KC, kk = t - q * KC;
       double r[5];
       chunk_ao_pt(c, kk, xt[3 * q], xt[3 * q + 1], xt[3 * q + 2], sh + q * A.ns_max * SH, SH, r);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        uint16_t h, m, l;
-        split3_trunc(r[k], h, m, l);
-        const int idx = (5 * q + k) * KP16 + kk;
-        at[idx] = h;
-        at[ROWS * KP16 + idx] = m;
-        at[2 * ROWS * KP16 + idx] = l;
+        if constexpr (MODE == 0) {
+          static_cast<double*>(at)[(5 * q + k) * LDA64 + kk] = r[k];
+        } else if constexpr (MODE == 1) {
+          static_cast<float*>(at)[(5 * q + k) * LDA32 + kk] = (float)r[k];
+        } else {
+          uint16_t h, m, l;
+          split3_trunc(r[k], h, m, l);
+          uint16_t* T = static_cast<uint16_t*>(at);
+          const int idx = (5 * q + k) * KP16 + kk;
+          T[idx] = h;
+          T[ROWS * KP16 + idx] = m;
+          T[2 * ROWS * KP16 + idx] = l;
+        }
       }
     }
     __syncthreads();
-    // 3. products of the chunk on matrix cores
-    if (has_main) {
-      bf8 b[3];
-      load_b(ch, wave, b);
+    // 3. products of the chunk on matrix cores (C_t read from global memory, where the
+    //    chunk's rows stay in L2 for every workgroup)
+    const void* bch = bsrc + (int64_t)ch * A.b_chunk_bytes;
+    if (has1)
 #pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        bf8 a[3];
-        load_a(r, a);
-        const f4 q4 = bf16x3_product(a, b);
+      for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc1[r], at, bch, r, wave, li, lg);
+    if constexpr (G > 5) {
+      if (has2)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc[r][k] += (double)q4[k];
-      }
-    }
-    if constexpr (G == 5) {
-      bf8 b[3];
-      load_b(ch, 4, b);
+        for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc2[r], at, bch, r, wave + 4, li, lg);
+    } else if constexpr (G == 5) {
 #pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        const int r = wave + 4 * x;
-        if (r >= RT) continue;
-        bf8 a[3];
-        load_a(r, a);
-        const f4 q4 = bf16x3_product(a, b);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) accx[x][k] += (double)q4[k];
-      }
+      for (int x = 0; x < 2; ++x)
+        if (wave + 4 * x < RT) tile_product<MODE, G>(acc2[x], at, bch, wave + 4 * x, 4, li, lg);
     }
   }
-  // result rows: f32 (bf16) MFMA register k holds row 4 lg + k of the tile
+  // result rows: f64 MFMA register k holds row lg + 4 k of the tile, f32 (and bf16) 4 lg + k
   auto store = [&](const d4& v, int r, int j) {
     const int64_t col = (int64_t)(group * G + j) * 16 + li;
     if (col >= A.N) return;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int row = 16 * r + 4 * lg + k;
+      const int row = 16 * r + (MODE == 0 ? lg + 4 * k : 4 * lg + k);
       if (row < 5 * ntp) A.C[(tp0 * 5 + row) * A.ldc + col] = v[k];
     }
   };
-  if (has_main)
+  if (has1)
 #pragma unroll
-    for (int r = 0; r < RT; ++r) store(acc[r], r, wave);
-  if constexpr (G == 5)
+    for (int r = 0; r < RT; ++r) store(acc1[r], r, wave);
+  if constexpr (G > 5) {
+    if (has2)
+#pragma unroll
+      for (int r = 0; r < RT; ++r) store(acc2[r], r, wave + 4);
+  } else if constexpr (G == 5) {
 #pragma unroll
     for (int x = 0; x < 2; ++x)
-      if (wave + 4 * x < RT) store(accx[x], wave + 4 * x, 4);
+      if (wave + 4 * x < RT) store(acc2[x], wave + 4 * x, 4);
+  }
 }
 
 template <class K>
@@ -401,21 +453,30 @@ hipError_t launch_kernel(K kernel, const AoMoArgs& a, dim3 grid, dim3 block, siz
 
 template <int MODE, int G>
 hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
+  const bool tile = MODE == 2 || a.tile;
+  if constexpr (MODE != 2 && G <= AO_MO_CM_MAX_G) {
+    if (!tile) {
+      static bool attr = false;
+      if (!attr) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ao_mo_cm<MODE, G>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+      }
+      const dim3 grid((unsigned)((a.np + 16 * NWAVE - 1) / (16 * NWAVE)), (unsigned)a.ngroup);
+      return launch_kernel(k_ao_mo_cm<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
+    }
+  }
+  if (!tile) return hipErrorInvalidValue;
   static bool attr = false;
-  const void* fn = MODE == 2 ? reinterpret_cast<const void*>(&k_ao_mo_tile<G>)
-                             : reinterpret_cast<const void*>(&k_ao_mo_cm<MODE == 2 ? 0 : MODE, G>);
   if (!attr) {
-    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ao_mo_tile<MODE, G>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     attr = true;
   }
-  if constexpr (MODE == 2) {
-    const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
-    return launch_kernel(k_ao_mo_tile<G>, a, grid, dim3(64 * NWAVE), lds, s);
-  } else {
-    const dim3 grid((unsigned)((a.np + 16 * NWAVE - 1) / (16 * NWAVE)), (unsigned)a.ngroup);
-    return launch_kernel(k_ao_mo_cm<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
-  }
+  const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
+  return launch_kernel(k_ao_mo_tile<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
 }
 
 template <int MODE>
@@ -426,6 +487,9 @@ hipError_t launch_mode(const AoMoArgs& a, size_t lds, hipStream_t s) {
     case 3: return launch_g<MODE, 3>(a, lds, s);
     case 4: return launch_g<MODE, 4>(a, lds, s);
     case 5: return launch_g<MODE, 5>(a, lds, s);
+    case 6: return launch_g<MODE, 6>(a, lds, s);
+    case 7: return launch_g<MODE, 7>(a, lds, s);
+    case 8: return launch_g<MODE, 8>(a, lds, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -433,17 +497,17 @@ hipError_t launch_mode(const AoMoArgs& a, size_t lds, hipStream_t s) {
 }  // namespace
 
 size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
-  if (mode == 2)
+  if (mode == 2 || a.tile)
     return sizeof(double) * (2 * (size_t)a.desc_max + 3 * TP + (size_t)TP * a.ns_max * SH) +
-           sizeof(uint16_t) * 3 * ROWS * KP16;
+           (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
   return sizeof(double) * (2 * (size_t)a.desc_max + (size_t)NWAVE * 16 * rad_stride(a.ns_max)) +
          2 * (size_t)a.b_chunk_bytes;
 }
 
 hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s) {
   if (a.np <= 0) return hipSuccess;
-  if (a.G < 1 || a.G > AO_MO_MAX_G || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
-      a.b_chunk_bytes % 16)
+  if (a.G < 1 || a.G > AO_MO_MAX_G || (!a.tile && mode != 2 && a.G > AO_MO_CM_MAX_G) || a.nchunk < 1 ||
+      a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 || a.b_chunk_bytes % 16)
     return hipErrorInvalidValue;
   const size_t lds = ao_mo_lds_bytes(a, mode);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
