@@ -89,8 +89,12 @@ __host__ __device__ __forceinline__ int gn_jp(int64_t Ne) { return (int)((Ne + 3
 // e-e and e-n terms, and the q = 0 sums of the e-e-n tensor.
 // R0 scratch layout: [W][Ne][cord-1][5]; gN: see gn_index.
 // ------------------------------------------------------------------------------------
+// C (= cord, 0..8) is a template parameter so that only the C - 1 orders of the q = 0 sums
+// are held in registers.
+template <int C>
 __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
                                                      double* __restrict__ gN, double* __restrict__ Sg) {
+  constexpr int K1 = C >= 2 ? C - 1 : 0;
   extern __shared__ double lds[];
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
   const int NAg = gn_na(Nn), JPg = gn_jp(Ne);
@@ -103,21 +107,24 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
 
   const double kee = v.rescale_ee, kinv = 1.0 / v.rescale_ee;
   const double* b = v.b_vector;
-  const int bord = (int)v.bord, aord = (int)v.aord, C = (int)v.cord;
+  const int bord = (int)v.bord, aord = (int)v.aord;
   const int up = (int)v.up_num;
   double fee = 0.0, fen = 0.0;
   for (int i = threadIdx.x; i < Ne; i += kBlock) {
     const double xi = xe[3 * i], yi = xe[3 * i + 1], zi = xe[3 * i + 2];
     double g0 = 0, g1 = 0, g2 = 0, g3 = 0;
-    double r0[7][5];
+    double r0[K1 > 0 ? K1 : 1][5];
 #pragma unroll
-    for (int k = 0; k < 7; ++k)
+    for (int k = 0; k < (K1 > 0 ? K1 : 1); ++k)
 #pragma unroll
       for (int c = 0; c < 5; ++c) r0[k][c] = 0.0;
+    // No branch on j == i: for j == i the distance is replaced by 1 and every gradient,
+    // Laplacian and e-e-n term is multiplied by an exact zero (dx = dy = dz = 0,
+    // ed = em = 0), which leaves the sums unchanged; the value term only takes j < i.
     for (int j = 0; j < Ne; ++j) {
-      if (j == i) continue;
+      const bool use = j != i;
       const double dx = xi - xe[3 * j], dy = yi - xe[3 * j + 1], dz = zi - xe[3 * j + 2];
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      const double d = use ? sqrt(dx * dx + dy * dy + dz * dz) : 1.0;
       const double e = exp(-kee * d);
       const double x = (1.0 - e) * kinv;  // rescaled distance, org/qmckl_distance.org:975-985
       // value: the pair (j, i) with j < i, as in the doc loop over j > i' of the reference
@@ -138,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
       // gradient/Laplacian w.r.t. electron i, org/qmckl_distance.org:1440-1452
       const double dd = d > 1.e-20 ? d : 1.e-20;
       const double dinv = 1.0 / dd;
-      const double ed = (d > 1.e-20) ? e : exp(-kee * dd);
+      const double ed = use ? ((d > 1.e-20) ? e : exp(-kee * dd)) : 0.0;
       const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
       const double gl4 = (2.0 * dinv - kee) * ed;
       const double denom = 1.0 + b[1] * x;
@@ -168,11 +175,11 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
       // a-independent e-e-n sums (k >= 1); g_e = exp(-kappa r), diagonal excluded
       const double rinv = 1.0 / d;
       const double ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
+      const double em = use ? e : 0.0;
       double gk = 1.0;
 #pragma unroll
-      for (int k = 1; k < 8; ++k) {
-        if (k >= C) break;
-        gk *= e;
+      for (int k = 1; k <= K1; ++k) {
+        gk *= em;
         r0[k - 1][0] += gk;
         r0[k - 1][1] += gk * ux;
         r0[k - 1][2] += gk * uy;
@@ -185,14 +192,12 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
     og[i + Ne] = g1;
     og[i + 2 * Ne] = g2;
     og[i + 3 * Ne] = g3;
-    if (C >= 2) {
-      double* rr = R0 + ((w * Ne + i) * (C - 1)) * 5;
+    if constexpr (K1 > 0) {
+      double* rr = R0 + ((w * Ne + i) * K1) * 5;
 #pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        if (k >= C - 1) break;
+      for (int k = 0; k < K1; ++k)
 #pragma unroll
         for (int c = 0; c < 5; ++c) rr[k * 5 + c] = r0[k][c];
-      }
     }
 
     // e-n term, org/qmckl_jastrow_champ.org:7021-7095 and :7500-7590
@@ -250,21 +255,24 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   }
   // Sg[w][a][q-1] = sum_i g_a(r_i)^q, q = 1..cord-1: the k = 0 sums of the e-e-n term over
   // all electrons (the barriers of block_sum make this block's gN writes visible).
-  if (Sg != nullptr && C >= 2) {
+  if constexpr (K1 > 0) {
     for (int a = threadIdx.x; a < Nn; a += kBlock) {
-      double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      double acc[K1];
+#pragma unroll
+      for (int q = 0; q < K1; ++q) acc[q] = 0.0;
       const double* ga = gN + gn_index(w, a, 0, NAg, JPg);
 #pragma unroll 4
       for (int i = 0; i < Ne; ++i) {
         const double g = ga[16 * i];
         double p = g;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) {
+        for (int q = 0; q < K1; ++q) {
           acc[q] += p;
           p *= g;
         }
       }
-      for (int q = 0; q < C - 1 && q < 7; ++q) Sg[(w * Nn + a) * (C - 1) + q] = acc[q];
+#pragma unroll
+      for (int q = 0; q < K1; ++q) Sg[(w * Nn + a) * K1 + q] = acc[q];
     }
   }
   // zero padding of gN: nuclei Nn .. 16 NA - 1 for every j, electrons Ne .. JP - 1 for a < Nn
@@ -1572,8 +1580,25 @@ hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t
   const ScratchView sv = scratch_view(v, o);
   const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kBlock);
   if (lds_pair > 160 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_jas_pair, dim3((unsigned)v.walk_num), dim3(kBlock), lds_pair, s, v, o, sv.R0, sv.gN,
-                     v.cord >= 2 ? sv.Sg : nullptr);
+  switch (v.cord) {
+#define PAIR_CASE(CC)                                                                                         \
+  case CC:                                                                                                    \
+    hipLaunchKernelGGL(k_jas_pair<CC>, dim3((unsigned)v.walk_num), dim3(kBlock), lds_pair, s, v, o, sv.R0, sv.gN, \
+                       sv.Sg);                                                                                \
+    break;
+    PAIR_CASE(0)
+    PAIR_CASE(1)
+    PAIR_CASE(2)
+    PAIR_CASE(3)
+    PAIR_CASE(4)
+    PAIR_CASE(5)
+    PAIR_CASE(6)
+    PAIR_CASE(7)
+    PAIR_CASE(8)
+#undef PAIR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
