@@ -150,9 +150,10 @@ k_ao_mo_cm(AoMoArgs A) {
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int RS = rad_stride(A.ns_max);
-  double* dbuf[2] = {lds, lds + A.desc_max};
-  char* bbuf[2] = {reinterpret_cast<char*>(lds + 2 * A.desc_max),
-                   reinterpret_cast<char*>(lds + 2 * A.desc_max) + A.b_chunk_bytes};
+  // LDS: descriptor buffers [2][desc_max], C_t chunk buffers [2][b_chunk_bytes], radial
+  // staging; buffer addresses are computed from `lds` (never picked from an array of
+  // pointers) so that every access compiles to an LDS instruction
+  char* bbase = reinterpret_cast<char*>(lds + 2 * A.desc_max);
   double* rad = lds + 2 * A.desc_max + A.b_chunk_bytes / 4 + wave * 16 * RS;  // 2 b buffers = b/4 doubles
   const int group = blockIdx.y;
   const int64_t p = A.p0 + (int64_t)blockIdx.x * (16 * NWAVE) + 16 * wave + li;  // the lane's point
@@ -168,18 +169,19 @@ k_ao_mo_cm(AoMoArgs A) {
 #pragma unroll
     for (int j = 0; j < G; ++j) acc[r][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  lds_copy(dbuf[0], A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
-  lds_copy(bbuf[0], bsrc, A.b_chunk_bytes, wave, lane);
+  lds_copy(lds, A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
+  lds_copy(bbase, bsrc, A.b_chunk_bytes, wave, lane);
   for (int ch = 0; ch < A.nchunk; ++ch) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();  // chunk ch in LDS; every wave done with chunk ch - 1
     if (ch + 1 < A.nchunk) {
       const int nb = (ch + 1) & 1;
-      lds_copy(dbuf[nb], A.desc + A.desc_off[ch + 1], 8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave, lane);
-      lds_copy(bbuf[nb], bsrc + (int64_t)(ch + 1) * A.b_chunk_bytes, A.b_chunk_bytes, wave, lane);
+      lds_copy(lds + nb * A.desc_max, A.desc + A.desc_off[ch + 1], 8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]),
+               wave, lane);
+      lds_copy(bbase + nb * A.b_chunk_bytes, bsrc + (int64_t)(ch + 1) * A.b_chunk_bytes, A.b_chunk_bytes, wave, lane);
     }
-    const Chunk c = chunk_view(dbuf[ch & 1]);
-    const char* bl = bbuf[ch & 1];
+    const Chunk c = chunk_view(lds + (ch & 1) * A.desc_max);
+    const char* bl = bbase + (ch & 1) * A.b_chunk_bytes;
     // 1. radial VGL of the chunk's shells at the wave's points: lane (li, lg) takes shells
     //    lg, lg + 4, ... at point li
     for (int s = lg; s < c.ns; s += 4) {
@@ -341,7 +343,8 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : 3) k_ao_mo_tile(AoMoAr
   constexpr int NMAIN = G < NWAVE ? G : NWAVE;  // waves with a first column tile
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  double* dbuf[2] = {lds, lds + A.desc_max};
+  // LDS: descriptor buffers [2][desc_max] (addresses computed from `lds`, so that every
+  // access compiles to an LDS instruction), then the points, radial staging and tile
   double* xt = lds + 2 * A.desc_max;  // [TP][3]
   double* sh = xt + 3 * TP;           // [TP][ns_max][SH]
   void* at = sh + TP * A.ns_max * SH;  // operand tile (TileA<MODE>)
@@ -361,14 +364,14 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : 3) k_ao_mo_tile(AoMoAr
   for (int r = 0; r < RT; ++r) acc1[r] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int r = 0; r < (G > 5 ? RT : 2); ++r) acc2[r] = d4{0.0, 0.0, 0.0, 0.0};
-  lds_copy(dbuf[0], A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
+  lds_copy(lds, A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
   for (int ch = 0; ch < A.nchunk; ++ch) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();  // descriptor ch in LDS; every wave done with chunk ch - 1
     if (ch + 1 < A.nchunk)
-      lds_copy(dbuf[(ch + 1) & 1], A.desc + A.desc_off[ch + 1], 8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave,
-               lane);
-    const Chunk c = chunk_view(dbuf[ch & 1]);
+      lds_copy(lds + ((ch + 1) & 1) * A.desc_max, A.desc + A.desc_off[ch + 1],
+               8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave, lane);
+    const Chunk c = chunk_view(lds + (ch & 1) * A.desc_max);
     // 1. radial VGL of the chunk's shells at the 16 points
     for (int t = tid; t < TP * c.ns; t += 64 * NWAVE) {
       const int q = t / c.ns, s = t - q * c.ns;
