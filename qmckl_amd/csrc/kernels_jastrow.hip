@@ -503,6 +503,25 @@ __global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, cons
 // (sums over all j from k_jas_pair less the j = i term) by the waves with k = 1.
 // ------------------------------------------------------------------------------------
 constexpr int EM = 16;  // electrons i per workgroup
+
+constexpr int NXCD = 8;  // accelerator complex dies of the MI355X, each with its own L2
+
+// Workgroup -> (walker, electron tile).  Workgroups are dealt to the XCDs round-robin, so
+// the tiles of a walker are given to workgroups b, b + 8, b + 16, ... of one XCD, which then
+// shares the walker's g_a(r_j) table in its L2; walkers past the last full group of 8 keep
+// the plain order.
+__device__ __forceinline__ void een_block_work(int64_t b, int64_t W, int ntiles, int64_t& w, int& tile) {
+  const int64_t full = (W / NXCD) * NXCD * ntiles;
+  if (b < full) {
+    const int64_t x = b % NXCD, s = b / NXCD;
+    w = NXCD * (s / ntiles) + x;
+    tile = (int)(s % ntiles);
+  } else {
+    w = b / ntiles;
+    tile = (int)(b - w * ntiles);
+  }
+}
+
 constexpr int EMFMA_MAX_C = 6;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -687,8 +706,9 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
   double* tot = lds + L.tot;
   double* jrow = lds + L.jrow;
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
-  const int64_t w = blockIdx.x / ntiles;
-  const int tile = (int)(blockIdx.x - w * ntiles);
+  int64_t w;
+  int tile;
+  een_block_work(blockIdx.x, v.walk_num, ntiles, w, tile);
   const int i0 = tile * EM;
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (k, q) role
@@ -1151,8 +1171,9 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   double* st = lds + L.st;
   double* cfl = lds + L.cfl;
   double* red = lds + L.red;
-  const int64_t w = blockIdx.x / ntiles;
-  const int tile = (int)(blockIdx.x - w * ntiles);
+  int64_t w;
+  int tile;
+  een_block_work(blockIdx.x, v.walk_num, ntiles, w, tile);
   const int i0 = tile * EM;
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);

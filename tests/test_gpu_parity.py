@@ -230,8 +230,10 @@ def test_jastrow_n2_vs_oracle():
     q.context_destroy(ctx)
 
 
-@pytest.mark.parametrize("name,W", [("small", 256), ("medium", 8)])
+@pytest.mark.parametrize("name,W", [("small", 256), ("medium", 8), ("small", 13)])
 def test_jastrow_synthetic_vs_oracle(name, W):
+    """13 walkers: the first 8 take the e-e-n kernel's XCD-grouped workgroup order, the
+    last 5 the plain order."""
     s, x, W = S.workload(name, walk_num=W)
     ctx = make_ctx(s, x, W, mo=False)
     got = _jastrow_gpu(ctx, W, s.elec_num)
