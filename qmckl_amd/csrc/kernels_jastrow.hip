@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -1046,12 +1047,20 @@ constexpr int een_single_q(int C, int p) { return C - een_single_k(C, p); }
 
 // The contraction of group (K, Q .. Q + NQ - 1) over nsteps x 4 electrons j; lane (li, lg)
 // feeds A(i = li, j = 4 s + lg) and B(j = 4 s + lg, a = li) of v_mfma_f64_16x16x4_f64.
-template <int K, int Q, int NQ>
-__device__ __forceinline__ void een_res_pass(d4 (&acc)[NQ][5], const double* jr, const double* gt,
+// MINI: the last nucleus tile holds at most 4 nuclei and the products run on
+// v_mfma_f64_4x4x4f64 (4 blocks), whose lane l = 16 m + 4 b + n holds A(block b, row n,
+// k = m), B(b, k = m, column n) and D(b, row m, column n): with the blocks as the four
+// groups of 4 electrons, a lane's A element is the same A(i = li, j = 4 s + lg), its B element
+// is B(j = 4 s + lg, a = li mod 4), and its one result is row lg + 4 (li / 4), column li mod 4.
+template <bool MINI>
+using EenAcc = typename std::conditional<MINI, double, d4>::type;
+
+template <int K, int Q, int NQ, bool MINI>
+__device__ __forceinline__ void een_res_pass(EenAcc<MINI> (&acc)[NQ][5], const double* jr, const double* gt,
                                              const double (&xi)[3], int nsteps, int li, int lg) {
   const double* pj = jr + lg * JREC;
   const double* pi = pj + li;
-  const double* pg = gt + lg * 16 + li;
+  const double* pg = gt + lg * 16 + (MINI ? (li & 3) : li);
   for (int st = 0; st < nsteps; ++st) {
     const double Ge = pi[0], ri = pi[16], g = pg[0];
     const double ux = (xi[0] - pj[32]) * ri, uy = (xi[1] - pj[33]) * ri, uz = (xi[2] - pj[34]) * ri;
@@ -1069,7 +1078,12 @@ __device__ __forceinline__ void een_res_pass(d4 (&acc)[NQ][5], const double* jr,
     for (int n = 0; n < NQ; ++n) {
       if (n > 0) gq *= g;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) acc[n][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, acc[n][c], 0, 0, 0);
+      for (int c = 0; c < 5; ++c) {
+        if constexpr (MINI)
+          acc[n][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[c], gq, acc[n][c], 0, 0, 0);
+        else
+          acc[n][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, acc[n][c], 0, 0, 0);
+      }
     }
   }
 }
@@ -1082,71 +1096,103 @@ struct EenResCtx {
   int nsteps, li, lg, i0, Ne, a, NA16;
 };
 
+// Fold of row r of the products (X, 5 components per unit of the group) of lane column
+// x.a into out: the role sums of each unit applied to its products, and with q = 1 or k = 1
+// the roles on the q = 0 and k = 0 sums.  gcol is the lane's column in the tile.
 template <int C, int K, int Q, int NQ>
-__device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
+__device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int gcol, const double (&X)[NQ][5],
+                                                 double (&out)[5]) {
   constexpr int K1 = C - 1;
-  d4 acc[NQ][5];
+  const int i = min(x.i0 + il, x.Ne - 1);
+  double gp[C + 1];
+  gp[0] = 1.0;
+  gp[1] = x.gt[i * 16 + gcol];  // g_a(r_i)
+#pragma unroll
+  for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * gp[1];
+  const double rinv = x.enr[il * x.NA16 + x.a];
+  EenEp e;
+  e.h0 = -x.ka * ((x.xt[il * 3] - x.nx) * rinv);
+  e.h1 = -x.ka * ((x.xt[il * 3 + 1] - x.ny) * rinv);
+  e.h2 = -x.ka * ((x.xt[il * 3 + 2] - x.nz) * rinv);
+  e.kr = 2.0 * x.ka * rinv;
+  e.ka2 = x.ka * x.ka;
+  const double kk = (double)K * x.kee;
+#pragma unroll
+  for (int n = 0; n < NQ; ++n) {
+    EenTs tm, tr, tz;
+    if (n == 0)
+      een_tsums<C, K, Q>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
+    else
+      een_tsums<C, K, (NQ > 1 ? Q + 1 : Q)>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
+    een_apply<false>(out, tm, X[n], kk, e);
+    if (Q + n == 1) {  // X[K][0], the q = 0 sums of the tile electrons
+      const double* XR = x.r0t + il * K1 * 5 + (K - 1) * 5;
+      een_apply<false>(out, tr, XR, kk, e);
+    }
+    if (K == 1) {
+      // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
+      const double X0[1] = {x.st[x.a * K1 + Q + n - 1] - __builtin_fma(gp[Q + n - 1], gp[1], 0.0)};
+      een_apply<true>(out, tz, X0, 0.0, e);
+    }
+  }
+}
+
+// One group: contraction, then the fold.  Full tile: lane (li, lg) holds rows lg + 4 r
+// (register r of each accumulator) of column a = 16 at + li, folded into ra[r].  MINI: one
+// row lg + 4 (li / 4) of column a = 16 at + li mod 4 per lane, folded into rm.  No branch on
+// the row or the nucleus, so the rows interleave: a padding nucleus has zero c-coefficients,
+// kappa, 1/R, products and k = 0 sums and adds exactly 0; a row past the last electron is
+// discarded by the caller.
+template <int C, int K, int Q, int NQ, bool MINI>
+__device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5],
+                                              double (&rm)[5]) {
+  EenAcc<MINI> acc[NQ][5];
 #pragma unroll
   for (int n = 0; n < NQ; ++n)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) acc[n][c] = d4{0.0, 0.0, 0.0, 0.0};
-  een_res_pass<K, Q, NQ>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
-  // No branch on the row or the nucleus, so the four rows interleave: a padding nucleus has
-  // zero c-coefficients, kappa, 1/R, products and k = 0 sums and adds exactly 0; a row past
-  // the last electron is discarded by the caller.
+    for (int c = 0; c < 5; ++c) {
+      if constexpr (MINI)
+        acc[n][c] = 0.0;
+      else
+        acc[n][c] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  een_res_pass<K, Q, NQ, MINI>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+  if constexpr (MINI) {
+    double X[NQ][5];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int il = x.lg + 4 * r, i = min(x.i0 + il, x.Ne - 1);
-    {
-      double gp[C + 1];
-      gp[0] = 1.0;
-      gp[1] = x.gt[i * 16 + x.li];  // g_a(r_i)
+    for (int n = 0; n < NQ; ++n)
 #pragma unroll
-      for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * gp[1];
-      const double rinv = x.enr[il * x.NA16 + x.a];
-      EenEp e;
-      e.h0 = -x.ka * ((x.xt[il * 3] - x.nx) * rinv);
-      e.h1 = -x.ka * ((x.xt[il * 3 + 1] - x.ny) * rinv);
-      e.h2 = -x.ka * ((x.xt[il * 3 + 2] - x.nz) * rinv);
-      e.kr = 2.0 * x.ka * rinv;
-      e.ka2 = x.ka * x.ka;
-      const double kk = (double)K * x.kee;
+      for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c];
+    een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, rm);
+  } else {
 #pragma unroll
-      for (int n = 0; n < NQ; ++n) {
-        const double X[5] = {acc[n][0][r], acc[n][1][r], acc[n][2][r], acc[n][3][r], acc[n][4][r]};
-        EenTs tm, tr, tz;
-        if (n == 0)
-          een_tsums<C, K, Q>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
-        else
-          een_tsums<C, K, (NQ > 1 ? Q + 1 : Q)>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
-        een_apply<false>(ra[r], tm, X, kk, e);
-        if (Q + n == 1) {  // X[K][0], the q = 0 sums of the tile electrons
-          const double* XR = x.r0t + il * K1 * 5 + (K - 1) * 5;
-          een_apply<false>(ra[r], tr, XR, kk, e);
-        }
-        if (K == 1) {
-          // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
-          const double X0[1] = {x.st[x.a * K1 + Q + n - 1] - __builtin_fma(gp[Q + n - 1], gp[1], 0.0)};
-          een_apply<true>(ra[r], tz, X0, 0.0, e);
-        }
-      }
+    for (int r = 0; r < 4; ++r) {
+      double X[NQ][5];
+#pragma unroll
+      for (int n = 0; n < NQ; ++n)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c][r];
+      een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * r, x.li, X, ra[r]);
     }
   }
 }
 
 template <int C, int G = 0>
-__device__ __forceinline__ void een_res_dispatch(int g, const EenResCtx& x, const double (&xi)[3],
-                                                 double (&ra)[4][5]) {
+__device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResCtx& x, const double (&xi)[3],
+                                                 double (&ra)[4][5], double (&rm)[5]) {
   constexpr int NPR = een_npair(C);
   if constexpr (G < NPR + een_nsingle(C)) {
     if (g == G) {
-      if constexpr (G < NPR)
-        een_res_group<C, een_pair_k(C, G), een_pair_q(C, G), 2>(x, xi, ra);
+      constexpr int K = G < NPR ? een_pair_k(C, G) : een_single_k(C, G - NPR);
+      constexpr int Q = G < NPR ? een_pair_q(C, G) : een_single_q(C, G - NPR);
+      constexpr int NQ = G < NPR ? 2 : 1;
+      if (mini)
+        een_res_group<C, K, Q, NQ, true>(x, xi, ra, rm);
       else
-        een_res_group<C, een_single_k(C, G - NPR), een_single_q(C, G - NPR), 1>(x, xi, ra);
+        een_res_group<C, K, Q, NQ, false>(x, xi, ra, rm);
       return;
     }
-    een_res_dispatch<C, G + 1>(g, x, xi, ra);
+    een_res_dispatch<C, G + 1>(g, mini, x, xi, ra, rm);
   }
 }
 
@@ -1226,15 +1272,19 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   }
   __syncthreads();
 
-  // 3. groups of units, (group, nucleus tile) items dealt longest first: every pair item,
-  // then every single item, each to the least loaded wave (lowest index on ties).  Every
-  // wave runs the same scalar schedule and processes the items it owns.
+  // 3. groups of units dealt longest first to the least loaded wave (lowest index on ties):
+  // the pairs of every full nucleus tile, then its single units, then, when the last tile
+  // holds at most 4 nuclei, its pairs and single units on the 4x4x4 products (about a
+  // third of the work of a full-tile unit).  Every wave runs the same scalar schedule and
+  // processes the items it owns.
   const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
-  double ra[4][5];
+  double ra[4][5], rm[5];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) rm[c] = 0.0;
   EenResCtx x;
   x.jr = jr;
   x.xt = xt;
@@ -1250,29 +1300,45 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   x.Ne = Ne;
   x.NA16 = NA16;
   constexpr int NPR = een_npair(C), NSG = een_nsingle(C);
+  const bool mini = Nn - 16 * (NA - 1) <= 4;  // nuclei in the last tile
+  const int NAF = mini ? NA - 1 : NA;         // tiles on the 16x16x4 products
+  const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n2 = n1 + (mini ? NPR : 0), n3 = n2 + (mini ? NSG : 0);
   int load[EQ_WAVES];
 #pragma unroll
   for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
-  const int nitems = (NPR + NSG) * NA;
-  for (int it = 0; it < nitems; ++it) {
-    const bool pair = it < NPR * NA;
-    const int r = pair ? it : it - NPR * NA, per = pair ? NPR : NSG;
-    const int at = r / per, g = (pair ? 0 : NPR) + r - at * per;
+  for (int it = 0; it < n3; ++it) {
+    int at, g, weight;
+    if (it < n0) {
+      at = it / NPR, g = it - at * NPR, weight = 12;
+    } else if (it < n1) {
+      const int r = it - n0;
+      at = r / NSG, g = NPR + r - at * NSG, weight = 6;
+    } else if (it < n2) {
+      at = NA - 1, g = it - n1, weight = 4;
+    } else {
+      at = NA - 1, g = NPR + it - n2, weight = 2;
+    }
     int owner = 0;
 #pragma unroll
     for (int wv = 1; wv < EQ_WAVES; ++wv)
       if (load[wv] < load[owner]) owner = wv;
 #pragma unroll
-    for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] += (wv == owner) ? (pair ? 2 : 1) : 0;
+    for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] += (wv == owner) ? weight : 0;
     if (owner != wave) continue;
+    const bool mu = it >= n1;  // on the 4x4x4 products
     x.gt = gch + at * JP * 16;
-    x.a = 16 * at + li;
+    x.a = 16 * at + (mu ? (li & 3) : li);
     x.nx = nuc[x.a];
     x.ny = nuc[NA16 + x.a];
     x.nz = nuc[2 * NA16 + x.a];
     x.ka = nuc[3 * NA16 + x.a];
-    een_res_dispatch<C>(g, x, xi, ra);
+    een_res_dispatch<C>(g, mu, x, xi, ra, rm);
   }
+  // the 4x4x4 results belong to row lg + 4 (li / 4)
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ra[r][c] += (r == (li >> 2)) ? rm[c] : 0.0;
   // 4. sums over the nuclei (lanes li) and the waves
 #pragma unroll
   for (int r = 0; r < 4; ++r)
