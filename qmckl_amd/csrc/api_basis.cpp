@@ -226,40 +226,27 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     d.mo_pad = pad;
     d.k_pad = kpad;
     up(d.coef_t, ct.data(), 8 * ct.size());
-    // the operand layouts of the fused kernels: per (group of G column tiles, 32-AO chunk).
-    // FP64 / FP32: the component-major kernel when every column tile fits one pass, else
-    // the tile kernel with up to AO_MO_MAX_G tiles per pass; BF16X3: the tile kernel.
+    // the operand layouts of the fused kernel: per (group of G column tiles, 32-AO chunk)
     const int64_t nct = pad / 16, nchunk = kpad / AO_MO_KC;
-    const bool cm = nct <= AO_MO_CM_MAX_G;
-    const int Gd = cm ? (int)nct : (int)std::min<int64_t>(AO_MO_MAX_G, nct);
-    const int Gb = (int)std::min<int64_t>(AO_MO_MAX_G, nct);
-    const int64_t ngd = (nct + Gd - 1) / Gd, ngb = (nct + Gb - 1) / Gb;
-    for (int m = 0; m < 2; ++m) {
-      d.ao_mo_g[m] = Gd;
-      d.ao_mo_ngroup[m] = (int)ngd;
-      d.ao_mo_tile[m] = cm ? 0 : 1;
-    }
-    d.ao_mo_g[2] = Gb;
-    d.ao_mo_ngroup[2] = (int)ngb;
-    d.ao_mo_tile[2] = 1;
-    const int64_t NGd = 16 * Gd, NGb = 16 * Gb;
-    std::vector<double> b64((size_t)(ngd * nchunk) * AO_MO_KC * NGd, 0.0);
+    const int G = (int)std::min<int64_t>(AO_MO_MAX_G, nct);
+    const int64_t ngroup = (nct + G - 1) / G, NG16 = 16 * G;
+    d.ao_mo_g = G;
+    d.ao_mo_ngroup = (int)ngroup;
+    std::vector<double> b64((size_t)(ngroup * nchunk) * AO_MO_KC * NG16, 0.0);
     std::vector<float> b32(b64.size(), 0.0f);
-    std::vector<uint16_t> bbf((size_t)(ngb * nchunk) * 3 * NGb * AO_MO_BF_LD, 0);
+    std::vector<uint16_t> bbf((size_t)(ngroup * nchunk) * 3 * NG16 * AO_MO_BF_LD, 0);
     for (int64_t ch = 0; ch < nchunk; ++ch)
       for (int kk = 0; kk < AO_MO_KC; ++kk) {
         const int64_t k = ch * AO_MO_KC + kk;
         for (int64_t col = 0; col < pad; ++col) {
           const double x = ct[k * pad + col];
-          const int64_t g = col / NGd, n = col - g * NGd;
-          const size_t i = (((size_t)(g * nchunk + ch)) * AO_MO_KC + kk) * NGd + n;
-          b64[i] = x;
-          b32[i] = (float)x;
-          const int64_t gb = col / NGb, nb = col - gb * NGb;
+          const int64_t g = col / NG16, n = col - g * NG16;
+          const size_t blk = (size_t)(g * nchunk + ch);
+          b64[(blk * AO_MO_KC + kk) * NG16 + n] = x;
+          b32[(blk * AO_MO_KC + kk) * NG16 + n] = (float)x;
           uint16_t h3[3];
           split3_host(x, h3);
-          for (int pl = 0; pl < 3; ++pl)
-            bbf[((((size_t)(gb * nchunk + ch)) * 3 + pl) * NGb + nb) * AO_MO_BF_LD + kk] = h3[pl];
+          for (int pl = 0; pl < 3; ++pl) bbf[((blk * 3 + pl) * NG16 + n) * AO_MO_BF_LD + kk] = h3[pl];
         }
       }
     up(d.coef_ch64, b64.data(), 8 * b64.size());
@@ -353,9 +340,8 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     a.desc_max = d.ao_desc_max;
     a.ns_max = c->tables.chunk_shells_max;
     a.nchunk = d.ao_nchunk;
-    a.G = d.ao_mo_g[mode];
-    a.ngroup = d.ao_mo_ngroup[mode];
-    a.tile = d.ao_mo_tile[mode];
+    a.G = d.ao_mo_g;
+    a.ngroup = d.ao_mo_ngroup;
     a.B = mode == QMCKL_AMD_MO_FP64 ? d.coef_ch64.ptr : mode == QMCKL_AMD_MO_FP32 ? d.coef_ch32.ptr : d.coef_chbf.ptr;
     a.b_chunk_bytes = mode == QMCKL_AMD_MO_FP64   ? AO_MO_KC * 16 * a.G * 8
                       : mode == QMCKL_AMD_MO_FP32 ? AO_MO_KC * 16 * a.G * 4

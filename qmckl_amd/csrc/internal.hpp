@@ -49,11 +49,11 @@ struct DevConst {
   DevBuf prim_expo, prim_coef, ao_shell, ao_pow, ao_factor, nucl_range;
   DevBuf nucl_coord;  // [3][nucl]
   DevBuf coef_t;      // MO coefficients transposed [k_pad][mo_pad] (FP64, zero-padded)
-  // C_t in the per-chunk operand layouts of the fused AO->MO kernels, [group][chunk][...]:
-  // FP64 / FP32 [32][16 G], BF16X3 [3][16 G][AO_MO_BF_LD] (zero-padded); G per mode
+  // C_t in the per-chunk operand layouts of the fused AO->MO kernel, [group][chunk][...]:
+  // FP64 / FP32 [32][16 G], BF16X3 [3][16 G][AO_MO_BF_LD] (zero-padded)
   DevBuf coef_ch64, coef_ch32, coef_chbf;
   int64_t mo_pad = 0, k_pad = 0;
-  int ao_mo_g[3] = {1, 1, 1}, ao_mo_ngroup[3] = {1, 1, 1}, ao_mo_tile[3] = {0, 0, 1};  // per mode
+  int ao_mo_g = 1, ao_mo_ngroup = 1;  // column tiles per pass of the fused kernel, passes
   // 32-AO chunk descriptors of the fused kernel (AoMoArgs), offsets in doubles
   DevBuf ao_desc, ao_desc_off;
   int ao_desc_max = 0, ao_nchunk = 0;
@@ -236,8 +236,7 @@ hipError_t launch_split_bf16x3(const double* src, int64_t n, uint16_t* dst, int6
 // in the operand layout of the mode (see DevConst), b_chunk_bytes per (group, chunk); the MO
 // columns are taken G tiles of 16 at a time, ngroup groups.  C: [points][5][ldc], columns < N.
 constexpr int AO_MO_KC = 32;
-constexpr int AO_MO_MAX_G = 8;     // column tiles per pass, tile kernel
-constexpr int AO_MO_CM_MAX_G = 5;  // column tiles per pass, component-major FP64/FP32 kernel
+constexpr int AO_MO_MAX_G = 8;  // column tiles per pass
 constexpr int AO_MO_BF_LD = 40;  // bf16 row of a BF16X3 C_t chunk: 32 AOs + 8 padding
 struct AoMoArgs {
   const double* coord;
@@ -247,7 +246,6 @@ struct AoMoArgs {
   int desc_max, ns_max, nchunk;
   const void* B;
   int b_chunk_bytes, G, ngroup;
-  int tile;  // FP64/FP32: 1 = tile kernel, 0 = component-major kernel (BF16X3: always tile)
   double* C;
   int64_t ldc, N;
   double cutoff;

@@ -1,39 +1,29 @@
 // Fused AO VGL -> MO VGL on gfx950 (org/qmckl_ao.org:8438-8572 followed by
 // org/qmckl_mo.org:1905-1940): the AO matrix never leaves the chip.
 //
-// The product MO[p][c][:] = sum_k AO[p][c][k] C_t[k][:] runs on matrix cores with the rows
-// in component-major order: row tile c holds component c (value, d/dx, d/dy, d/dz,
-// Laplacian) of 16 points.  A 64-lane wave owns 16 points (lane li = point) and G column
-// tiles of MOs, i.e. 5 G accumulator tiles.  With that order the A operand a lane feeds is
-// exactly the VGL of its own point for the AOs of its k slots, so every lane evaluates the
-// AO values it needs straight into registers: no AO tile in LDS, and every wave does the
-// same work.  The AOs are walked in chunks of KC = 32:
-//   1. the radial VGL of every shell of the chunk at the wave's 16 points (FP64, screened
-//      exactly as the reference) into the wave's LDS staging area;
-//   2. per k slot, the AO VGL (monomial x radial x normalisation) of the lane's point, then
-//      the products of the 5 components with G column tiles of C_t.
-// The per-chunk data (nuclei, shells, primitives and AOs of the chunk: the chunk
-// descriptor) and the chunk of C_t in the operand layout are copied to LDS one chunk ahead
-// with the gfx950 global-to-LDS loads, so the loop issues no dependent global loads.
-// Operand precisions of this kernel (k_ao_mo_cm):
-//   FP64   v_mfma_f64_16x16x4_f64, lane (li, lg) feeds AO k0 + 4 s + lg at step s.
-//   FP32   v_mfma_f32_16x16x4_f32, same slots; FP32 sums of each chunk added into FP64.
-// A workgroup is 4 waves (64 points); the accumulators live in AGPRs (one wave per SIMD).
-// On gfx950 FP64 matrix and vector instructions share one issue budget, and this layout
-// keeps both at their minimum (no operand-tile traffic, balanced tiles).
-//
-// With more than AO_MO_CM_MAX_G column tiles that layout would evaluate the AOs once per
-// group of 5 column tiles; the tile kernel (k_ao_mo_tile) then serves FP64 and FP32.
-//
-// BF16X3 always runs the tile kernel: it is bound by the vector work of the AO evaluation,
-// not by its matrix instructions, and needs more waves per SIMD to hide latency than 25 FP64
-// accumulator tiles per wave allow.  It keeps the cooperative layout: a workgroup of 4 waves
-// owns 16 points (rows 5 p + c), evaluates their AO VGL for the chunk once into an LDS
-// operand tile, and each wave multiplies it by its column tiles; both BF16X3 operands are
-// three bfloat16 terms (the AO values by an exact truncation split of float(x), C_t by a
-// rounded split of the FP64 value), the six leading cross products (smallest first) go
-// through v_mfma_f32_16x16x32_bf16 into FP32 per chunk and are added into FP64.  It reads
-// the same chunk descriptors (async, LDS) and the chunked C_t of its mode (global, L2).
+// A workgroup of 4 waves owns TP = 16 points, i.e. the 80 rows (point, component) of the
+// product MO[p][c][:] = sum_k AO[p][c][k] C_t[k][:] (row 5 p + c), and G <= 8 column tiles of
+// 16 MOs (grid.y walks the groups of G tiles).  The AOs are walked in chunks of KC = 32:
+//   1. the radial VGL of every shell of the chunk at the 16 points (FP64, screened exactly as
+//      the reference) into LDS;
+//   2. the 16 x 32 AO VGL values of the chunk (5 components each), assembled in FP64, into
+//      an LDS operand tile in the precision of the product;
+//   3. each wave multiplies the tile by its column tiles of C_t on matrix cores.
+// The per-chunk basis data (nuclei, shells, primitives and AOs of the chunk: the chunk
+// descriptor) is one contiguous record copied to LDS one chunk ahead with the gfx950
+// global-to-LDS loads, so the loop issues no dependent global loads.  C_t is stored chunk by
+// chunk in the operand layout of the precision and read from L2.
+// Operand precisions:
+//   FP64   v_mfma_f64_16x16x4_f64, FP64 accumulation.
+//   FP32   v_mfma_f32_16x16x4_f32; FP32 sums of each chunk added into FP64.
+//   BF16X3 both operands as three bfloat16 terms (the AO values by an exact truncation split
+//          of float(x), C_t by a rounded split of the FP64 value); per chunk the six leading
+//          cross products (smallest first) through v_mfma_f32_16x16x32_bf16 into FP32, then
+//          added into FP64.
+// Why one shared tile, not a per-wave layout: the AO evaluation dominates the vector work,
+// and evaluating each AO value once per workgroup at 2-3 waves per SIMD beats layouts that
+// keep 25 FP64 accumulator tiles per wave in registers (one wave per SIMD) by 15-30 % at
+// every precision on the MI355X.
 #include <hip/hip_runtime.h>
 
 #include "ao_device.hpp"
@@ -50,15 +40,7 @@ typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr int KC = AO_MO_KC;  // AOs per chunk
-constexpr int NWAVE = 4;      // waves (16 points each) per workgroup
-constexpr int RAD = 6;        // doubles per staged (point, shell): 5 radial VGL + pad
-
-// Row stride (doubles) of a point's radial staging, 2 mod 32 so that the 16 points of a
-// wave start in different LDS banks.
-__host__ __device__ constexpr int rad_stride(int ns_max) {
-  const int n = RAD * ns_max;
-  return n + ((2 - n) % 32 + 32) % 32;
-}
+constexpr int NWAVE = 4;      // waves per workgroup
 
 // Copies n bytes (a multiple of 16, both addresses 16-byte aligned) from global memory to
 // LDS, 16 bytes per lane, the waves of the block taking 1 KiB slices in turn.  Asynchronous:
@@ -72,7 +54,7 @@ __device__ __forceinline__ void lds_copy(void* dst, const void* src, int nbytes,
   }
 }
 
-struct Chunk {  // views of one chunk descriptor in LDS (layout: AoMoDesc)
+struct Chunk {  // views of one chunk descriptor in LDS (layout: AoMoArgs)
   const double *nuc, *shl, *prm, *ao;
   int ns, na;
 };
@@ -137,128 +119,9 @@ __device__ __forceinline__ void chunk_ao_pt(const Chunk& c, int kk, double px, d
   r[3] = (pv[3] * s0 + pv[0] * s3) * f;
   r[4] = (pv[4] * s0 + pv[0] * s4 + 2.0 * (pv[1] * s1 + pv[2] * s2 + pv[3] * s3)) * f;
 }
-__device__ __forceinline__ void chunk_ao(const Chunk& c, int kk, double px, double py, double pz, const double* rad,
-                                         double r[5]) {
-  chunk_ao_pt(c, kk, px, py, pz, rad, RAD, r);
-}
-
-template <int MODE, int G>
-__global__ void __launch_bounds__(64 * NWAVE) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_ao_mo_cm(AoMoArgs A) {
-  extern __shared__ __attribute__((aligned(16))) double lds[];
-  constexpr int NG16 = 16 * G;
-  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int RS = rad_stride(A.ns_max);
-  // LDS: descriptor buffers [2][desc_max], C_t chunk buffers [2][b_chunk_bytes], radial
-  // staging; buffer addresses are computed from `lds` (never picked from an array of
-  // pointers) so that every access compiles to an LDS instruction
-  char* bbase = reinterpret_cast<char*>(lds + 2 * A.desc_max);
-  double* rad = lds + 2 * A.desc_max + A.b_chunk_bytes / 4 + wave * 16 * RS;  // 2 b buffers = b/4 doubles
-  const int group = blockIdx.y;
-  const int64_t p = A.p0 + (int64_t)blockIdx.x * (16 * NWAVE) + 16 * wave + li;  // the lane's point
-  const bool pv = p < A.p0 + A.np;
-  const double px = pv ? A.coord[p] : 0.0, py = pv ? A.coord[p + A.coord_ld] : 0.0,
-               pz = pv ? A.coord[p + 2 * A.coord_ld] : 0.0;
-  const char* bsrc = static_cast<const char*>(A.B) + (int64_t)group * A.nchunk * A.b_chunk_bytes;
-  const double cutoff = A.cutoff;
-
-  d4 acc[5][G];
-#pragma unroll
-  for (int r = 0; r < 5; ++r)
-#pragma unroll
-    for (int j = 0; j < G; ++j) acc[r][j] = d4{0.0, 0.0, 0.0, 0.0};
-
-  lds_copy(lds, A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
-  lds_copy(bbase, bsrc, A.b_chunk_bytes, wave, lane);
-  for (int ch = 0; ch < A.nchunk; ++ch) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();  // chunk ch in LDS; every wave done with chunk ch - 1
-    if (ch + 1 < A.nchunk) {
-      const int nb = (ch + 1) & 1;
-      lds_copy(lds + nb * A.desc_max, A.desc + A.desc_off[ch + 1], 8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]),
-               wave, lane);
-      lds_copy(bbase + nb * A.b_chunk_bytes, bsrc + (int64_t)(ch + 1) * A.b_chunk_bytes, A.b_chunk_bytes, wave, lane);
-    }
-    const Chunk c = chunk_view(lds + (ch & 1) * A.desc_max);
-    const char* bl = bbase + (ch & 1) * A.b_chunk_bytes;
-    // 1. radial VGL of the chunk's shells at the wave's points: lane (li, lg) takes shells
-    //    lg, lg + 4, ... at point li
-    for (int s = lg; s < c.ns; s += 4) {
-      const int nl = (int)c.shl[2 * s] & 255;
-      double r[5];
-      chunk_radial(c, s, px - c.nuc[4 * nl], py - c.nuc[4 * nl + 1], pz - c.nuc[4 * nl + 2], cutoff, r);
-      double* o = rad + li * RS + RAD * s;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) o[q] = r[q];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const double* myrad = rad + li * RS;
-    // 2. AO VGL of the lane's slots and the products
-    if constexpr (MODE == 0) {
-      const double* B = reinterpret_cast<const double*>(bl);  // [32][16 G]
-#pragma unroll 2
-      for (int ks = 0; ks < KC / 4; ++ks) {
-        double a[5];
-        chunk_ao(c, 4 * ks + lg, px, py, pz, myrad, a);
-        const double* brow = B + (4 * ks + lg) * NG16 + li;
-#pragma unroll
-        for (int j = 0; j < G; ++j) {
-          const double b = brow[16 * j];
-#pragma unroll
-          for (int r = 0; r < 5; ++r) acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][j], 0, 0, 0);
-        }
-      }
-    } else {
-      const float* B = reinterpret_cast<const float*>(bl);  // [32][16 G]
-      f4 part[5][G];
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int j = 0; j < G; ++j) part[r][j] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-      for (int ks = 0; ks < KC / 4; ++ks) {
-        double a[5];
-        chunk_ao(c, 4 * ks + lg, px, py, pz, myrad, a);
-        const float* brow = B + (4 * ks + lg) * NG16 + li;
-#pragma unroll
-        for (int j = 0; j < G; ++j) {
-          const float b = brow[16 * j];
-#pragma unroll
-          for (int r = 0; r < 5; ++r)
-            part[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a[r], b, part[r][j], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int j = 0; j < G; ++j)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[r][j][q] += (double)part[r][j][q];
-    }
-  }
-  // rows of the result tiles: f64 MFMA register q holds point lg + 4 q, f32 4 lg + q
-#pragma unroll
-  for (int j = 0; j < G; ++j) {
-    const int64_t col = (int64_t)(group * G + j) * 16 + li;
-    if (col >= A.N) continue;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int prow = MODE == 0 ? lg + 4 * q : 4 * lg + q;
-      const int64_t pt = A.p0 + (int64_t)blockIdx.x * (16 * NWAVE) + 16 * wave + prow;
-      if (pt >= A.p0 + A.np) continue;
-#pragma unroll
-      for (int r = 0; r < 5; ++r) A.C[(pt * 5 + r) * A.ldc + col] = acc[r][j][q];
-    }
-  }
-}
-
-// Tile kernel (BF16X3 always; FP64 and FP32 when the MOs need more than AO_MO_CM_MAX_G
-// column tiles, where one evaluation of the AO tile feeds up to 8 column tiles).  TP = 16
-// points per workgroup (5 row tiles, point-major rows 5 p + c), KC = 32 AOs per chunk.  The
-// 5 x G products of a chunk (row tile r, column tile j) go to the waves as: wave w takes
+// The kernel.  TP = 16 points per workgroup (5 row tiles, point-major rows 5 p + c), KC = 32
+// AOs per chunk.  The 5 x G products of a chunk (row tile r, column tile j) go to the waves
+// as: wave w takes
 // column tiles w and w + 4 (all row tiles) when they exist, except that with G = 5 the tiles
 // (r, 4) are spread over the waves by r mod 4 (at most 7 accumulator tiles per wave).
 constexpr int TP = 16;
@@ -456,21 +319,6 @@ hipError_t launch_kernel(K kernel, const AoMoArgs& a, dim3 grid, dim3 block, siz
 
 template <int MODE, int G>
 hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
-  const bool tile = MODE == 2 || a.tile;
-  if constexpr (MODE != 2 && G <= AO_MO_CM_MAX_G) {
-    if (!tile) {
-      static bool attr = false;
-      if (!attr) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ao_mo_cm<MODE, G>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-      }
-      const dim3 grid((unsigned)((a.np + 16 * NWAVE - 1) / (16 * NWAVE)), (unsigned)a.ngroup);
-      return launch_kernel(k_ao_mo_cm<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
-    }
-  }
-  if (!tile) return hipErrorInvalidValue;
   static bool attr = false;
   if (!attr) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ao_mo_tile<MODE, G>),
@@ -500,17 +348,14 @@ hipError_t launch_mode(const AoMoArgs& a, size_t lds, hipStream_t s) {
 }  // namespace
 
 size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
-  if (mode == 2 || a.tile)
-    return sizeof(double) * (2 * (size_t)a.desc_max + 3 * TP + (size_t)TP * a.ns_max * SH) +
-           (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
-  return sizeof(double) * (2 * (size_t)a.desc_max + (size_t)NWAVE * 16 * rad_stride(a.ns_max)) +
-         2 * (size_t)a.b_chunk_bytes;
+  return sizeof(double) * (2 * (size_t)a.desc_max + 3 * TP + (size_t)TP * a.ns_max * SH) +
+         (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
 }
 
 hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s) {
   if (a.np <= 0) return hipSuccess;
-  if (a.G < 1 || a.G > AO_MO_MAX_G || (!a.tile && mode != 2 && a.G > AO_MO_CM_MAX_G) || a.nchunk < 1 ||
-      a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 || a.b_chunk_bytes % 16)
+  if (a.G < 1 || a.G > AO_MO_MAX_G || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
+      a.b_chunk_bytes % 16)
     return hipErrorInvalidValue;
   const size_t lds = ao_mo_lds_bytes(a, mode);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
