@@ -65,18 +65,25 @@ struct Lkm {
   }
 };
 
-__device__ __forceinline__ double block_sum(double x, double* red) {
-  // Deterministic tree reduction over the 256 lanes of the block.
-  red[threadIdx.x] = x;
+// Deterministic sum over the lanes of a block of any multiple of 64 threads: a fixed
+// shuffle tree per wave, then the wave partials in wave order.  red holds one double per
+// wave.
+__device__ __forceinline__ double block_sum_waves(double x, double* red) {
+  x += __shfl_xor(x, 1, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 8, 64);
+  x += __shfl_xor(x, 16, 64);
+  x += __shfl_xor(x, 32, 64);
+  const int nw = (int)blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  const double r = red[0];
+  double s = 0.0;
+  for (int wv = 0; wv < nw; ++wv) s += red[wv];
   __syncthreads();
-  return r;
+  return s;
 }
+
 
 // g_a(r_j) = exp(-kappa_a |r_j - R_a|) scratch, laid out as the e-e-n kernels stage it in
 // LDS: [W][nucleus tile of 16][j, Ne rounded up to 4][16], zero in the padding.
@@ -91,7 +98,8 @@ __host__ __device__ __forceinline__ int gn_jp(int64_t Ne) { return (int)((Ne + 3
 // R0 scratch layout: [W][Ne][cord-1][5]; gN: see gn_index.
 // ------------------------------------------------------------------------------------
 // C (= cord, 0..8) is a template parameter so that only the C - 1 orders of the q = 0 sums
-// are held in registers.
+// are held in registers.  The block has 64 ceil(Ne / 64) threads (at most 256), one
+// electron i per thread.
 template <int C>
 __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
                                                      double* __restrict__ gN, double* __restrict__ Sg) {
@@ -101,8 +109,9 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   const int NAg = gn_na(Nn), JPg = gn_jp(Ne);
   const int64_t w = blockIdx.x;
   double* xe = lds;            // [Ne][3]
-  double* red = lds + 3 * Ne;  // [kBlock]
-  for (int t = threadIdx.x; t < Ne; t += kBlock)
+  double* red = lds + 3 * Ne;  // [waves]
+  const int nth = (int)blockDim.x;
+  for (int t = threadIdx.x; t < Ne; t += nth)
     for (int d = 0; d < 3; ++d) xe[3 * t + d] = v.coord[w * Ne + t + d * v.coord_ld];
   __syncthreads();
 
@@ -111,7 +120,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   const int bord = (int)v.bord, aord = (int)v.aord;
   const int up = (int)v.up_num;
   double fee = 0.0, fen = 0.0;
-  for (int i = threadIdx.x; i < Ne; i += kBlock) {
+  for (int i = threadIdx.x; i < Ne; i += nth) {
     const double xi = xe[3 * i], yi = xe[3 * i + 1], zi = xe[3 * i + 2];
     double g0 = 0, g1 = 0, g2 = 0, g3 = 0;
     double r0[K1 > 0 ? K1 : 1][5];
@@ -248,16 +257,16 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
     oh[i + 2 * Ne] = h2;
     oh[i + 3 * Ne] = h3;
   }
-  const double see = block_sum(fee, red);
-  const double sen = block_sum(fen, red);
+  const double see = block_sum_waves(fee, red);
+  const double sen = block_sum_waves(fen, red);
   if (threadIdx.x == 0) {
     o.fee[w] = see;
     o.fen[w] = sen;
   }
   // Sg[w][a][q-1] = sum_i g_a(r_i)^q, q = 1..cord-1: the k = 0 sums of the e-e-n term over
-  // all electrons (the barriers of block_sum make this block's gN writes visible).
+  // all electrons (the barriers of block_sum_waves make this block's gN writes visible).
   if constexpr (K1 > 0) {
-    for (int a = threadIdx.x; a < Nn; a += kBlock) {
+    for (int a = threadIdx.x; a < Nn; a += nth) {
       double acc[K1];
 #pragma unroll
       for (int q = 0; q < K1; ++q) acc[q] = 0.0;
@@ -278,11 +287,11 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   }
   // zero padding of gN: nuclei Nn .. 16 NA - 1 for every j, electrons Ne .. JP - 1 for a < Nn
   const int npad_a = 16 * NAg - Nn;
-  for (int t = threadIdx.x; t < npad_a * JPg; t += kBlock) {
+  for (int t = threadIdx.x; t < npad_a * JPg; t += nth) {
     const int j = t / npad_a, a = Nn + t - j * npad_a;
     gN[gn_index(w, a, j, NAg, JPg)] = 0.0;
   }
-  for (int t = threadIdx.x; t < (JPg - Ne) * Nn; t += kBlock) {
+  for (int t = threadIdx.x; t < (JPg - Ne) * Nn; t += nth) {
     const int j = Ne + t / Nn, a = t % Nn;
     gN[gn_index(w, a, j, NAg, JPg)] = 0.0;
   }
@@ -1665,13 +1674,14 @@ ScratchView scratch_view(const JasDevView& v, const JasOut& o) {
 
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s) {
   const ScratchView sv = scratch_view(v, o);
-  const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kBlock);
+  const int pair_threads = (int)std::min<int64_t>(kBlock, 64 * ((v.elec_num + 63) / 64));
+  const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kBlock / 64);
   if (lds_pair > 160 * 1024) return hipErrorInvalidValue;
   switch (v.cord) {
 #define PAIR_CASE(CC)                                                                                         \
   case CC:                                                                                                    \
-    hipLaunchKernelGGL(k_jas_pair<CC>, dim3((unsigned)v.walk_num), dim3(kBlock), lds_pair, s, v, o, sv.R0, sv.gN, \
-                       sv.Sg);                                                                                \
+    hipLaunchKernelGGL(k_jas_pair<CC>, dim3((unsigned)v.walk_num), dim3(pair_threads), lds_pair, s, v, o, sv.R0,   \
+                       sv.gN, sv.Sg);                                                                         \
     break;
     PAIR_CASE(0)
     PAIR_CASE(1)
