@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "exp_neg.hpp"
 #include "internal.hpp"
 
 namespace qmckl_amd {
@@ -83,7 +84,7 @@ __device__ __forceinline__ void shell_radial(const AoDevView& v, int s, int a, d
       const double al = v.prim_expo[p0 + i];
       const double ar2 = al * r2;
       if (ar2 > cutoff) continue;
-      const double e = v.prim_coef[p0 + i] * exp(-ar2);
+      const double e = v.prim_coef[p0 + i] * exp_neg(-ar2);
       const double two_a = -2.0 * al * e;
       v0 += e;
       v1 += two_a * x;

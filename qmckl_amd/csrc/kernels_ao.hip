@@ -149,7 +149,7 @@ __global__ void k_prim_vgl(const double* __restrict__ nucl_coord, int64_t Nn, co
   const double ar2 = expo[i] * r2;
   double r[5] = {0, 0, 0, 0, 0};
   if (!(ar2 > cutoff)) {
-    const double e = exp(-ar2);
+    const double e = exp_neg(-ar2);
     const double two_a = -2.0 * expo[i] * e;
     r[0] = e;
     r[1] = two_a * x;

@@ -71,7 +71,8 @@ __device__ __forceinline__ Chunk chunk_view(const double* d) {
 }
 
 // Radial VGL of shell s of the chunk at displacement (x, y, z) from its nucleus, screened as
-// shell_radial (org/qmckl_ao.org:4769-4868).
+// shell_radial (org/qmckl_ao.org:4769-4868); e^x by the degree-DEG series of exp_neg_deg.
+template <int DEG>
 __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, double y, double z, double cutoff,
                                              double r[5]) {
   const double code = c.shl[2 * s];
@@ -83,7 +84,7 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
       const double al = c.prm[2 * (p0 + i)];
       const double ar2 = al * r2;
       if (ar2 > cutoff) continue;
-      const double e = c.prm[2 * (p0 + i) + 1] * exp(-ar2);
+      const double e = c.prm[2 * (p0 + i) + 1] * exp_neg_deg<DEG>(-ar2);
       const double two_a = -2.0 * al * e;
       v0 += e;
       v1 += two_a * x;
@@ -127,7 +128,7 @@ __device__ __forceinline__ void chunk_ao_pt(const Chunk& c, int kk, double px, d
 constexpr int TP = 16;
 constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row = 5 p + c)
 constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
-constexpr int SH = 8;          // doubles per staged (point, shell): 5 radial VGL + dx, dy, dz
+constexpr int SH = 6;          // doubles per staged (point, shell): 5 radial VGL + padding
 constexpr int KP16 = AO_MO_BF_LD;
 // Row strides of the FP64 / FP32 operand tiles: the operand reads A[16 r + li][4 s + lg] of
 // a wave then cover the 64 four-byte LDS banks evenly (2 * 36 = 8 mod 64 for doubles;
@@ -201,7 +202,7 @@ __device__ __forceinline__ void tile_product(d4& acc, const void* at, const void
 }
 
 template <int MODE, int G>
-__global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : 3) k_ao_mo_tile(AoMoArgs A) {
+__global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k_ao_mo_tile(AoMoArgs A) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
   constexpr int NMAIN = G < NWAVE ? G : NWAVE;  // waves with a first column tile
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : 3) k_ao_mo_tile(AoMoAr
       const double x = xt[3 * q] - c.nuc[4 * nl], y = xt[3 * q + 1] - c.nuc[4 * nl + 1],
                    z = xt[3 * q + 2] - c.nuc[4 * nl + 2];
       double r[5];
-      chunk_radial(c, s, x, y, z, A.cutoff, r);
+      chunk_radial<MODE == 0 ? 13 : 9>(c, s, x, y, z, A.cutoff, r);
       double* o = sh + (q * A.ns_max + s) * SH;
 #pragma unroll
       for (int k = 0; k < 5; ++k) o[k] = r[k];
