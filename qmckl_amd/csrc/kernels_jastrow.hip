@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -908,7 +909,7 @@ constexpr int JREC = 36;  // doubles per electron-j record: G[16], R[16], x, y, 
 // Dynamic LDS of k_jas_een_res, in doubles, every array on a 16-byte boundary.  JP = Ne
 // rounded up to 4, NA16 = 16 x number of nucleus tiles.
 struct EenResLds {
-  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, total;
+  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, own, total;
   __host__ __device__ EenResLds(int JP, int NA16, int C, int dim) {
     int o = 0;
     auto take = [&o](int n) { const int at = o; o = (o + n + 1) & ~1; return at; };
@@ -921,6 +922,9 @@ struct EenResLds {
     st = take(NA16 * (C - 1));         // sum over all j of g_a(r_j)^q, [a][q-1]
     cfl = take(dim * NA16);            // c-coefficients, [n][a]
     red = take(EQ_WAVES * EM * 5);     // per-wave row sums
+    int ngroup = 0;                    // groups of units per nucleus tile
+    for (int k = 1; k < C; ++k) ngroup += (C - k + 1) / 2;
+    own = take((ngroup * (NA16 / 16) + 1) / 2);  // wave of each group, as int
     total = o;
   }
 };
@@ -1205,6 +1209,11 @@ __device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResC
   }
 }
 
+// Persistent: the grid holds one workgroup per CU (the LDS allows no more) and workgroup b
+// takes the items (walker, tile) in [b N / grid, (b + 1) N / grid) in walker-major order, so
+// the per-walker inputs (g_a(r_j) of every nucleus, x_j, the q sums) are copied to LDS once
+// per walker instead of once per tile, the constant ones (nucleus table, c-coefficients)
+// once per workgroup, and no workgroup launch separates two tiles.
 template <int C>
 __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, const double* __restrict__ R0,
                                                               const double* __restrict__ gN,
@@ -1214,169 +1223,197 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   constexpr int NT = 64 * EQ_WAVES;
   constexpr int DIM = Lkm<C>::dim;
   extern __shared__ double lds[];
-  const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
-  const int JP = gn_jp(Ne), NA = gn_na(Nn), NA16 = 16 * NA;
-  const EenResLds L(JP, NA16, C, DIM);
-  double* xt = lds + L.xt;
-  double* jr = lds + L.jr;
-  double* gch = lds + L.gch;
-  double* nuc = lds + L.nuc;
-  double* enr = lds + L.enr;
-  double* r0t = lds + L.r0t;
-  double* st = lds + L.st;
-  double* cfl = lds + L.cfl;
-  double* red = lds + L.red;
-  int64_t w;
-  int tile;
-  een_block_work(blockIdx.x, v.walk_num, ntiles, w, tile);
-  const int i0 = tile * EM;
-  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const double* coord = v.coord;
-  const int64_t cld = v.coord_ld;
-
-  // 1. inputs of the walker.  The contiguous ones (g_a(r_j) of every nucleus tile, the
-  // nucleus table, c-coefficients, k = 0 and q = 0 sums) go straight from global memory to
-  // LDS, all in flight together; the scattered ones through registers; one wait for both.
-  // The rows of r0t past the last electron keep stale data: they only reach discarded rows.
-  lds_copy16(gch, gN + gn_index(w, 0, 0, NA, JP), NA16 * JP, wave, lane);
-  lds_copy4(nuc, v.nuc_pad, 4 * NA16, wave, lane);
-  lds_copy4(cfl, v.cfull_pad, DIM * NA16, wave, lane);
-  lds_copy4(st, Sg + w * Nn * K1, Nn * K1, wave, lane);
-  lds_copy4(r0t, R0 + (w * Ne + i0) * K1 * 5, min(EM, Ne - i0) * K1 * 5, wave, lane);
-  for (int t = Nn * K1 + tid; t < NA16 * K1; t += NT) st[t] = 0.0;  // padding nuclei
-  for (int t = tid; t < EM * 3; t += NT) {
-    const int il = t / 3, d = t - 3 * il;
-    xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
-  }
-  for (int t = tid; t < 4 * JP; t += NT) {
-    const int d = t / JP, jj = t - d * JP;  // d = 3: the pad slot
-    jr[jj * JREC + 32 + d] = (d < 3 && jj < Ne) ? coord[w * Ne + jj + d * cld] : 0.0;
-  }
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  // 2. pair data of the tile and the e-n inverse distances
-  for (int t = tid; t < EM * JP; t += NT) {
-    const int il = t & (EM - 1), jj = t / EM;
-    const int i = i0 + il;
-    double Ge = 0.0, ri = 0.0;
-    if (i < Ne && jj < Ne && i != jj) {
-      const double* pj = jr + jj * JREC + 32;
-      const double dx = xt[il * 3] - pj[0], dy = xt[il * 3 + 1] - pj[1], dz = xt[il * 3 + 2] - pj[2];
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
-      ri = 1.0 / d;
-      Ge = exp(-v.rescale_ee * d);
-    }
-    jr[jj * JREC + il] = Ge;
-    jr[jj * JREC + 16 + il] = ri;
-  }
-  for (int t = tid; t < EM * NA16; t += NT) {
-    const int il = t / NA16, a = t - il * NA16;
-    double rinv = 0.0;
-    if (i0 + il < Ne && a < Nn) {
-      const double dx = xt[il * 3] - nuc[a], dy = xt[il * 3 + 1] - nuc[NA16 + a], dz = xt[il * 3 + 2] - nuc[2 * NA16 + a];
-      rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    }
-    enr[t] = rinv;
-  }
-  __syncthreads();
-
-  // 3. groups of units dealt longest first to the least loaded wave (lowest index on ties):
-  // the pairs of every full nucleus tile, then its single units, then, when the last tile
-  // holds at most 4 nuclei, its pairs and single units on the 4x4x4 products (about a
-  // third of the work of a full-tile unit).  Every wave runs the same scalar schedule and
-  // processes the items it owns.
-  const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
-  double ra[4][5], rm[5];
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
-#pragma unroll
-  for (int c = 0; c < 5; ++c) rm[c] = 0.0;
-  EenResCtx x;
-  x.jr = jr;
-  x.xt = xt;
-  x.r0t = r0t;
-  x.enr = enr;
-  x.cfl = cfl;
-  x.st = st;
-  x.kee = v.rescale_ee;
-  x.nsteps = JP >> 2;
-  x.li = li;
-  x.lg = lg;
-  x.i0 = i0;
-  x.Ne = Ne;
-  x.NA16 = NA16;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int nitem = (int)v.walk_num * ntiles;  // < 2^31 (checked by the launcher)
+  const int it0 = (int)((int64_t)nitem * blockIdx.x / gridDim.x),
+            it1 = (int)((int64_t)nitem * (blockIdx.x + 1) / gridDim.x);
   constexpr int NPR = een_npair(C), NSG = een_nsingle(C);
-  const bool mini = Nn - 16 * (NA - 1) <= 4;  // nuclei in the last tile
-  const int NAF = mini ? NA - 1 : NA;         // tiles on the 16x16x4 products
-  const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n2 = n1 + (mini ? NPR : 0), n3 = n2 + (mini ? NSG : 0);
-  int load[EQ_WAVES];
-#pragma unroll
-  for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
-  for (int it = 0; it < n3; ++it) {
-    int at, g, weight;
-    if (it < n0) {
-      at = it / NPR, g = it - at * NPR, weight = 12;
-    } else if (it < n1) {
-      const int r = it - n0;
-      at = r / NSG, g = NPR + r - at * NSG, weight = 6;
-    } else if (it < n2) {
-      at = NA - 1, g = it - n1, weight = 4;
-    } else {
-      at = NA - 1, g = NPR + it - n2, weight = 2;
+  {
+    // Once per workgroup (ready at the first item's barrier): the constant inputs, and the
+    // owner wave of every group of units of a tile.  The groups are dealt longest first to
+    // the least loaded wave (lowest index on ties): the pairs of every full nucleus tile,
+    // then its single units, then, when the last tile holds at most 4 nuclei, its pairs
+    // and single units on the 4x4x4 products (about a third of the work of a full-tile
+    // unit).
+    const int Nn = (int)v.nucl_num, NA = gn_na(Nn), NA16 = 16 * NA;
+    const EenResLds L(gn_jp((int)v.elec_num), NA16, C, DIM);
+    lds_copy4(lds + L.nuc, v.nuc_pad, 4 * NA16, wave, threadIdx.x & 63);
+    lds_copy4(lds + L.cfl, v.cfull_pad, DIM * NA16, wave, threadIdx.x & 63);
+    if (threadIdx.x == 0) {
+      const bool mini = Nn - 16 * (NA - 1) <= 4;
+      const int NAF = mini ? NA - 1 : NA;
+      const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n3 = n1 + (mini ? NPR + NSG : 0);
+      int load[EQ_WAVES];
+      for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
+      int* own = reinterpret_cast<int*>(lds + L.own);
+      for (int it = 0; it < n3; ++it) {
+        const int weight = it < n0 ? 12 : it < n1 ? 6 : it < n1 + NPR ? 4 : 2;
+        int owner = 0;
+        for (int wv = 1; wv < EQ_WAVES; ++wv)
+          if (load[wv] < load[owner]) owner = wv;
+        load[owner] += weight;
+        own[it] = owner;
+      }
     }
-    int owner = 0;
-#pragma unroll
-    for (int wv = 1; wv < EQ_WAVES; ++wv)
-      if (load[wv] < load[owner]) owner = wv;
-#pragma unroll
-    for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] += (wv == owner) ? weight : 0;
-    if (owner != wave) continue;
-    const bool mu = it >= n1;  // on the 4x4x4 products
-    x.gt = gch + at * JP * 16;
-    x.a = 16 * at + (mu ? (li & 3) : li);
-    x.nx = nuc[x.a];
-    x.ny = nuc[NA16 + x.a];
-    x.nz = nuc[2 * NA16 + x.a];
-    x.ka = nuc[3 * NA16 + x.a];
-    een_res_dispatch<C>(g, mu, x, xi, ra, rm);
   }
-  // the 4x4x4 results belong to row lg + 4 (li / 4)
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 5; ++c) ra[r][c] += (r == (li >> 2)) ? rm[c] : 0.0;
-  // 4. sums over the nuclei (lanes li) and the waves
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double y = ra[r][c];
-      y += __shfl_xor(y, 1, 64);
-      y += __shfl_xor(y, 2, 64);
-      y += __shfl_xor(y, 4, 64);
-      y += __shfl_xor(y, 8, 64);
-      if (li == 0) red[(wave * EM + lg + 4 * r) * 5 + c] = y;
+  int wcur = -1;
+  for (int item = it0; item < it1; ++item) {
+    // The sizes and the thread index pass through an empty asm statement, so that what
+    // derives from them (LDS offsets, bounds, lane offsets) is recomputed per item instead
+    // of being held in registers across the loop.
+    int Ne = (int)v.elec_num, Nn = (int)v.nucl_num, tidl = threadIdx.x;
+    asm volatile("" : "+s"(Ne), "+s"(Nn), "+v"(tidl));
+    const int tid = tidl, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+    const int JP = gn_jp(Ne), NA = gn_na(Nn), NA16 = 16 * NA;
+    const EenResLds L(JP, NA16, C, DIM);
+    double* xt = lds + L.xt;
+    double* jr = lds + L.jr;
+    double* gch = lds + L.gch;
+    double* nuc = lds + L.nuc;
+    double* enr = lds + L.enr;
+    double* r0t = lds + L.r0t;
+    double* st = lds + L.st;
+    double* cfl = lds + L.cfl;
+    double* red = lds + L.red;
+    const double* coord = v.coord;
+    const int64_t cld = v.coord_ld;
+    const bool mini = Nn - 16 * (NA - 1) <= 4;  // nuclei in the last tile
+    const int NAF = mini ? NA - 1 : NA;         // tiles on the 16x16x4 products
+    const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n2 = n1 + (mini ? NPR : 0), n3 = n2 + (mini ? NSG : 0);
+    const int w32 = item / ntiles;
+    const int tile = item - w32 * ntiles;
+    const int64_t w = w32;
+    const int i0 = tile * EM;
+    // 1. inputs.  Per walker (when it changes): g_a(r_j) of every nucleus tile and the
+    // k = 0 sums straight from global memory to LDS, x_j through registers.  Per tile: the
+    // q = 0 sums (the rows past the last electron keep stale data: they only reach
+    // discarded rows) and x_i.  One wait for all of them.
+    if (w32 != wcur) {
+      wcur = w32;
+      lds_copy16(gch, gN + gn_index(w, 0, 0, NA, JP), NA16 * JP, wave, lane);
+      lds_copy4(st, Sg + w * Nn * K1, Nn * K1, wave, lane);
+      for (int t = Nn * K1 + tid; t < NA16 * K1; t += NT) st[t] = 0.0;  // padding nuclei
+      for (int t = tid; t < 4 * JP; t += NT) {
+        const int d = t / JP, jj = t - d * JP;  // d = 3: the pad slot
+        jr[jj * JREC + 32 + d] = (d < 3 && jj < Ne) ? coord[w * Ne + jj + d * cld] : 0.0;
+      }
     }
-  __syncthreads();
-  double* jrow = xt;  // free after the units
-  for (int t = tid; t < EM * 5; t += NT) {
-    double y = 0.0;
+    lds_copy4(r0t, R0 + (w * Ne + i0) * K1 * 5, min(EM, Ne - i0) * K1 * 5, wave, lane);
+    for (int t = tid; t < EM * 3; t += NT) {
+      const int il = t / 3, d = t - 3 * il;
+      xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // 2. pair data of the tile and the e-n inverse distances
+    for (int t = tid; t < EM * JP; t += NT) {
+      const int il = t & (EM - 1), jj = t / EM;
+      const int i = i0 + il;
+      double Ge = 0.0, ri = 0.0;
+      if (i < Ne && jj < Ne && i != jj) {
+        const double* pj = jr + jj * JREC + 32;
+        const double dx = xt[il * 3] - pj[0], dy = xt[il * 3 + 1] - pj[1], dz = xt[il * 3 + 2] - pj[2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        ri = 1.0 / d;
+        Ge = exp(-v.rescale_ee * d);
+      }
+      jr[jj * JREC + il] = Ge;
+      jr[jj * JREC + 16 + il] = ri;
+    }
+    for (int t = tid; t < EM * NA16; t += NT) {
+      const int il = t / NA16, a = t - il * NA16;
+      double rinv = 0.0;
+      if (i0 + il < Ne && a < Nn) {
+        const double dx = xt[il * 3] - nuc[a], dy = xt[il * 3 + 1] - nuc[NA16 + a],
+                     dz = xt[il * 3 + 2] - nuc[2 * NA16 + a];
+        rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      }
+      enr[t] = rinv;
+    }
+    __syncthreads();
+
+    // 3. the groups of units this wave owns (owner table above)
+    const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
+    double ra[4][5], rm[5];
 #pragma unroll
-    for (int wv = 0; wv < EQ_WAVES; ++wv) y += red[wv * EM * 5 + t];
-    const int il = t / 5, c = t - 5 * il, i = i0 + il;
-    if (c == 0)
-      jrow[il] = (i < Ne) ? y : 0.0;
-    else if (i < Ne)
-      o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = y;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double sj = 0.0;
-    for (int il = 0; il < EM; ++il) sj += jrow[il];
-    Jt[w * ntiles + tile] = sj;
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) rm[c] = 0.0;
+    EenResCtx x;
+    x.jr = jr;
+    x.xt = xt;
+    x.r0t = r0t;
+    x.enr = enr;
+    x.cfl = cfl;
+    x.st = st;
+    x.kee = v.rescale_ee;
+    x.nsteps = JP >> 2;
+    x.li = li;
+    x.lg = lg;
+    x.i0 = i0;
+    x.Ne = Ne;
+    x.NA16 = NA16;
+    const int* own = reinterpret_cast<const int*>(lds + L.own);
+    for (int it = 0; it < n3; ++it) {
+      if (__builtin_amdgcn_readfirstlane(own[it]) != wave) continue;
+      int at, g;
+      if (it < n0) {
+        at = it / NPR, g = it - at * NPR;
+      } else if (it < n1) {
+        const int r = it - n0;
+        at = r / NSG, g = NPR + r - at * NSG;
+      } else if (it < n2) {
+        at = NA - 1, g = it - n1;
+      } else {
+        at = NA - 1, g = NPR + it - n2;
+      }
+      const bool mu = it >= n1;  // on the 4x4x4 products
+      x.gt = gch + at * JP * 16;
+      x.a = 16 * at + (mu ? (li & 3) : li);
+      x.nx = nuc[x.a];
+      x.ny = nuc[NA16 + x.a];
+      x.nz = nuc[2 * NA16 + x.a];
+      x.ka = nuc[3 * NA16 + x.a];
+      een_res_dispatch<C>(g, mu, x, xi, ra, rm);
+    }
+    // the 4x4x4 results belong to row lg + 4 (li / 4)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ra[r][c] += (r == (li >> 2)) ? rm[c] : 0.0;
+    // 4. sums over the nuclei (lanes li) and the waves
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double y = ra[r][c];
+        y += __shfl_xor(y, 1, 64);
+        y += __shfl_xor(y, 2, 64);
+        y += __shfl_xor(y, 4, 64);
+        y += __shfl_xor(y, 8, 64);
+        if (li == 0) red[(wave * EM + lg + 4 * r) * 5 + c] = y;
+      }
+    __syncthreads();
+    double* jrow = xt;  // free after the units
+    for (int t = tid; t < EM * 5; t += NT) {
+      double y = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < EQ_WAVES; ++wv) y += red[wv * EM * 5 + t];
+      const int il = t / 5, c = t - 5 * il, i = i0 + il;
+      if (c == 0)
+        jrow[il] = (i < Ne) ? y : 0.0;
+      else if (i < Ne)
+        o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = y;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double sj = 0.0;
+      for (int il = 0; il < EM; ++il) sj += jrow[il];
+      Jt[w * ntiles + tile] = sj;
+    }
+    __syncthreads();  // xt, jr, enr, r0t and red are rewritten by the next item
   }
 }
 
@@ -1699,6 +1736,21 @@ hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t
   return hipGetLastError();
 }
 
+// Workgroups of the persistent e-e-n kernel: one per CU of the current device (its LDS
+// allows no more), at most one per item.
+static unsigned een_res_grid(int64_t nitem) {
+  static std::atomic<int> ncu[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  int n = (dev >= 0 && dev < 64) ? ncu[dev].load() : 0;
+  if (n <= 0) {
+    if (dev < 0 || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    if (dev >= 0 && dev < 64) ncu[dev].store(n);
+  }
+  return (unsigned)std::min<int64_t>(nitem, n);
+}
+
 hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s) {
   const ScratchView sv = scratch_view(v, o);
   const int C = (int)v.cord;
@@ -1707,6 +1759,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
   const dim3 grid((unsigned)(v.walk_num * sh.ntiles));
   if (sh.mfma) {
     if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
+    if (sh.res && v.walk_num * sh.ntiles > INT32_MAX) return hipErrorInvalidValue;  // 32-bit item index
 #define EEN_MFMA_CASE(CC)                                                                                      \
   case CC: {                                                                                                   \
     static bool attr_set = false;                                                                              \
@@ -1720,7 +1773,8 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       attr_set = true;                                                                                         \
     }                                                                                                          \
     if (sh.res)                                                                                                \
-      hipLaunchKernelGGL(k_jas_een_res<CC>, grid, dim3(64 * EQ_WAVES), sh.lds, s, v, o, sv.R0, sv.gN, sv.Sg,    \
+      hipLaunchKernelGGL(k_jas_een_res<CC>, dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES),     \
+                         sh.lds, s, v, o, sv.R0, sv.gN, sv.Sg,                                                 \
                          sv.Jt, sh.ntiles);                                                                    \
     else                                                                                                       \
       hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,    \
