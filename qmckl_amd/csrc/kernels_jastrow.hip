@@ -1152,13 +1152,12 @@ __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int
 
 // One group: contraction, then the fold.  Full tile: lane (li, lg) holds rows lg + 4 r
 // (register r of each accumulator) of column a = 16 at + li, folded into ra[r].  MINI: one
-// row lg + 4 (li / 4) of column a = 16 at + li mod 4 per lane, folded into rm.  No branch on
+// row lg + 4 (li / 4) of column a = 16 at + li mod 4 per lane, folded into that row of ra.  No branch on
 // the row or the nucleus, so the rows interleave: a padding nucleus has zero c-coefficients,
 // kappa, 1/R, products and k = 0 sums and adds exactly 0; a row past the last electron is
 // discarded by the caller.
 template <int C, int K, int Q, int NQ, bool MINI>
-__device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5],
-                                              double (&rm)[5]) {
+__device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
   EenAcc<MINI> acc[NQ][5];
 #pragma unroll
   for (int n = 0; n < NQ; ++n)
@@ -1176,7 +1175,12 @@ __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (
     for (int n = 0; n < NQ; ++n)
 #pragma unroll
       for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c];
-    een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, rm);
+    double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, out);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ra[r][c] += (r == (x.li >> 2)) ? out[c] : 0.0;
   } else {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -1192,7 +1196,7 @@ __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (
 
 template <int C, int G = 0>
 __device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResCtx& x, const double (&xi)[3],
-                                                 double (&ra)[4][5], double (&rm)[5]) {
+                                                 double (&ra)[4][5]) {
   constexpr int NPR = een_npair(C);
   if constexpr (G < NPR + een_nsingle(C)) {
     if (g == G) {
@@ -1200,12 +1204,12 @@ __device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResC
       constexpr int Q = G < NPR ? een_pair_q(C, G) : een_single_q(C, G - NPR);
       constexpr int NQ = G < NPR ? 2 : 1;
       if (mini)
-        een_res_group<C, K, Q, NQ, true>(x, xi, ra, rm);
+        een_res_group<C, K, Q, NQ, true>(x, xi, ra);
       else
-        een_res_group<C, K, Q, NQ, false>(x, xi, ra, rm);
+        een_res_group<C, K, Q, NQ, false>(x, xi, ra);
       return;
     }
-    een_res_dispatch<C, G + 1>(g, mini, x, xi, ra, rm);
+    een_res_dispatch<C, G + 1>(g, mini, x, xi, ra);
   }
 }
 
@@ -1333,31 +1337,34 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     __syncthreads();
 
     // 3. the groups of units this wave owns (owner table above)
-    const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
-    double ra[4][5], rm[5];
+    double ra[4][5];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) rm[c] = 0.0;
-    EenResCtx x;
-    x.jr = jr;
-    x.xt = xt;
-    x.r0t = r0t;
-    x.enr = enr;
-    x.cfl = cfl;
-    x.st = st;
-    x.kee = v.rescale_ee;
-    x.nsteps = JP >> 2;
-    x.li = li;
-    x.lg = lg;
-    x.i0 = i0;
-    x.Ne = Ne;
-    x.NA16 = NA16;
     const int* own = reinterpret_cast<const int*>(lds + L.own);
     for (int it = 0; it < n3; ++it) {
       if (__builtin_amdgcn_readfirstlane(own[it]) != wave) continue;
+      // the lane indices pass through an empty asm statement once per group, so that the
+      // addresses derived from them live for one group, not across the unit loop
+      int tg = threadIdx.x;
+      asm volatile("" : "+v"(tg));
+      const int gli = tg & 15, glg = (tg >> 4) & 3;
+      const double xi[3] = {xt[gli * 3], xt[gli * 3 + 1], xt[gli * 3 + 2]};  // electron i of row gli
+      EenResCtx x;
+      x.jr = jr;
+      x.xt = xt;
+      x.r0t = r0t;
+      x.enr = enr;
+      x.cfl = cfl;
+      x.st = st;
+      x.kee = v.rescale_ee;
+      x.nsteps = JP >> 2;
+      x.li = gli;
+      x.lg = glg;
+      x.i0 = i0;
+      x.Ne = Ne;
+      x.NA16 = NA16;
       int at, g;
       if (it < n0) {
         at = it / NPR, g = it - at * NPR;
@@ -1371,18 +1378,13 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
       }
       const bool mu = it >= n1;  // on the 4x4x4 products
       x.gt = gch + at * JP * 16;
-      x.a = 16 * at + (mu ? (li & 3) : li);
+      x.a = 16 * at + (mu ? (gli & 3) : gli);
       x.nx = nuc[x.a];
       x.ny = nuc[NA16 + x.a];
       x.nz = nuc[2 * NA16 + x.a];
       x.ka = nuc[3 * NA16 + x.a];
-      een_res_dispatch<C>(g, mu, x, xi, ra, rm);
+      een_res_dispatch<C>(g, mu, x, xi, ra);
     }
-    // the 4x4x4 results belong to row lg + 4 (li / 4)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int c = 0; c < 5; ++c) ra[r][c] += (r == (li >> 2)) ? rm[c] : 0.0;
     // 4. sums over the nuclei (lanes li) and the waves
 #pragma unroll
     for (int r = 0; r < 4; ++r)
