@@ -230,10 +230,12 @@ def test_jastrow_n2_vs_oracle():
     q.context_destroy(ctx)
 
 
-@pytest.mark.parametrize("name,W", [("small", 256), ("medium", 8), ("small", 13)])
+@pytest.mark.parametrize("name,W", [("small", 256), ("medium", 8), ("small", 13), ("small", 600), ("medium", 27)])
 def test_jastrow_synthetic_vs_oracle(name, W):
-    """13 walkers: the first 8 take the e-e-n kernel's XCD-grouped workgroup order, the
-    last 5 the plain order."""
+    """The persistent e-e-n kernel runs one workgroup per CU over a walker-major range of
+    (walker, tile) items.  Up to 256 items every workgroup takes one; 600 walkers of one
+    tile and 27 walkers of 10 tiles give workgroups two or three items whose ranges start
+    and end inside a walker, so the per-walker inputs are reloaded mid-range."""
     s, x, W = S.workload(name, walk_num=W)
     ctx = make_ctx(s, x, W, mo=False)
     got = _jastrow_gpu(ctx, W, s.elec_num)
