@@ -130,10 +130,12 @@ constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row =
 constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
 constexpr int SH = 6;          // doubles per staged (point, shell): 5 radial VGL + padding
 constexpr int KP16 = AO_MO_BF_LD;
-// Row strides of the FP64 / FP32 operand tiles: the operand reads A[16 r + li][4 s + lg] of
-// a wave then cover the 64 four-byte LDS banks evenly (2 * 36 = 8 mod 64 for doubles;
-// 36 li mod 64 distinct multiples of 4 for floats).
-constexpr int LDA64 = KC + 4;
+// Row strides of the FP64 / FP32 operand tiles, so that the operand reads A[16 r + li][4 s
+// + lg] of a wave cover the 64 four-byte LDS banks evenly.  Doubles: lane (li, lg) starts at
+// bank 2 (34 li + lg) = 4 li + 2 lg mod 64, all 32 even banks once in each half-wave (lg =
+// 0, 1 and lg = 2, 3); a stride of 36 (8 li + 2 lg) put two lanes of a half-wave on the same
+// banks.  Floats: 36 li mod 64 are 16 distinct multiples of 4, + lg fills all 64 banks.
+constexpr int LDA64 = KC + 2;
 constexpr int LDA32 = KC + 4;
 
 template <int MODE>
