@@ -37,8 +37,7 @@ def log(msg):
 
 
 def per_walker_costs(s):
-    """Algorithmic work per walker (SURVEY-independent restatement of the formulas):
-    AO bytes = 8 (5 Ne Nao + 3 Ne); MO flops = 2*5*Ne*Nao*Nmo;
+    """Algorithmic work per walker: AO bytes = 8 (5 Ne Nao + 3 Ne); MO flops = 2*5*Ne*Nao*Nmo;
     e-e-n flops = 2 Ne^2 Nn * 5 T with T = c(c-1)/2 (value + 4 derivative components)."""
     Ne, Nao, Nmo, Nn, c = s.elec_num, s.ao_num, s.mo_num_kept, s.nucl_num, s.cord
     T = c * (c - 1) // 2
@@ -131,6 +130,14 @@ def main():
         parts = [q.get_array(ctx, f"qmckl_get_jastrow_champ_factor_{k}", W) for k in ("ee", "en", "een")]
         return parts[0] + parts[1] + parts[2]
 
+    def kinetic_surrogate():
+        """-1/2 sum_i (lap_i J + |grad_i J|^2) per walker: the part of the Jastrow-only local
+        energy surrogate E_L = -1/2 sum_i (lap J + |grad J|^2) + V_ee + V_en + V_nn that the
+        evaluation computes (the potentials depend on the coordinates only)."""
+        g = sum(q.get_array(ctx, f"qmckl_get_jastrow_champ_factor_{k}_gl", n_gl).reshape(W, 4, Ne)
+                for k in ("ee", "en", "een"))
+        return -0.5 * (g[:, 3] + g[:, 0] ** 2 + g[:, 1] ** 2 + g[:, 2] ** 2).sum(axis=1)
+
     def kernel_times(mode, n=3):
         q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, mode))
         step()
@@ -150,6 +157,7 @@ def main():
     t64 = timed(q.MO_FP64)
     mo64 = mo.to_array((P, 5, Nmo))
     J64 = jastrow_sum()
+    K64 = kinetic_surrogate()
     log(f"FP64: {t64 * 1e3:.3f} ms/step")
     k64 = kernel_times(q.MO_FP64)
     log(f"timing {args.mode} path")
@@ -162,6 +170,8 @@ def main():
     mo_err = [float(np.abs(a[:, c] - mo64[:, c]).max() / np.abs(mo64[:, c]).max()) for c in range(5)]
     del a, mo64
     dJ = float(np.abs(jastrow_sum() - J64).max())
+    dEL = float(np.abs(kinetic_surrogate() - K64).max())
+    ao_density = ao_nonzero_density(q, S, s, x, W)
 
     costs = per_walker_costs(s)
     # dominant kernel of the split path and its roofline position
@@ -228,10 +238,14 @@ def main():
             "walkers": W, "elec_num": Ne, "ao_num": s.ao_num, "mo_num": Nmo, "nucl_num": s.nucl_num,
             "mo_precision": args.mode,
         },
-        "fp64_path": {"value": W / t64, "ms_per_step": t64 * 1e3, "kernels": k64},
-        "split_path": {"kernels": ksp, "speedup_vs_fp64": t64 / tsp},
+        "fp64_path": {"value": W / t64, "ms_per_step": t64 * 1e3, "kernels": k64,
+                      "mo_product": mo_rate(costs, W, k64, "fp64")},
+        "split_path": {"kernels": ksp, "speedup_vs_fp64": t64 / tsp,
+                       "mo_product": mo_rate(costs, W, ksp, args.mode)},
         "mo_max_rel_err_vs_fp64": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
         "jastrow_max_abs_dJ_vs_fp64": dJ,
+        "local_energy_surrogate_max_abs_dE_vs_fp64": dEL,
+        "ao_nonzero_density": ao_density,
         "host_buffer_path": {"value": W / th, "ms_per_step": th * 1e3,
                              "note": "same step through the host-pointer getters (PCIe-inclusive)"},
         "roofline": roof,
@@ -394,6 +408,41 @@ def measured_traffic(workload, W, mode, kernel):
     return None if k is None else k.get("bytes_per_launch")
 
 
+def mo_rate(costs, W, kernels, mode):
+    """Rate of the fused AO->MO kernel in FP32-equivalent (dense) flops, F_MO = 2*5*Ne*Nao*Nmo
+    per walker, and for 3xBF16 in bf16 matrix flops (6 cross products per FP32-equivalent
+    flop), each with its fraction of the matching MI355X peak."""
+    k = kernels["mo_gemm"]
+    t = k["ms_per_step"] * 1e-3
+    if t <= 0:
+        return None
+    tf = costs["mo_flops"] * W / t / 1e12
+    out = {"dense_TFs": tf}
+    if mode == "bf16x3":
+        out.update({"bf16_products_TFs": 6 * tf, "frac_of_bf16_peak": 6 * tf / PEAK["bf16_TFs"]})
+    else:
+        peak = PEAK["fp64_TFs"] if mode == "fp64" else PEAK["fp32_TFs"]
+        out["frac_of_peak"] = tf / peak
+    return out
+
+
+def ao_nonzero_density(q, S, s, x, W):
+    """Fraction of the AO values that are not exactly zero after screening, from the
+    library's AO getter on the first walkers of the workload (at most 2400 points): the
+    fused kernels multiply the dense AO tile, the reference skips the zero AOs."""
+    Ne = s.elec_num
+    nw = max(1, min(W, 2400 // Ne))
+    xs = np.ascontiguousarray(x.reshape(3, W, Ne)[:, :nw, :].reshape(3, nw * Ne))
+    c = q.context_create()
+    try:
+        S.load(c, s, jastrow=False, mo=False)
+        S.set_coords(c, s, xs, nw)
+        ao = q.get_array(c, "qmckl_get_ao_basis_ao_vgl", nw * Ne * 5 * s.ao_num).reshape(nw * Ne, 5, s.ao_num)
+        return {"value": float(np.count_nonzero(ao[:, 0, :])) / (nw * Ne * s.ao_num), "points": nw * Ne}
+    finally:
+        q.context_destroy(c)
+
+
 def cpu_baseline(s, x, W, seconds):
     """The oracle (FP64 C restatement, OpenMP over walkers/points) on a bounded sample of
     walkers of the same workload: AO VGL + MO VGL + Jastrow value and GL."""
@@ -418,9 +467,25 @@ def cpu_baseline(s, x, W, seconds):
         nw *= 2
         t = run(nw)
         log(f"cpu oracle: {nw} walkers in {t:.2f} s")
-    return {"value": nw / t, "unit": "walker-evals/s", "cores": threads, "kind": "port",
-            "sample": f"{nw} walkers of the same workload (AO VGL + MO VGL + Jastrow value/GL), "
-                      f"{t:.1f} s on {threads} OpenMP threads"}
+    out = {"value": nw / t, "unit": "walker-evals/s", "cores": threads, "kind": "port",
+           "sample": f"{nw} walkers of the same workload (AO VGL + MO VGL + Jastrow value/GL), "
+                     f"{t:.1f} s on {threads} OpenMP threads"}
+    # one thread, on a smaller sample (the OpenMP runtime of the oracle library)
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        n1 = 1
+        t1 = run(n1)
+        while t1 < seconds / 6 and n1 * 2 <= W:
+            n1 *= 2
+            t1 = run(n1)
+        gomp.omp_set_num_threads(threads)
+        log(f"cpu oracle, 1 thread: {n1} walkers in {t1:.2f} s")
+        out["single_thread"] = {"value": n1 / t1, "cores": 1, "sample": f"{n1} walkers, {t1:.1f} s"}
+    except OSError:
+        out["single_thread"] = None
+    return out
 
 
 if __name__ == "__main__":
