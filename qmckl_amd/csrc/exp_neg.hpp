@@ -11,37 +11,49 @@
 //            over the domain (measured maximum 0.80 ulp).
 //   DEG = 9 (exp_neg_f32): truncation error |r|^10 / 10! < 7.1e-12 relative, for values
 //            that are rounded to FP32 (2^-24 = 6e-8) before they are used.
-// Below -708 the result leaves the normal range and is not meant to be used.
+// Below -708 the result leaves the normal range (gradual underflow through ldexp, 0 below
+// about -745).
+// REMAT: on the device the constants pass through an empty asm statement at each call, so
+// that they are rematerialised in scalar registers there instead of being hoisted out of
+// the caller's loops into vector registers for the whole kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace qmckl_amd {
 
-template <int DEG>
+template <bool REMAT>
+__host__ __device__ inline double exp_const(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (REMAT) asm volatile("" : "+s"(c));
+#endif
+  return c;
+}
+
+template <int DEG, bool REMAT = false>
 __host__ __device__ inline double exp_neg_deg(double x) {
   static_assert(DEG == 9 || DEG == 13, "degree");
-  const double inv_ln2 = 0x1.71547652b82fep+0;  // 1 / ln 2
-  const double ln2_hi = 0x1.62e42ff000000p-1;   // ln 2 to 32 significant bits
-  const double ln2_lo = -0x1.718432a1b0e26p-35;  // ln 2 - ln2_hi
+  const double inv_ln2 = exp_const<REMAT>(0x1.71547652b82fep+0);  // 1 / ln 2
+  const double ln2_hi = exp_const<REMAT>(0x1.62e42ff000000p-1);   // ln 2 to 32 significant bits
+  const double ln2_lo = exp_const<REMAT>(-0x1.718432a1b0e26p-35);  // ln 2 - ln2_hi
   const double n = __builtin_rint(x * inv_ln2);
   const double r = __builtin_fma(n, -ln2_lo, __builtin_fma(n, -ln2_hi, x));
   double p;
   if constexpr (DEG == 13) {
-    p = 0x1.6124613a86d09p-33;                       // 1/13!
-    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
-    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
-    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
-    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
+    p = exp_const<REMAT>(0x1.6124613a86d09p-33);                       // 1/13!
+    p = __builtin_fma(p, r, exp_const<REMAT>(0x1.1eed8eff8d898p-29));  // 1/12!
+    p = __builtin_fma(p, r, exp_const<REMAT>(0x1.ae64567f544e4p-26));  // 1/11!
+    p = __builtin_fma(p, r, exp_const<REMAT>(0x1.27e4fb7789f5cp-22));  // 1/10!
+    p = __builtin_fma(p, r, exp_const<REMAT>(0x1.71de3a556c734p-19));  // 1/9!
   } else {
-    p = 0x1.71de3a556c734p-19;  // 1/9!
+    p = exp_const<REMAT>(0x1.71de3a556c734p-19);  // 1/9!
   }
-  p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
-  p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
-  p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
-  p = __builtin_fma(p, r, 0x1.1111111111111p-7);   // 1/5!
-  p = __builtin_fma(p, r, 0x1.5555555555555p-5);   // 1/4!
-  p = __builtin_fma(p, r, 0x1.5555555555555p-3);   // 1/3!
+  p = __builtin_fma(p, r, exp_const<REMAT>(0x1.a01a01a01a01ap-16));  // 1/8!
+  p = __builtin_fma(p, r, exp_const<REMAT>(0x1.a01a01a01a01ap-13));  // 1/7!
+  p = __builtin_fma(p, r, exp_const<REMAT>(0x1.6c16c16c16c17p-10));  // 1/6!
+  p = __builtin_fma(p, r, exp_const<REMAT>(0x1.1111111111111p-7));   // 1/5!
+  p = __builtin_fma(p, r, exp_const<REMAT>(0x1.5555555555555p-5));   // 1/4!
+  p = __builtin_fma(p, r, exp_const<REMAT>(0x1.5555555555555p-3));   // 1/3!
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r, 1.0);
   p = __builtin_fma(p, r, 1.0);

@@ -1,6 +1,7 @@
 // CHAMP Jastrow factor on gfx950, FP64 throughout (org/qmckl_jastrow_champ.org).
 //
-// Three kernels per evaluation:
+// Three kernels per evaluation, or two when the LDS-resident e-e-n kernel applies (it then
+// also computes what k_jas_pair provides, from the pair data it builds anyway):
 //   k_jas_pair   one workgroup per walker, one lane per electron i.  e-e term (:3582-3650,
 //                :4188-4295), e-n term (:7021-7095, :7500-7590), and the a-independent
 //                sums of the e-e-n tensor R[k][c] = sum_{j!=i} g_e(r_ij)^k w_c(ij),
@@ -26,6 +27,7 @@
 #include <atomic>
 #include <type_traits>
 
+#include "exp_neg.hpp"
 #include "internal.hpp"
 
 namespace qmckl_amd {
@@ -95,6 +97,132 @@ __host__ __device__ __forceinline__ int gn_na(int64_t Nn) { return (int)((Nn + 1
 __host__ __device__ __forceinline__ int gn_jp(int64_t Ne) { return (int)((Ne + 3) & ~3); }
 
 // ------------------------------------------------------------------------------------
+// One e-e pair (i, j) with displacement (dx, dy, dz) = r_i - r_j: the value term of the e-e
+// factor when j < i (org/qmckl_jastrow_champ.org:3582-3650, the doc loop over j > i' of the
+// reference), its gradient and Laplacian with respect to electron i (:4188-4295 with
+// org/qmckl_distance.org:1440-1452), and the q = 0 e-e-n sums R0[k-1][c] = sum_j g_e^k w_c
+// (k = 1..K1, w = (1, u, 1/r)).  No branch on j == i: the distance is replaced by 1 and
+// every gradient, Laplacian and e-e-n term is multiplied by an exact zero (dx = dy = dz = 0,
+// ed = em = 0), which leaves the sums unchanged.  Returns g_e(r_ij) = e^(-kappa r_ij) and
+// 1/r_ij, or 0 and 0 for j == i.
+struct EeAcc {
+  double fee = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+};
+template <int K1>
+__device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, double dx, double dy, double dz,
+                                        double kee, double kinv, EeAcc& s, double (&r0)[K1 > 0 ? K1 : 1][5],
+                                        double& ge, double& ri) {
+  const double* b = v.b_vector;
+  const int bord = (int)v.bord, up = (int)v.up_num;
+  const bool use = j != i;
+  const double d = use ? sqrt(dx * dx + dy * dy + dz * dz) : 1.0;
+  const double e = exp_neg_deg<13, true>(-kee * d);
+  const double x = (1.0 - e) * kinv;  // rescaled distance, org/qmckl_distance.org:975-985
+  if (j < i) {
+    const bool same = (v.spin_independent != 1) && (i < up || j >= up);
+    if (v.spin_independent == 1)
+      s.fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
+    else if (same)
+      s.fee += 0.5 * b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb0;
+    else
+      s.fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
+    double xk = x;
+    for (int k = 2; k <= bord; ++k) {
+      xk *= x;
+      s.fee += b[k] * xk;
+    }
+  }
+  const double dd = d > 1.e-20 ? d : 1.e-20;
+  const double dinv = 1.0 / dd;
+  const double ed = use ? ((d > 1.e-20) ? e : exp_neg_deg<13, true>(-kee * dd)) : 0.0;
+  const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
+  const double gl4 = (2.0 * dinv - kee) * ed;
+  const double denom = 1.0 + b[1] * x;
+  const double invd = 1.0 / denom, invd2 = invd * invd;
+  const double gc2 = gx * gx + gy * gy + gz * gz;
+  double f;
+  if (v.spin_independent == 1)
+    f = b[0] * invd2;
+  else if ((i < up && j < up) || (i >= up && j >= up))
+    f = 0.5 * b[0] * invd2;
+  else
+    f = b[0] * invd2;
+  s.g0 += f * gx;
+  s.g1 += f * gy;
+  s.g2 += f * gz;
+  s.g3 += f * (gl4 - 2.0 * b[1] * gc2 * invd);
+  double kf = 2.0, xp = 1.0;
+  for (int k = 2; k <= bord; ++k) {
+    f = b[k] * kf * xp;
+    s.g0 += f * x * gx;
+    s.g1 += f * x * gy;
+    s.g2 += f * x * gz;
+    s.g3 += f * (x * gl4 + (kf - 1.0) * gc2);
+    xp *= x;
+    kf += 1.0;
+  }
+  const double rinv = 1.0 / d;
+  const double ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
+  const double em = use ? e : 0.0;
+  double gk = 1.0;
+#pragma unroll
+  for (int k = 1; k <= K1; ++k) {
+    gk *= em;
+    r0[k - 1][0] += gk;
+    r0[k - 1][1] += gk * ux;
+    r0[k - 1][2] += gk * uy;
+    r0[k - 1][3] += gk * uz;
+    r0[k - 1][4] += gk * rinv;
+  }
+  ge = em;
+  ri = use ? rinv : 0.0;
+}
+
+// One e-n pair (electron i, nucleus a of type t) with displacement (dx, dy, dz) = r_i - R_a,
+// distance d and e = g_a(r_i) = e^(-kappa_a d): the value term of the e-n factor
+// (org/qmckl_jastrow_champ.org:7021-7095) and its gradient and Laplacian with respect to
+// electron i (:7500-7590).
+struct EnAcc {
+  double fen = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;
+};
+__device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
+                                        double e, EnAcc& s) {
+  const int aord = (int)v.aord;
+  const double* av = v.a_vector + t * (aord + 1);
+  const double ka = v.kappa_en[t];
+  const double x = (1.0 - e) * (1.0 / ka);
+  s.fen += av[0] * x / (1.0 + av[1] * x) - v.asymp_jasa[t];
+  double xk = x;
+  for (int p = 2; p <= aord; ++p) {
+    xk *= x;
+    s.fen += av[p] * xk;
+  }
+  const double dd = d > 1.e-20 ? d : 1.e-20;
+  const double dinv = 1.0 / dd;
+  const double ed = (d > 1.e-20) ? e : exp_neg_deg<13, true>(-ka * dd);
+  const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
+  const double gl4 = (2.0 * dinv - ka) * ed;
+  const double denom = 1.0 + av[1] * x;
+  const double invd = 1.0 / denom, invd2 = invd * invd;
+  double f = av[0] * invd2;
+  const double gc2 = gx * gx + gy * gy + gz * gz;
+  s.h0 += f * gx;
+  s.h1 += f * gy;
+  s.h2 += f * gz;
+  s.h3 += f * (gl4 - 2.0 * av[1] * gc2 * invd);
+  double kf = 2.0, xp = 1.0;
+  for (int k = 2; k <= aord; ++k) {
+    f = av[k] * kf * xp;
+    s.h0 += f * x * gx;
+    s.h1 += f * x * gy;
+    s.h2 += f * x * gz;
+    s.h3 += f * (x * gl4 + (kf - 1.0) * gc2);
+    xp *= x;
+    kf += 1.0;
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // e-e and e-n terms, and the q = 0 sums of the e-e-n tensor.
 // R0 scratch layout: [W][Ne][cord-1][5]; gN: see gn_index.
 // ------------------------------------------------------------------------------------
@@ -117,92 +245,25 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
   __syncthreads();
 
   const double kee = v.rescale_ee, kinv = 1.0 / v.rescale_ee;
-  const double* b = v.b_vector;
-  const int bord = (int)v.bord, aord = (int)v.aord;
-  const int up = (int)v.up_num;
   double fee = 0.0, fen = 0.0;
   for (int i = threadIdx.x; i < Ne; i += nth) {
     const double xi = xe[3 * i], yi = xe[3 * i + 1], zi = xe[3 * i + 2];
-    double g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+    EeAcc se;
     double r0[K1 > 0 ? K1 : 1][5];
 #pragma unroll
     for (int k = 0; k < (K1 > 0 ? K1 : 1); ++k)
 #pragma unroll
       for (int c = 0; c < 5; ++c) r0[k][c] = 0.0;
-    // No branch on j == i: for j == i the distance is replaced by 1 and every gradient,
-    // Laplacian and e-e-n term is multiplied by an exact zero (dx = dy = dz = 0,
-    // ed = em = 0), which leaves the sums unchanged; the value term only takes j < i.
     for (int j = 0; j < Ne; ++j) {
-      const bool use = j != i;
-      const double dx = xi - xe[3 * j], dy = yi - xe[3 * j + 1], dz = zi - xe[3 * j + 2];
-      const double d = use ? sqrt(dx * dx + dy * dy + dz * dz) : 1.0;
-      const double e = exp(-kee * d);
-      const double x = (1.0 - e) * kinv;  // rescaled distance, org/qmckl_distance.org:975-985
-      // value: the pair (j, i) with j < i, as in the doc loop over j > i' of the reference
-      if (j < i) {
-        const bool same = (v.spin_independent != 1) && (i < up || j >= up);
-        if (v.spin_independent == 1)
-          fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
-        else if (same)
-          fee += 0.5 * b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb0;
-        else
-          fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
-        double xk = x;
-        for (int k = 2; k <= bord; ++k) {
-          xk *= x;
-          fee += b[k] * xk;
-        }
-      }
-      // gradient/Laplacian w.r.t. electron i, org/qmckl_distance.org:1440-1452
-      const double dd = d > 1.e-20 ? d : 1.e-20;
-      const double dinv = 1.0 / dd;
-      const double ed = use ? ((d > 1.e-20) ? e : exp(-kee * dd)) : 0.0;
-      const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
-      const double gl4 = (2.0 * dinv - kee) * ed;
-      const double denom = 1.0 + b[1] * x;
-      const double invd = 1.0 / denom, invd2 = invd * invd;
-      const double gc2 = gx * gx + gy * gy + gz * gz;
-      double f;
-      if (v.spin_independent == 1)
-        f = b[0] * invd2;
-      else if ((i < up && j < up) || (i >= up && j >= up))
-        f = 0.5 * b[0] * invd2;
-      else
-        f = b[0] * invd2;
-      g0 += f * gx;
-      g1 += f * gy;
-      g2 += f * gz;
-      g3 += f * (gl4 - 2.0 * b[1] * gc2 * invd);
-      double kf = 2.0, xp = 1.0;
-      for (int k = 2; k <= bord; ++k) {
-        f = b[k] * kf * xp;
-        g0 += f * x * gx;
-        g1 += f * x * gy;
-        g2 += f * x * gz;
-        g3 += f * (x * gl4 + (kf - 1.0) * gc2);
-        xp *= x;
-        kf += 1.0;
-      }
-      // a-independent e-e-n sums (k >= 1); g_e = exp(-kappa r), diagonal excluded
-      const double rinv = 1.0 / d;
-      const double ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
-      const double em = use ? e : 0.0;
-      double gk = 1.0;
-#pragma unroll
-      for (int k = 1; k <= K1; ++k) {
-        gk *= em;
-        r0[k - 1][0] += gk;
-        r0[k - 1][1] += gk * ux;
-        r0[k - 1][2] += gk * uy;
-        r0[k - 1][3] += gk * uz;
-        r0[k - 1][4] += gk * rinv;
-      }
+      double ge, ri;
+      ee_pair<K1>(v, i, j, xi - xe[3 * j], yi - xe[3 * j + 1], zi - xe[3 * j + 2], kee, kinv, se, r0, ge, ri);
     }
+    fee += se.fee;
     double* og = o.fee_gl + w * 4 * Ne;
-    og[i] = g0;
-    og[i + Ne] = g1;
-    og[i + 2 * Ne] = g2;
-    og[i + 3 * Ne] = g3;
+    og[i] = se.g0;
+    og[i + Ne] = se.g1;
+    og[i + 2 * Ne] = se.g2;
+    og[i + 3 * Ne] = se.g3;
     if constexpr (K1 > 0) {
       double* rr = R0 + ((w * Ne + i) * K1) * 5;
 #pragma unroll
@@ -210,53 +271,21 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
 #pragma unroll
         for (int c = 0; c < 5; ++c) rr[k * 5 + c] = r0[k][c];
     }
-
-    // e-n term, org/qmckl_jastrow_champ.org:7021-7095 and :7500-7590
-    double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+    EnAcc sn;
     for (int a = 0; a < Nn; ++a) {
       const int t = (int)v.type_nucl[a];
-      const double* av = v.a_vector + t * (aord + 1);
-      const double ka = v.kappa_en[t];
       const double dx = xi - v.nucl_coord[a], dy = yi - v.nucl_coord[a + Nn], dz = zi - v.nucl_coord[a + 2 * Nn];
       const double d = sqrt(dx * dx + dy * dy + dz * dz);
-      const double e = exp(-ka * d);
+      const double e = exp_neg_deg<13, true>(-v.kappa_en[t] * d);
       gN[gn_index(w, a, i, NAg, JPg)] = e;  // g_a(r_i), reused by the e-e-n kernels
-      const double x = (1.0 - e) * (1.0 / ka);
-      fen += av[0] * x / (1.0 + av[1] * x) - v.asymp_jasa[t];
-      double xk = x;
-      for (int p = 2; p <= aord; ++p) {
-        xk *= x;
-        fen += av[p] * xk;
-      }
-      const double dd = d > 1.e-20 ? d : 1.e-20;
-      const double dinv = 1.0 / dd;
-      const double ed = (d > 1.e-20) ? e : exp(-ka * dd);
-      const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
-      const double gl4 = (2.0 * dinv - ka) * ed;
-      const double denom = 1.0 + av[1] * x;
-      const double invd = 1.0 / denom, invd2 = invd * invd;
-      double f = av[0] * invd2;
-      const double gc2 = gx * gx + gy * gy + gz * gz;
-      h0 += f * gx;
-      h1 += f * gy;
-      h2 += f * gz;
-      h3 += f * (gl4 - 2.0 * av[1] * gc2 * invd);
-      double kf = 2.0, xp = 1.0;
-      for (int k = 2; k <= aord; ++k) {
-        f = av[k] * kf * xp;
-        h0 += f * x * gx;
-        h1 += f * x * gy;
-        h2 += f * x * gz;
-        h3 += f * (x * gl4 + (kf - 1.0) * gc2);
-        xp *= x;
-        kf += 1.0;
-      }
+      en_pair(v, t, dx, dy, dz, d, e, sn);
     }
+    fen += sn.fen;
     double* oh = o.fen_gl + w * 4 * Ne;
-    oh[i] = h0;
-    oh[i + Ne] = h1;
-    oh[i + 2 * Ne] = h2;
-    oh[i + 3 * Ne] = h3;
+    oh[i] = sn.h0;
+    oh[i + Ne] = sn.h1;
+    oh[i + 2 * Ne] = sn.h2;
+    oh[i + 3 * Ne] = sn.h3;
   }
   const double see = block_sum_waves(fee, red);
   const double sen = block_sum_waves(fen, red);
@@ -901,7 +930,10 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
 // x_j, y_j, z_j) addressed from one register, and a group builds its A operand once for
 // ten matrix instructions.  Each wave keeps its (i, component) partial sums in registers
 // across its groups; the sums over nuclei (lanes) and waves are done once at the end, in a
-// fixed order.
+// fixed order.  The kernel also does the work of k_jas_pair for its electrons: per walker
+// g_a(r_j) and the k = 0 sums, per tile the e-e and e-n terms (value, gradient,
+// Laplacian) and the q = 0 sums, two electrons i per wave; so that pass and its trip
+// through memory (g_a(r_j), R0, S) are not needed.
 // ------------------------------------------------------------------------------------
 constexpr int EQ_WAVES = 8;
 constexpr int JREC = 36;  // doubles per electron-j record: G[16], R[16], x, y, z, pad
@@ -909,7 +941,7 @@ constexpr int JREC = 36;  // doubles per electron-j record: G[16], R[16], x, y, 
 // Dynamic LDS of k_jas_een_res, in doubles, every array on a 16-byte boundary.  JP = Ne
 // rounded up to 4, NA16 = 16 x number of nucleus tiles.
 struct EenResLds {
-  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, own, total;
+  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, pee, own, total;
   __host__ __device__ EenResLds(int JP, int NA16, int C, int dim) {
     int o = 0;
     auto take = [&o](int n) { const int at = o; o = (o + n + 1) & ~1; return at; };
@@ -922,6 +954,7 @@ struct EenResLds {
     st = take(NA16 * (C - 1));         // sum over all j of g_a(r_j)^q, [a][q-1]
     cfl = take(dim * NA16);            // c-coefficients, [n][a]
     red = take(EQ_WAVES * EM * 5);     // per-wave row sums
+    pee = take(2 * EQ_WAVES);          // per-wave e-e and e-n value sums of the tile
     int ngroup = 0;                    // groups of units per nucleus tile
     for (int k = 1; k < C; ++k) ngroup += (C - k + 1) / 2;
     own = take((ngroup * (NA16 / 16) + 1) / 2);  // wave of each group, as int
@@ -1213,16 +1246,34 @@ __device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResC
   }
 }
 
+// Butterfly reduce-scatter of vec[0 .. N) over the 32 lanes l32 of a half-wave, step S
+// first: at each step a lane keeps the half of its entries selected by bit S of l32, adds
+// the partner's (l32 ^ S) copy of that half, and hands over the other half.  After the
+// steps S = 16, 8, 4, 2, 1, vec[m] of lane l32 is the sum over the 32 lanes of entry
+// l32 * N / 32 + m, added in a fixed order.
+template <int N, int S>
+__device__ __forceinline__ void een_butterfly(double (&vec)[N], int l32) {
+  if constexpr (S >= 1) {
+    constexpr int H = N * S / 32;  // half of the N S / 16 entries held before step S
+    const bool up = (l32 & S) != 0;
+#pragma unroll
+    for (int m = 0; m < H; ++m) {
+      const double keep = up ? vec[m + H] : vec[m];
+      const double send = up ? vec[m] : vec[m + H];
+      vec[m] = keep + __shfl_xor(send, S, 64);
+    }
+    een_butterfly<N, S / 2>(vec, l32);
+  }
+}
+
 // Persistent: the grid holds one workgroup per CU (the LDS allows no more) and workgroup b
 // takes the items (walker, tile) in [b N / grid, (b + 1) N / grid) in walker-major order, so
 // the per-walker inputs (g_a(r_j) of every nucleus, x_j, the q sums) are copied to LDS once
 // per walker instead of once per tile, the constant ones (nucleus table, c-coefficients)
 // once per workgroup, and no workgroup launch separates two tiles.
 template <int C>
-__global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, const double* __restrict__ R0,
-                                                              const double* __restrict__ gN,
-                                                              const double* __restrict__ Sg,
-                                                              double* __restrict__ Jt, int ntiles) {
+__global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, double* __restrict__ Jt,
+                                                              double* __restrict__ JtP, int ntiles) {
   constexpr int K1 = C - 1;
   constexpr int NT = 64 * EQ_WAVES;
   constexpr int DIM = Lkm<C>::dim;
@@ -1288,51 +1339,141 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     const int tile = item - w32 * ntiles;
     const int64_t w = w32;
     const int i0 = tile * EM;
-    // 1. inputs.  Per walker (when it changes): g_a(r_j) of every nucleus tile and the
-    // k = 0 sums straight from global memory to LDS, x_j through registers.  Per tile: the
-    // q = 0 sums (the rows past the last electron keep stale data: they only reach
-    // discarded rows) and x_i.  One wait for all of them.
+    // 1. inputs.  Per walker (when it changes): x_j, then g_a(r_j) = e^(-kappa_a |r_j - R_a|)
+    // for every nucleus and electron j (zero in the padding), then the k = 0 sums
+    // st[a][q-1] = sum_j g_a(r_j)^q.  Per tile: x_i.
     if (w32 != wcur) {
       wcur = w32;
-      lds_copy16(gch, gN + gn_index(w, 0, 0, NA, JP), NA16 * JP, wave, lane);
-      lds_copy4(st, Sg + w * Nn * K1, Nn * K1, wave, lane);
-      for (int t = Nn * K1 + tid; t < NA16 * K1; t += NT) st[t] = 0.0;  // padding nuclei
       for (int t = tid; t < 4 * JP; t += NT) {
         const int d = t / JP, jj = t - d * JP;  // d = 3: the pad slot
         jr[jj * JREC + 32 + d] = (d < 3 && jj < Ne) ? coord[w * Ne + jj + d * cld] : 0.0;
       }
+      __builtin_amdgcn_s_waitcnt(0);  // the nucleus table and c-coefficients (first item)
+      __syncthreads();
+      for (int t = tid; t < NA16 * JP; t += NT) {  // gch[(tile * JP + j) * 16 + a mod 16]
+        const int al = t & 15, jj = (t >> 4) % JP, a = 16 * ((t >> 4) / JP) + al;
+        double g = 0.0;
+        if (jj < Ne && a < Nn) {
+          const double* pj = jr + jj * JREC + 32;
+          const double dx = pj[0] - nuc[a], dy = pj[1] - nuc[NA16 + a], dz = pj[2] - nuc[2 * NA16 + a];
+          g = exp_neg_deg<13, true>(-nuc[3 * NA16 + a] * sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        gch[t] = g;
+      }
+      __syncthreads();
+      for (int a = tid; a < NA16; a += NT) {
+        double acc[K1];
+#pragma unroll
+        for (int q = 0; q < K1; ++q) acc[q] = 0.0;
+        const double* ga = gch + (a >> 4) * JP * 16 + (a & 15);
+        for (int jj = 0; jj < Ne; ++jj) {
+          const double g = ga[16 * jj];
+          double p = g;
+#pragma unroll
+          for (int q = 0; q < K1; ++q) {
+            acc[q] += p;
+            p *= g;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < K1; ++q) st[a * K1 + q] = acc[q];
+      }
     }
-    lds_copy4(r0t, R0 + (w * Ne + i0) * K1 * 5, min(EM, Ne - i0) * K1 * 5, wave, lane);
     for (int t = tid; t < EM * 3; t += NT) {
       const int il = t / 3, d = t - 3 * il;
       xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
     }
-    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // 2. pair data of the tile and the e-n inverse distances
-    for (int t = tid; t < EM * JP; t += NT) {
-      const int il = t & (EM - 1), jj = t / EM;
-      const int i = i0 + il;
-      double Ge = 0.0, ri = 0.0;
-      if (i < Ne && jj < Ne && i != jj) {
-        const double* pj = jr + jj * JREC + 32;
-        const double dx = xt[il * 3] - pj[0], dy = xt[il * 3 + 1] - pj[1], dz = xt[il * 3 + 2] - pj[2];
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        ri = 1.0 / d;
-        Ge = exp(-v.rescale_ee * d);
+    // 2. pairs of the tile: wave wv takes electrons i = i0 + 2 wv and i0 + 2 wv + 1, 32 lanes
+    // each over the electrons j and the nuclei a.  They give the pair records g_e(r_ij),
+    // 1/r_ij of the units, the e-n inverse distances of the fold, and the e-e and e-n
+    // factors with their gradients and Laplacians and the q = 0 e-e-n sums of electron i,
+    // summed over the 32 lanes by a fixed butterfly (after it, lane l32 holds entry
+    // l32 * NVP / 32 + m of the NVP-entry vector).
+    {
+      // lane index and kappa through an empty asm statement: what derives from them lives
+      // in this phase only
+      int tp = threadIdx.x;
+      double kee = v.rescale_ee;
+      asm volatile("" : "+v"(tp), "+s"(kee));
+      const int il = 2 * wave + ((tp & 63) >> 5), l32 = tp & 31, i = i0 + il;
+      const double kinv = 1.0 / kee;
+      const double xi = xt[il * 3], yi = xt[il * 3 + 1], zi = xt[il * 3 + 2];
+      EeAcc se;
+      EnAcc sn;
+      double r0[K1][5];
+#pragma unroll
+      for (int k = 0; k < K1; ++k)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) r0[k][c] = 0.0;
+      // two electrons j per iteration, so that their independent chains interleave
+      for (int j0 = l32; j0 < JP; j0 += 64) {
+        const int j1 = j0 + 32;
+        double ge0 = 0.0, ri0 = 0.0, ge1 = 0.0, ri1 = 0.0;
+        if (i < Ne && j0 < Ne) {
+          const double* pj = jr + j0 * JREC + 32;
+          ee_pair<K1>(v, i, j0, xi - pj[0], yi - pj[1], zi - pj[2], kee, kinv, se, r0, ge0, ri0);
+        }
+        if (i < Ne && j1 < Ne) {
+          const double* pj = jr + j1 * JREC + 32;
+          ee_pair<K1>(v, i, j1, xi - pj[0], yi - pj[1], zi - pj[2], kee, kinv, se, r0, ge1, ri1);
+        }
+        jr[j0 * JREC + il] = ge0;
+        jr[j0 * JREC + 16 + il] = ri0;
+        if (j1 < JP) {
+          jr[j1 * JREC + il] = ge1;
+          jr[j1 * JREC + 16 + il] = ri1;
+        }
       }
-      jr[jj * JREC + il] = Ge;
-      jr[jj * JREC + 16 + il] = ri;
-    }
-    for (int t = tid; t < EM * NA16; t += NT) {
-      const int il = t / NA16, a = t - il * NA16;
-      double rinv = 0.0;
-      if (i0 + il < Ne && a < Nn) {
-        const double dx = xt[il * 3] - nuc[a], dy = xt[il * 3 + 1] - nuc[NA16 + a],
-                     dz = xt[il * 3 + 2] - nuc[2 * NA16 + a];
-        rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      for (int a = l32; a < NA16; a += 32) {
+        double rinv = 0.0;
+        if (i < Ne && a < Nn) {
+          const double dx = xi - nuc[a], dy = yi - nuc[NA16 + a], dz = zi - nuc[2 * NA16 + a];
+          const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          rinv = 1.0 / d;
+          en_pair(v, (int)v.type_nucl[a], dx, dy, dz, d, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
+        }
+        enr[il * NA16 + a] = rinv;
       }
-      enr[t] = rinv;
+      // the tile's e-e and e-n values: sums over the wave, then over the waves at the end
+      double pe = se.fee, pn = sn.fen;
+#pragma unroll
+      for (int m = 1; m < 64; m <<= 1) {
+        pe += __shfl_xor(pe, m, 64);
+        pn += __shfl_xor(pn, m, 64);
+      }
+      if (lane == 0) {
+        lds[L.pee + 2 * wave] = pe;
+        lds[L.pee + 2 * wave + 1] = pn;
+      }
+      // per-electron vector: R0 (5 K1), e-e GL (4), e-n GL (4), zero padding to NVP
+      constexpr int NV = 5 * K1 + 8, NVP = NV <= 32 ? 32 : 64;
+      double vec[NVP];
+#pragma unroll
+      for (int k = 0; k < K1; ++k)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) vec[5 * k + c] = r0[k][c];
+      vec[5 * K1] = se.g0;
+      vec[5 * K1 + 1] = se.g1;
+      vec[5 * K1 + 2] = se.g2;
+      vec[5 * K1 + 3] = se.g3;
+      vec[5 * K1 + 4] = sn.h0;
+      vec[5 * K1 + 5] = sn.h1;
+      vec[5 * K1 + 6] = sn.h2;
+      vec[5 * K1 + 7] = sn.h3;
+#pragma unroll
+      for (int n = NV; n < NVP; ++n) vec[n] = 0.0;
+      een_butterfly<NVP, 16>(vec, l32);
+#pragma unroll
+      for (int m = 0; m < NVP / 32; ++m) {
+        const int idx = l32 * (NVP / 32) + m;
+        if (idx < 5 * K1) {
+          r0t[il * K1 * 5 + idx] = vec[m];
+        } else if (idx < NV && i < Ne) {
+          double* gl = (idx < 5 * K1 + 4 ? o.fee_gl : o.fen_gl) + w * 4 * Ne;
+          gl[(int64_t)((idx - 5 * K1) & 3) * Ne + i] = vec[m];
+        }
+      }
     }
     __syncthreads();
 
@@ -1411,16 +1552,23 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     }
     __syncthreads();
     if (tid == 0) {
-      double sj = 0.0;
+      double sj = 0.0, se = 0.0, sn = 0.0;
       for (int il = 0; il < EM; ++il) sj += jrow[il];
+      for (int wv = 0; wv < EQ_WAVES; ++wv) {
+        se += lds[L.pee + 2 * wv];
+        sn += lds[L.pee + 2 * wv + 1];
+      }
       Jt[w * ntiles + tile] = sj;
+      JtP[2 * (w * ntiles + tile)] = se;
+      JtP[2 * (w * ntiles + tile) + 1] = sn;
     }
-    __syncthreads();  // xt, jr, enr, r0t and red are rewritten by the next item
+    __syncthreads();  // xt, jr, enr, r0t, red and pee are rewritten by the next item
   }
 }
 
 __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, const double* __restrict__ Jt,
-                                                      int ntiles, int have_een) {
+                                                      const double* __restrict__ JtP, int ntiles, int have_een,
+                                                      int fused) {
   __shared__ double sval;
   const int Ne = (int)v.elec_num;
   const int64_t w = blockIdx.x;
@@ -1429,6 +1577,15 @@ __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, co
     if (have_een)
       for (int t = 0; t < ntiles; ++t) een += Jt[w * ntiles + t];
     o.feen[w] = een;
+    if (fused) {  // the e-e and e-n values of the resident e-e-n kernel, per tile
+      double fee = 0.0, fen = 0.0;
+      for (int t = 0; t < ntiles; ++t) {
+        fee += JtP[2 * (w * ntiles + t)];
+        fen += JtP[2 * (w * ntiles + t) + 1];
+      }
+      o.fee[w] = fee;
+      o.fen[w] = fen;
+    }
     const double J = o.fee[w] + o.fen[w] + een;
     const double val = (J < -100.0) ? 0.0 : exp(J);
     o.value[w] = val;
@@ -1683,7 +1840,7 @@ __global__ void k_en_distance(const double* __restrict__ coord, int64_t P, const
 // Scratch of the Jastrow kernels: gN, R0, Jt and Sg, each starting on a 256-byte boundary
 // (the e-e-n kernel copies gN into LDS 16 bytes per lane).
 struct ScratchLayout {
-  int64_t gN, R0, Jt, Sg, total;
+  int64_t gN, R0, Jt, Sg, JtP, total;
   explicit ScratchLayout(const JasDevView& v) {
     auto up = [](int64_t n) { return (n + 31) & ~int64_t(31); };
     const int64_t C = v.cord, W = v.walk_num;
@@ -1691,14 +1848,15 @@ struct ScratchLayout {
     R0 = up(gN + W * 16 * gn_na(v.nucl_num) * gn_jp(v.elec_num));
     Jt = up(R0 + (C >= 2 ? W * v.elec_num * (C - 1) * 5 : 0));
     Sg = up(Jt + W * een_shape(v).ntiles);
-    total = up(Sg + (C >= 2 ? W * v.nucl_num * (C - 1) : 0));
+    JtP = up(Sg + (C >= 2 ? W * v.nucl_num * (C - 1) : 0));
+    total = up(JtP + 2 * W * een_shape(v).ntiles);
   }
 };
 
 int64_t jastrow_scratch_doubles(const JasDevView& v) { return ScratchLayout(v).total; }
 
 struct ScratchView {
-  double *R0, *gN, *Jt, *Sg;
+  double *R0, *gN, *Jt, *Sg, *JtP;
 };
 
 ScratchView scratch_view(const JasDevView& v, const JasOut& o) {
@@ -1708,10 +1866,21 @@ ScratchView scratch_view(const JasDevView& v, const JasOut& o) {
   sv.R0 = o.scratch + l.R0;
   sv.Jt = o.scratch + l.Jt;
   sv.Sg = o.scratch + l.Sg;
+  sv.JtP = o.scratch + l.JtP;
   return sv;
 }
 
+// Whether the evaluation runs the persistent LDS-resident e-e-n kernel, which then also
+// computes the e-e and e-n terms and the sums k_jas_pair would provide (k_jas_pair is not
+// launched; k_jas_total adds the per-tile value sums).
+bool een_fused(const JasDevView& v) {
+  if (!(v.cord >= 2 && v.dim_c > 0)) return false;
+  const EenShape sh = een_shape(v);
+  return sh.mfma && sh.res && v.walk_num * sh.ntiles <= INT32_MAX;
+}
+
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s) {
+  if (een_fused(v)) return hipSuccess;  // done by the resident e-e-n kernel
   const ScratchView sv = scratch_view(v, o);
   const int pair_threads = (int)std::min<int64_t>(kBlock, 64 * ((v.elec_num + 63) / 64));
   const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kBlock / 64);
@@ -1776,8 +1945,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
     }                                                                                                          \
     if (sh.res)                                                                                                \
       hipLaunchKernelGGL(k_jas_een_res<CC>, dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES),     \
-                         sh.lds, s, v, o, sv.R0, sv.gN, sv.Sg,                                                 \
-                         sv.Jt, sh.ntiles);                                                                    \
+                         sh.lds, s, v, o, sv.Jt, sv.JtP, sh.ntiles);                                           \
     else                                                                                                       \
       hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,    \
                          sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                       \
@@ -1820,7 +1988,8 @@ hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_
   const ScratchView sv = scratch_view(v, o);
   const int have_een = (v.cord >= 2 && v.dim_c > 0) ? 1 : 0;
   const EenShape sh = een_shape(v);
-  hipLaunchKernelGGL(k_jas_total, dim3((unsigned)v.walk_num), dim3(kBlock), 0, s, v, o, sv.Jt, sh.ntiles, have_een);
+  hipLaunchKernelGGL(k_jas_total, dim3((unsigned)v.walk_num), dim3(kBlock), 0, s, v, o, sv.Jt, sv.JtP, sh.ntiles,
+                     have_een, een_fused(v) ? 1 : 0);
   return hipGetLastError();
 }
 
