@@ -957,7 +957,7 @@ struct EenResLds {
     pee = take(2 * EQ_WAVES);          // per-wave e-e and e-n value sums of the tile
     int ngroup = 0;                    // groups of units per nucleus tile
     for (int k = 1; k < C; ++k) ngroup += (C - k + 1) / 2;
-    own = take((ngroup * (NA16 / 16) + 1) / 2);  // wave of each group, as int
+    own = take(ngroup * (NA16 / 16));  // wave of each group, as int: full tiles, then tiles of <= 8 rows
     total = o;
   }
 };
@@ -1098,12 +1098,66 @@ constexpr int een_single_q(int C, int p) { return C - een_single_k(C, p); }
 // k = m), B(b, k = m, column n) and D(b, row m, column n): with the blocks as the four
 // groups of 4 electrons, a lane's A element is the same A(i = li, j = 4 s + lg), its B element
 // is B(j = 4 s + lg, a = li mod 4), and its one result is row lg + 4 (li / 4), column li mod 4.
-template <bool MINI>
-using EenAcc = typename std::conditional<MINI, double, d4>::type;
+// ROWS: the tile holds at most 8 electrons (the last tile of a walker); the same 4x4x4
+// products with the blocks as the four groups of 4 nuclei, once for rows 0-3 and once for
+// rows 4-7 (h = 0, 1): lane l = 16 m + 4 b + n feeds A(i = 4 h + n, j = 4 s + m) and
+// B(j = 4 s + m, a = 4 b + n), the B element of the full tile, and holds row 4 h + lg,
+// column li of the result, so that half the matrix work of a full tile covers its rows.
+enum { EEN_FULL = 0, EEN_MINI = 1, EEN_ROWS = 2 };
+template <int MODE>
+struct EenAccT {
+  using type = d4;
+};
+template <>
+struct EenAccT<EEN_MINI> {
+  using type = double;
+};
+template <>
+struct EenAccT<EEN_ROWS> {
+  using type = double[2];
+};
+template <int MODE>
+using EenAcc = typename EenAccT<MODE>::type;
 
-template <int K, int Q, int NQ, bool MINI>
-__device__ __forceinline__ void een_res_pass(EenAcc<MINI> (&acc)[NQ][5], const double* jr, const double* gt,
+template <int K, int Q, int NQ>
+__device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][5][2], const double* jr, const double* gt,
+                                                  const double* xt, int nsteps, int li, int lg) {
+  const int n4 = li & 3;
+  const double* pj = jr + lg * JREC;
+  const double* pg = gt + lg * 16 + li;
+  const double xa = xt[3 * n4], ya = xt[3 * n4 + 1], za = xt[3 * n4 + 2];
+  const double xb = xt[3 * (4 + n4)], yb = xt[3 * (4 + n4) + 1], zb = xt[3 * (4 + n4) + 2];
+  for (int st = 0; st < nsteps; ++st) {
+    const double Ga = pj[n4], ra = pj[16 + n4], Gb = pj[4 + n4], rb = pj[20 + n4], g = pg[0];
+    const double xj = pj[32], yj = pj[33], zj = pj[34];
+    pj += 4 * JREC;
+    pg += 64;
+    double Gka = Ga, Gkb = Gb, gq = g;
+#pragma unroll
+    for (int k = 2; k <= K; ++k) {
+      Gka *= Ga;
+      Gkb *= Gb;
+    }
+#pragma unroll
+    for (int q = 2; q <= Q; ++q) gq *= g;
+    const double Aa[5] = {Gka, Gka * ((xa - xj) * ra), Gka * ((ya - yj) * ra), Gka * ((za - zj) * ra), Gka * ra};
+    const double Ab[5] = {Gkb, Gkb * ((xb - xj) * rb), Gkb * ((yb - yj) * rb), Gkb * ((zb - zj) * rb), Gkb * rb};
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) {
+      if (n > 0) gq *= g;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        acc[n][c][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(Aa[c], gq, acc[n][c][0], 0, 0, 0);
+        acc[n][c][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(Ab[c], gq, acc[n][c][1], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int K, int Q, int NQ, int MODE>
+__device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const double* jr, const double* gt,
                                              const double (&xi)[3], int nsteps, int li, int lg) {
+  constexpr bool MINI = MODE == EEN_MINI;
   const double* pj = jr + lg * JREC;
   const double* pi = pj + li;
   const double* pg = gt + lg * 16 + (MINI ? (li & 3) : li);
@@ -1189,9 +1243,28 @@ __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int
 // the row or the nucleus, so the rows interleave: a padding nucleus has zero c-coefficients,
 // kappa, 1/R, products and k = 0 sums and adds exactly 0; a row past the last electron is
 // discarded by the caller.
-template <int C, int K, int Q, int NQ, bool MINI>
+template <int C, int K, int Q, int NQ, int MODE>
 __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
-  EenAcc<MINI> acc[NQ][5];
+  constexpr bool MINI = MODE == EEN_MINI;
+  if constexpr (MODE == EEN_ROWS) {
+    double acc[NQ][5][2];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[n][c][0] = acc[n][c][1] = 0.0;
+    een_res_pass_rows<K, Q, NQ>(acc, x.jr, x.gt, x.xt, x.nsteps, x.li, x.lg);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double X[NQ][5];
+#pragma unroll
+      for (int n = 0; n < NQ; ++n)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c][h];
+      een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * h, x.li, X, ra[h]);
+    }
+    return;
+  } else {
+  EenAcc<MODE> acc[NQ][5];
 #pragma unroll
   for (int n = 0; n < NQ; ++n)
 #pragma unroll
@@ -1201,7 +1274,7 @@ __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (
       else
         acc[n][c] = d4{0.0, 0.0, 0.0, 0.0};
     }
-  een_res_pass<K, Q, NQ, MINI>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+  een_res_pass<K, Q, NQ, MODE>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
   if constexpr (MINI) {
     double X[NQ][5];
 #pragma unroll
@@ -1225,10 +1298,11 @@ __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (
       een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * r, x.li, X, ra[r]);
     }
   }
+  }
 }
 
 template <int C, int G = 0>
-__device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResCtx& x, const double (&xi)[3],
+__device__ __forceinline__ void een_res_dispatch(int g, int mode, const EenResCtx& x, const double (&xi)[3],
                                                  double (&ra)[4][5]) {
   constexpr int NPR = een_npair(C);
   if constexpr (G < NPR + een_nsingle(C)) {
@@ -1236,13 +1310,15 @@ __device__ __forceinline__ void een_res_dispatch(int g, bool mini, const EenResC
       constexpr int K = G < NPR ? een_pair_k(C, G) : een_single_k(C, G - NPR);
       constexpr int Q = G < NPR ? een_pair_q(C, G) : een_single_q(C, G - NPR);
       constexpr int NQ = G < NPR ? 2 : 1;
-      if (mini)
-        een_res_group<C, K, Q, NQ, true>(x, xi, ra);
+      if (mode == EEN_MINI)
+        een_res_group<C, K, Q, NQ, EEN_MINI>(x, xi, ra);
+      else if (mode == EEN_ROWS)
+        een_res_group<C, K, Q, NQ, EEN_ROWS>(x, xi, ra);
       else
-        een_res_group<C, K, Q, NQ, false>(x, xi, ra);
+        een_res_group<C, K, Q, NQ, EEN_FULL>(x, xi, ra);
       return;
     }
-    een_res_dispatch<C, G + 1>(g, mini, x, xi, ra);
+    een_res_dispatch<C, G + 1>(g, mode, x, xi, ra);
   }
 }
 
@@ -1289,7 +1365,8 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     // the least loaded wave (lowest index on ties): the pairs of every full nucleus tile,
     // then its single units, then, when the last tile holds at most 4 nuclei, its pairs
     // and single units on the 4x4x4 products (about a third of the work of a full-tile
-    // unit).
+    // unit).  A second table serves the electron tiles of at most 8 rows, whose full
+    // nucleus tiles run on the 4x4x4 products by rows (about two thirds of the work).
     const int Nn = (int)v.nucl_num, NA = gn_na(Nn), NA16 = 16 * NA;
     const EenResLds L(gn_jp((int)v.elec_num), NA16, C, DIM);
     lds_copy4(lds + L.nuc, v.nuc_pad, 4 * NA16, wave, threadIdx.x & 63);
@@ -1298,16 +1375,18 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
       const bool mini = Nn - 16 * (NA - 1) <= 4;
       const int NAF = mini ? NA - 1 : NA;
       const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n3 = n1 + (mini ? NPR + NSG : 0);
-      int load[EQ_WAVES];
-      for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
       int* own = reinterpret_cast<int*>(lds + L.own);
-      for (int it = 0; it < n3; ++it) {
-        const int weight = it < n0 ? 12 : it < n1 ? 6 : it < n1 + NPR ? 4 : 2;
-        int owner = 0;
-        for (int wv = 1; wv < EQ_WAVES; ++wv)
-          if (load[wv] < load[owner]) owner = wv;
-        load[owner] += weight;
-        own[it] = owner;
+      for (int rows = 0; rows < 2; ++rows) {
+        int load[EQ_WAVES];
+        for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
+        for (int it = 0; it < n3; ++it) {
+          const int weight = it < n0 ? (rows ? 8 : 12) : it < n1 ? (rows ? 4 : 6) : it < n1 + NPR ? 4 : 2;
+          int owner = 0;
+          for (int wv = 1; wv < EQ_WAVES; ++wv)
+            if (load[wv] < load[owner]) owner = wv;
+          load[owner] += weight;
+          own[rows * n3 + it] = owner;
+        }
       }
     }
   }
@@ -1483,7 +1562,8 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int c = 0; c < 5; ++c) ra[r][c] = 0.0;
-    const int* own = reinterpret_cast<const int*>(lds + L.own);
+    const bool rows = Ne - i0 <= 8;  // a tile of at most 8 electrons
+    const int* own = reinterpret_cast<const int*>(lds + L.own) + (rows ? n3 : 0);
     for (int it = 0; it < n3; ++it) {
       if (__builtin_amdgcn_readfirstlane(own[it]) != wave) continue;
       // the lane indices pass through an empty asm statement once per group, so that the
@@ -1517,14 +1597,15 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
       } else {
         at = NA - 1, g = NPR + it - n2;
       }
-      const bool mu = it >= n1;  // on the 4x4x4 products
+      const bool mu = it >= n1;  // the last nucleus tile on the 4x4x4 products
+      const int mode = mu ? EEN_MINI : rows ? EEN_ROWS : EEN_FULL;
       x.gt = gch + at * JP * 16;
       x.a = 16 * at + (mu ? (gli & 3) : gli);
       x.nx = nuc[x.a];
       x.ny = nuc[NA16 + x.a];
       x.nz = nuc[2 * NA16 + x.a];
       x.ka = nuc[3 * NA16 + x.a];
-      een_res_dispatch<C>(g, mu, x, xi, ra);
+      een_res_dispatch<C>(g, mode, x, xi, ra);
     }
     // 4. sums over the nuclei (lanes li) and the waves
 #pragma unroll
