@@ -180,7 +180,9 @@ def main():
     if dom == "jastrow_een":
         work = costs["een_flops"] * W
         roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": PEAK["fp64_TFs"], "unit": "TFLOP/s"}
-        roof["note"] = "FP64 FMA work of the e-e-n contraction priced against the 78.6 TFLOP/s FP64 peak"
+        roof["note"] = ("FP64 FMA work of the e-e-n contraction priced against the 78.6 TFLOP/s FP64 peak; "
+                        "where the LDS-resident kernel applies the launch also computes the e-e and e-n "
+                        "terms (their few flops not counted)")
     elif dom == "mo_gemm":
         work = costs["mo_flops"] * W / max(1, ksp[dom]["launches_per_step"])
         peak = {"bf16x3": PEAK["bf16_TFs"] / 6, "fp32": PEAK["fp32_TFs"], "fp64": PEAK["fp64_TFs"]}[args.mode]
