@@ -42,6 +42,15 @@ __device__ __forceinline__ void split3(double x, uint16_t& h, uint16_t& m, uint1
 // y^n, y^(n-1) and y^(n-2) by repeated multiplication (negative powers are returned as 1:
 // they are always multiplied by a zero factor).
 __device__ __forceinline__ void powers3(double y, int n, double& p0, double& p1, double& p2) {
+  if (n <= 3) {
+    // s to f shells without a loop (the lanes of a wave hold different powers): the same
+    // products as the loop, so the same values
+    const double y2 = y * y, y3 = y2 * y;
+    p0 = n == 0 ? 1.0 : n == 1 ? y : n == 2 ? y2 : y3;
+    p1 = n <= 1 ? 1.0 : n == 2 ? y : y2;
+    p2 = n <= 2 ? 1.0 : y;
+    return;
+  }
   p0 = 1.0;
   p1 = 1.0;
   p2 = 1.0;
