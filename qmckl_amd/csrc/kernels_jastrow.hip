@@ -1140,8 +1140,9 @@ __device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][5][2], const
     }
 #pragma unroll
     for (int q = 2; q <= Q; ++q) gq *= g;
-    const double Aa[5] = {Gka, Gka * ((xa - xj) * ra), Gka * ((ya - yj) * ra), Gka * ((za - zj) * ra), Gka * ra};
-    const double Ab[5] = {Gkb, Gkb * ((xb - xj) * rb), Gkb * ((yb - yj) * rb), Gkb * ((zb - zj) * rb), Gkb * rb};
+    const double ta = Gka * ra, tb = Gkb * rb;
+    const double Aa[5] = {Gka, ta * (xa - xj), ta * (ya - yj), ta * (za - zj), ta};
+    const double Ab[5] = {Gkb, tb * (xb - xj), tb * (yb - yj), tb * (zb - zj), tb};
 #pragma unroll
     for (int n = 0; n < NQ; ++n) {
       if (n > 0) gq *= g;
@@ -1163,7 +1164,7 @@ __device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const d
   const double* pg = gt + lg * 16 + (MINI ? (li & 3) : li);
   for (int st = 0; st < nsteps; ++st) {
     const double Ge = pi[0], ri = pi[16], g = pg[0];
-    const double ux = (xi[0] - pj[32]) * ri, uy = (xi[1] - pj[33]) * ri, uz = (xi[2] - pj[34]) * ri;
+    const double dx = xi[0] - pj[32], dy = xi[1] - pj[33], dz = xi[2] - pj[34];
     pj += 4 * JREC;
     pi += 4 * JREC;
     pg += 64;
@@ -1172,8 +1173,10 @@ __device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const d
     for (int k = 2; k <= K; ++k) Gk *= Ge;
 #pragma unroll
     for (int q = 2; q <= Q; ++q) gq *= g;
-    // w = (1, u_x, u_y, u_z, 1/r); u and 1/r are 0 for j == i (1/r is stored 0 there)
-    const double A[5] = {Gk, Gk * ux, Gk * uy, Gk * uz, Gk * ri};
+    // g_e^k w with w = (1, u_x, u_y, u_z, 1/r), u = (r_i - r_j) / r, as g_e^k / r times
+    // (r_i - r_j); u and 1/r are 0 for j == i (1/r is stored 0 there)
+    const double t = Gk * ri;
+    const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
 #pragma unroll
     for (int n = 0; n < NQ; ++n) {
       if (n > 0) gq *= g;
