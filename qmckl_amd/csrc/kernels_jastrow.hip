@@ -108,6 +108,27 @@ __host__ __device__ __forceinline__ int gn_jp(int64_t Ne) { return (int)((Ne + 3
 struct EeAcc {
   double fee = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
 };
+
+// P(x) = sum_{k=2..n} c_k x^k with its first derivative and half its second derivative, by
+// Horner's rule carried for the derivatives (the k = 1 and k = 0 coefficients are 0).
+__device__ __forceinline__ void poly2_horner(const double* c, int n, double x, double& p, double& dp, double& hp) {
+  p = 0.0;
+  dp = 0.0;
+  hp = 0.0;
+  if (n < 2) return;
+  p = c[n];
+  for (int k = n - 1; k >= 2; --k) {
+    hp = hp * x + dp;
+    dp = dp * x + p;
+    p = p * x + c[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {  // the zero coefficients of x^1 and x^0
+    hp = hp * x + dp;
+    dp = dp * x + p;
+    p = p * x;
+  }
+}
 template <int K1>
 __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, double dx, double dy, double dz,
                                         double kee, double kinv, EeAcc& s, double (&r0)[K1 > 0 ? K1 : 1][5],
@@ -118,6 +139,8 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   const double d = use ? sqrt(dx * dx + dy * dy + dz * dz) : 1.0;
   const double e = exp_neg_deg<13, true>(-kee * d);
   const double x = (1.0 - e) * kinv;  // rescaled distance, org/qmckl_distance.org:975-985
+  double pq, dq, hq;  // sum_{k>=2} b_k x^k, its derivative, half its second derivative
+  poly2_horner(b, bord, x, pq, dq, hq);
   if (j < i) {
     const bool same = (v.spin_independent != 1) && (i < up || j >= up);
     if (v.spin_independent == 1)
@@ -126,11 +149,7 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
       s.fee += 0.5 * b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb0;
     else
       s.fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
-    double xk = x;
-    for (int k = 2; k <= bord; ++k) {
-      xk *= x;
-      s.fee += b[k] * xk;
-    }
+    s.fee += pq;
   }
   const double dd = d > 1.e-20 ? d : 1.e-20;
   const double dinv = 1.0 / dd;
@@ -151,16 +170,11 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   s.g1 += f * gy;
   s.g2 += f * gz;
   s.g3 += f * (gl4 - 2.0 * b[1] * gc2 * invd);
-  double kf = 2.0, xp = 1.0;
-  for (int k = 2; k <= bord; ++k) {
-    f = b[k] * kf * xp;
-    s.g0 += f * x * gx;
-    s.g1 += f * x * gy;
-    s.g2 += f * x * gz;
-    s.g3 += f * (x * gl4 + (kf - 1.0) * gc2);
-    xp *= x;
-    kf += 1.0;
-  }
+  // sum_{k>=2} b_k (k x^(k-1) grad x, k x^(k-1) lap x + k (k-1) x^(k-2) |grad x|^2)
+  s.g0 += dq * gx;
+  s.g1 += dq * gy;
+  s.g2 += dq * gz;
+  s.g3 += dq * gl4 + 2.0 * hq * gc2;
   const double rinv = 1.0 / d;
   const double ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
   const double em = use ? e : 0.0;
@@ -191,12 +205,10 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
   const double* av = v.a_vector + t * (aord + 1);
   const double ka = v.kappa_en[t];
   const double x = (1.0 - e) * (1.0 / ka);
+  double pq, dq, hq;  // sum_{p>=2} a_p x^p, its derivative, half its second derivative
+  poly2_horner(av, aord, x, pq, dq, hq);
   s.fen += av[0] * x / (1.0 + av[1] * x) - v.asymp_jasa[t];
-  double xk = x;
-  for (int p = 2; p <= aord; ++p) {
-    xk *= x;
-    s.fen += av[p] * xk;
-  }
+  s.fen += pq;
   const double dd = d > 1.e-20 ? d : 1.e-20;
   const double dinv = 1.0 / dd;
   const double ed = (d > 1.e-20) ? e : exp_neg_deg<13, true>(-ka * dd);
@@ -210,16 +222,10 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
   s.h1 += f * gy;
   s.h2 += f * gz;
   s.h3 += f * (gl4 - 2.0 * av[1] * gc2 * invd);
-  double kf = 2.0, xp = 1.0;
-  for (int k = 2; k <= aord; ++k) {
-    f = av[k] * kf * xp;
-    s.h0 += f * x * gx;
-    s.h1 += f * x * gy;
-    s.h2 += f * x * gz;
-    s.h3 += f * (x * gl4 + (kf - 1.0) * gc2);
-    xp *= x;
-    kf += 1.0;
-  }
+  s.h0 += dq * gx;
+  s.h1 += dq * gy;
+  s.h2 += dq * gz;
+  s.h3 += dq * gl4 + 2.0 * hq * gc2;
 }
 
 // ------------------------------------------------------------------------------------
