@@ -1035,25 +1035,62 @@ struct EenEp {
   double h0, h1, h2, kr, ka2;
 };
 
+// Which role sums of unit (K, Q) can be non-zero, from the c-vector layout alone: bit 0 T
+// (any term), bit 1 TJ (a term that carries J), bit 2 T1 and T2 (a term with PG != 0).
+// ROLE 0: m, 1: r, 2: z, with the conditions of een_tsums.
+template <int C, int K, int Q, int ROLE>
+constexpr unsigned een_role_mask() {
+  constexpr Lkm<C> T{};
+  unsigned f = 0;
+  for (int N = 0; N < T.dim; ++N) {
+    const int l = T.l[N], k = T.k[N], mm = T.m[N];
+    bool plain, jrole;  // een_tadd<mm + l, false> and een_tadd<mm, true>
+    if (ROLE == 0) {
+      plain = k == K && mm == Q;
+      jrole = k == K && mm + l == Q;
+    } else if (ROLE == 1) {
+      plain = Q == 1 && k == K && mm == 0;
+      jrole = Q == 1 && k == K && mm + l == 0;
+    } else {
+      plain = K == 1 && k == 0 && mm == Q;
+      jrole = K == 1 && k == 0 && mm + l == Q;
+    }
+    if (plain) f |= (mm + l != 0) ? 5u : 1u;
+    if (jrole) f |= (mm != 0) ? 7u : 3u;
+  }
+  return f;
+}
+
 // Adds to the row outputs o = (J, grad, lap) the epilogue of role sums s over X, which is
 // linear in the partial sums of k_jas_een_mfma (EenRow):
 //   o0 += TJ X0,   o_c += h_c T1 X0 - kk T X_c,
 //   o4 += ka2 T2 X0 - kr T1 X0 + kk^2 T X0 - 2 kk T1 (h . X_123) - 2 kk T X4;
-// VALUE_ONLY (the X[0][Q] roles, k = 0): the terms without kk.
-template <bool VALUE_ONLY>
+// VALUE_ONLY (the X[0][Q] roles, k = 0): the terms without kk.  F (een_role_mask) drops the
+// terms whose role sums are zero by construction: they would add exact zeros, which the
+// compiler must otherwise compute (kk or X could be infinite).
+template <bool VALUE_ONLY, unsigned F = 7u>
 __device__ __forceinline__ void een_apply(double (&o)[5], const EenTs& s, const double* X, double kk, const EenEp& e) {
-  const double t1x = s.T1 * X[0];
-  o[0] += s.TJ * X[0];
-  o[1] += e.h0 * t1x;
-  o[2] += e.h1 * t1x;
-  o[3] += e.h2 * t1x;
-  o[4] += e.ka2 * s.T2 * X[0] - e.kr * t1x;
-  if constexpr (!VALUE_ONLY) {
-    const double kt = kk * s.T, kt1 = kk * s.T1;
-    o[1] -= kt * X[1];
-    o[2] -= kt * X[2];
-    o[3] -= kt * X[3];
-    o[4] += kk * kt * X[0] - 2.0 * kt1 * (e.h0 * X[1] + e.h1 * X[2] + e.h2 * X[3]) - 2.0 * kt * X[4];
+  if constexpr ((F & 1u) != 0) {
+    if constexpr ((F & 2u) != 0) o[0] += s.TJ * X[0];
+    if constexpr ((F & 4u) != 0) {
+      const double t1x = s.T1 * X[0];
+      o[1] += e.h0 * t1x;
+      o[2] += e.h1 * t1x;
+      o[3] += e.h2 * t1x;
+      o[4] += e.ka2 * s.T2 * X[0] - e.kr * t1x;
+    }
+    if constexpr (!VALUE_ONLY) {
+      const double kt = kk * s.T;
+      o[1] -= kt * X[1];
+      o[2] -= kt * X[2];
+      o[3] -= kt * X[3];
+      if constexpr ((F & 4u) != 0) {
+        const double kt1 = kk * s.T1;
+        o[4] += kk * kt * X[0] - 2.0 * kt1 * (e.h0 * X[1] + e.h1 * X[2] + e.h2 * X[3]) - 2.0 * kt * X[4];
+      } else {
+        o[4] += kk * kt * X[0] - 2.0 * kt * X[4];
+      }
+    }
   }
 }
 
@@ -1168,12 +1205,16 @@ __device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const d
   const double* pj = jr + lg * JREC;
   const double* pi = pj + li;
   const double* pg = gt + lg * 16 + (MINI ? (li & 3) : li);
+  // the next step's operands are read before this step's products are issued
+  double nGe = pi[0], nri = pi[16], ng = pg[0], nxj = pj[32], nyj = pj[33], nzj = pj[34];
   for (int st = 0; st < nsteps; ++st) {
-    const double Ge = pi[0], ri = pi[16], g = pg[0];
-    const double dx = xi[0] - pj[32], dy = xi[1] - pj[33], dz = xi[2] - pj[34];
-    pj += 4 * JREC;
-    pi += 4 * JREC;
-    pg += 64;
+    const double Ge = nGe, ri = nri, g = ng;
+    const double dx = xi[0] - nxj, dy = xi[1] - nyj, dz = xi[2] - nzj;
+    const bool more = st + 1 < nsteps;  // the last step re-reads its own records
+    pj += more ? 4 * JREC : 0;
+    pi += more ? 4 * JREC : 0;
+    pg += more ? 64 : 0;
+    nGe = pi[0], nri = pi[16], ng = pg[0], nxj = pj[32], nyj = pj[33], nzj = pj[34];
     double Gk = Ge, gq = g;
 #pragma unroll
     for (int k = 2; k <= K; ++k) Gk *= Ge;
@@ -1205,6 +1246,25 @@ struct EenResCtx {
   int nsteps, li, lg, i0, Ne, a, NA16;
 };
 
+// The fold of unit (K, Q) of row il: its products X with the m roles, X[K][0] (the q = 0 sums
+// of the tile electrons) with the r roles when Q = 1, X[0][Q] with the z roles when K = 1.
+template <int C, int K, int Q>
+__device__ __forceinline__ void een_fold_unit(const EenResCtx& x, int il, const double (&X)[5], const EenTs& tm,
+                                              const EenTs& tr, const EenTs& tz, const double* gp, double kk,
+                                              const EenEp& e, double (&out)[5]) {
+  constexpr int K1 = C - 1;
+  een_apply<false, een_role_mask<C, K, Q, 0>()>(out, tm, X, kk, e);
+  if constexpr (Q == 1) {
+    const double* XR = x.r0t + il * K1 * 5 + (K - 1) * 5;
+    een_apply<false, een_role_mask<C, K, Q, 1>()>(out, tr, XR, kk, e);
+  }
+  if constexpr (K == 1) {
+    // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
+    const double X0[1] = {x.st[x.a * K1 + Q - 1] - __builtin_fma(gp[Q - 1], gp[1], 0.0)};
+    een_apply<true, een_role_mask<C, K, Q, 2>()>(out, tz, X0, 0.0, e);
+  }
+}
+
 // Fold of row r of the products (X, 5 components per unit of the group) of lane column
 // x.a into out: the role sums of each unit applied to its products, and with q = 1 or k = 1
 // the roles on the q = 0 and k = 0 sums.  gcol is the lane's column in the tile.
@@ -1229,19 +1289,13 @@ __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int
 #pragma unroll
   for (int n = 0; n < NQ; ++n) {
     EenTs tm, tr, tz;
-    if (n == 0)
+    if (n == 0) {
       een_tsums<C, K, Q>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
-    else
-      een_tsums<C, K, (NQ > 1 ? Q + 1 : Q)>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
-    een_apply<false>(out, tm, X[n], kk, e);
-    if (Q + n == 1) {  // X[K][0], the q = 0 sums of the tile electrons
-      const double* XR = x.r0t + il * K1 * 5 + (K - 1) * 5;
-      een_apply<false>(out, tr, XR, kk, e);
-    }
-    if (K == 1) {
-      // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
-      const double X0[1] = {x.st[x.a * K1 + Q + n - 1] - __builtin_fma(gp[Q + n - 1], gp[1], 0.0)};
-      een_apply<true>(out, tz, X0, 0.0, e);
+      een_fold_unit<C, K, Q>(x, il, X[n], tm, tr, tz, gp, kk, e, out);
+    } else {
+      constexpr int Q1 = NQ > 1 ? Q + 1 : Q;
+      een_tsums<C, K, Q1>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
+      een_fold_unit<C, K, Q1>(x, il, X[n], tm, tr, tz, gp, kk, e, out);
     }
   }
 }
