@@ -68,16 +68,61 @@ struct Lkm {
   }
 };
 
+// Cross-lane exchanges of doubles that stay off the LDS path (a shuffle through
+// ds_bpermute costs an LDS round trip): DPP moves within a row of 16 lanes, and gfx950's
+// v_permlane16_swap / v_permlane32_swap across rows.  Lane layouts, as measured on the
+// MI355X: row_mirror sends lane i of a row to lane 15 - i (xor 15), row_half_mirror lane i
+// of 8 to 7 - i (xor 7), quad_perm(1,0,3,2) is xor 1 and quad_perm(2,3,0,1) xor 2;
+// permlane16_swap(a, b) returns a with its odd rows (lanes 16-31, 48-63) replaced by the
+// even rows of b, and b with its even rows replaced by the odd rows of a; permlane32_swap
+// the same with the halves of the wave.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_XOR7 = 0x141, DPP_XOR15 = 0x140;
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// a + b after the 16-lane (ROWS16) or 32-lane swap: in the even rows (halves) a's own value
+// plus b's from the odd ones, in the odd rows (halves) a's value from the even ones plus b's own
+template <bool ROWS16>
+__device__ __forceinline__ double swap_sum_f64(double a, double b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  uint32_t alo = (uint32_t)ua, ahi = (uint32_t)(ua >> 32), blo = (uint32_t)ub, bhi = (uint32_t)(ub >> 32);
+  if constexpr (ROWS16) {
+    const auto l = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    alo = l[0], blo = l[1], ahi = h[0], bhi = h[1];
+  } else {
+    const auto l = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    alo = l[0], blo = l[1], ahi = h[0], bhi = h[1];
+  }
+  return __builtin_bit_cast(double, ((uint64_t)ahi << 32) | alo) + __builtin_bit_cast(double, ((uint64_t)bhi << 32) | blo);
+}
+// Sum over the 16 lanes of a row, in every lane of it, by the tree of xor 1, 2, 4, 8: after
+// the first two steps a quad holds one value, so xor 7 and xor 15 meet the values of xor 4
+// and xor 8.
+__device__ __forceinline__ double row_sum16(double x) {
+  x += dpp_f64<DPP_XOR1>(x);
+  x += dpp_f64<DPP_XOR2>(x);
+  x += dpp_f64<DPP_XOR7>(x);
+  x += dpp_f64<DPP_XOR15>(x);
+  return x;
+}
+// Sum over the 64 lanes of the wave, in every lane, by the tree of xor 1, 2, ..., 32.
+__device__ __forceinline__ double wave_sum64(double x) {
+  x = row_sum16(x);
+  x = swap_sum_f64<true>(x, x);
+  return swap_sum_f64<false>(x, x);
+}
+
 // Deterministic sum over the lanes of a block of any multiple of 64 threads: a fixed
 // shuffle tree per wave, then the wave partials in wave order.  red holds one double per
 // wave.
 __device__ __forceinline__ double block_sum_waves(double x, double* red) {
-  x += __shfl_xor(x, 1, 64);
-  x += __shfl_xor(x, 2, 64);
-  x += __shfl_xor(x, 4, 64);
-  x += __shfl_xor(x, 8, 64);
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 32, 64);
+  x = wave_sum64(x);
   const int nw = (int)blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
   __syncthreads();
@@ -887,12 +932,7 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
       // sum over the 16 nuclei of the tile: lanes li = 0..15 of this row group
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
-        double x = out[c];
-        x += __shfl_xor(x, 1, 64);
-        x += __shfl_xor(x, 2, 64);
-        x += __shfl_xor(x, 4, 64);
-        x += __shfl_xor(x, 8, 64);
-        out[c] = x;
+        out[c] = row_sum16(out[c]);
       }
       if (li == 0)
 #pragma unroll
@@ -1044,7 +1084,7 @@ constexpr unsigned een_role_mask() {
   unsigned f = 0;
   for (int N = 0; N < T.dim; ++N) {
     const int l = T.l[N], k = T.k[N], mm = T.m[N];
-    bool plain, jrole;  // een_tadd<mm + l, false> and een_tadd<mm, true>
+    bool plain = false, jrole = false;  // een_tadd<mm + l, false> and een_tadd<mm, true>
     if (ROLE == 0) {
       plain = k == K && mm == Q;
       jrole = k == K && mm + l == Q;
@@ -1387,19 +1427,30 @@ __device__ __forceinline__ void een_res_dispatch(int g, int mode, const EenResCt
 
 // Butterfly reduce-scatter of vec[0 .. N) over the 32 lanes l32 of a half-wave, step S
 // first: at each step a lane keeps the half of its entries selected by bit S of l32, adds
-// the partner's (l32 ^ S) copy of that half, and hands over the other half.  After the
+// the partner's copy of that half (a lane that differs in bit S), and hands over the other
+// half.  After the
 // steps S = 16, 8, 4, 2, 1, vec[m] of lane l32 is the sum over the 32 lanes of entry
 // l32 * N / 32 + m, added in a fixed order.
 template <int N, int S>
 __device__ __forceinline__ void een_butterfly(double (&vec)[N], int l32) {
   if constexpr (S >= 1) {
     constexpr int H = N * S / 32;  // half of the N S / 16 entries held before step S
-    const bool up = (l32 & S) != 0;
+    if constexpr (S == 16) {
+      // the rows of 16 lanes swap halves: the lower row keeps vec[m] and receives the upper
+      // row's vec[m], the upper row keeps vec[m + H] and receives the lower row's
 #pragma unroll
-    for (int m = 0; m < H; ++m) {
-      const double keep = up ? vec[m + H] : vec[m];
-      const double send = up ? vec[m] : vec[m + H];
-      vec[m] = keep + __shfl_xor(send, S, 64);
+      for (int m = 0; m < H; ++m) vec[m] = swap_sum_f64<true>(vec[m], vec[m + H]);
+    } else {
+      // within a row the partner is l32 ^ 15, ^ 7, ^ 2, ^ 1 (S = 8, 4, 2, 1): it differs in
+      // bit S and holds the same entries; with the rows these partners reach all 32 lanes
+      constexpr int CTRL = S == 8 ? DPP_XOR15 : S == 4 ? DPP_XOR7 : S == 2 ? DPP_XOR2 : DPP_XOR1;
+      const bool up = (l32 & S) != 0;
+#pragma unroll
+      for (int m = 0; m < H; ++m) {
+        const double keep = up ? vec[m + H] : vec[m];
+        const double send = up ? vec[m] : vec[m + H];
+        vec[m] = keep + dpp_f64<CTRL>(send);
+      }
     }
     een_butterfly<N, S / 2>(vec, l32);
   }
@@ -1578,12 +1629,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
         enr[il * NA16 + a] = rinv;
       }
       // the tile's e-e and e-n values: sums over the wave, then over the waves at the end
-      double pe = se.fee, pn = sn.fen;
-#pragma unroll
-      for (int m = 1; m < 64; m <<= 1) {
-        pe += __shfl_xor(pe, m, 64);
-        pn += __shfl_xor(pn, m, 64);
-      }
+      const double pe = wave_sum64(se.fee), pn = wave_sum64(sn.fen);
       if (lane == 0) {
         lds[L.pee + 2 * wave] = pe;
         lds[L.pee + 2 * wave + 1] = pn;
@@ -1651,7 +1697,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
       x.NA16 = NA16;
       int at, g;
       if (it < n0) {
-        at = it / NPR, g = it - at * NPR;
+        at = it / (NPR > 0 ? NPR : 1), g = it - at * NPR;  // n0 = 0 when NPR = 0
       } else if (it < n1) {
         const int r = it - n0;
         at = r / NSG, g = NPR + r - at * NSG;
@@ -1675,11 +1721,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
-        double y = ra[r][c];
-        y += __shfl_xor(y, 1, 64);
-        y += __shfl_xor(y, 2, 64);
-        y += __shfl_xor(y, 4, 64);
-        y += __shfl_xor(y, 8, 64);
+        const double y = row_sum16(ra[r][c]);
         if (li == 0) red[(wave * EM + lg + 4 * r) * 5 + c] = y;
       }
     __syncthreads();
