@@ -186,14 +186,16 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   const double x = (1.0 - e) * kinv;  // rescaled distance, org/qmckl_distance.org:975-985
   double pq, dq, hq;  // sum_{k>=2} b_k x^k, its derivative, half its second derivative
   poly2_horner(b, bord, x, pq, dq, hq);
+  // one reciprocal of the Pade denominator for the value and the derivatives
+  const double invd = 1.0 / (1.0 + b[1] * x), invd2 = invd * invd;
   if (j < i) {
     const bool same = (v.spin_independent != 1) && (i < up || j >= up);
     if (v.spin_independent == 1)
-      s.fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
+      s.fee += b[0] * x * invd - v.asymp_jasb1;
     else if (same)
-      s.fee += 0.5 * b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb0;
+      s.fee += 0.5 * b[0] * x * invd - v.asymp_jasb0;
     else
-      s.fee += b[0] * x / (1.0 + b[1] * x) - v.asymp_jasb1;
+      s.fee += b[0] * x * invd - v.asymp_jasb1;
     s.fee += pq;
   }
   const double dd = d > 1.e-20 ? d : 1.e-20;
@@ -201,8 +203,6 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   const double ed = use ? ((d > 1.e-20) ? e : exp_neg_deg<13, true>(-kee * dd)) : 0.0;
   const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
   const double gl4 = (2.0 * dinv - kee) * ed;
-  const double denom = 1.0 + b[1] * x;
-  const double invd = 1.0 / denom, invd2 = invd * invd;
   const double gc2 = gx * gx + gy * gy + gz * gz;
   double f;
   if (v.spin_independent == 1)
@@ -220,7 +220,8 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   s.g1 += dq * gy;
   s.g2 += dq * gz;
   s.g3 += dq * gl4 + 2.0 * hq * gc2;
-  const double rinv = 1.0 / d;
+  double rinv = dinv;  // 1 / d, the same quotient unless d <= 1e-20
+  if (!(d > 1.e-20)) rinv = 1.0 / d;
   const double ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
   const double em = use ? e : 0.0;
   double gk = 1.0;
@@ -244,23 +245,24 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
 struct EnAcc {
   double fen = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;
 };
-__device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
-                                        double e, EnAcc& s) {
+// Returns 1 / d (the caller's inverse distance).
+__device__ __forceinline__ double en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
+                                          double e, EnAcc& s) {
   const int aord = (int)v.aord;
   const double* av = v.a_vector + t * (aord + 1);
   const double ka = v.kappa_en[t];
   const double x = (1.0 - e) * (1.0 / ka);
   double pq, dq, hq;  // sum_{p>=2} a_p x^p, its derivative, half its second derivative
   poly2_horner(av, aord, x, pq, dq, hq);
-  s.fen += av[0] * x / (1.0 + av[1] * x) - v.asymp_jasa[t];
+  // one reciprocal of the Pade denominator for the value and the derivatives
+  const double invd = 1.0 / (1.0 + av[1] * x), invd2 = invd * invd;
+  s.fen += av[0] * x * invd - v.asymp_jasa[t];
   s.fen += pq;
   const double dd = d > 1.e-20 ? d : 1.e-20;
   const double dinv = 1.0 / dd;
   const double ed = (d > 1.e-20) ? e : exp_neg_deg<13, true>(-ka * dd);
   const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
   const double gl4 = (2.0 * dinv - ka) * ed;
-  const double denom = 1.0 + av[1] * x;
-  const double invd = 1.0 / denom, invd2 = invd * invd;
   double f = av[0] * invd2;
   const double gc2 = gx * gx + gy * gy + gz * gz;
   s.h0 += f * gx;
@@ -271,6 +273,9 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
   s.h1 += dq * gy;
   s.h2 += dq * gz;
   s.h3 += dq * gl4 + 2.0 * hq * gc2;
+  double rinv = dinv;  // the same quotient unless d <= 1e-20
+  if (!(d > 1.e-20)) rinv = 1.0 / d;
+  return rinv;
 }
 
 // ------------------------------------------------------------------------------------
@@ -1623,8 +1628,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
         if (i < Ne && a < Nn) {
           const double dx = xi - nuc[a], dy = yi - nuc[NA16 + a], dz = zi - nuc[2 * NA16 + a];
           const double d = sqrt(dx * dx + dy * dy + dz * dz);
-          rinv = 1.0 / d;
-          en_pair(v, (int)v.type_nucl[a], dx, dy, dz, d, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
+          rinv = en_pair(v, (int)v.type_nucl[a], dx, dy, dz, d, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
         }
         enr[il * NA16 + a] = rinv;
       }
