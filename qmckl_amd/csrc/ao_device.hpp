@@ -43,12 +43,14 @@ __device__ __forceinline__ void split3(double x, uint16_t& h, uint16_t& m, uint1
 // they are always multiplied by a zero factor).
 __device__ __forceinline__ void powers3(double y, int n, double& p0, double& p1, double& p2) {
   if (n <= 3) {
-    // s to f shells without a loop (the lanes of a wave hold different powers): the same
-    // products as the loop, so the same values
-    const double y2 = y * y, y3 = y2 * y;
-    p0 = n == 0 ? 1.0 : n == 1 ? y : n == 2 ? y2 : y3;
-    p1 = n <= 1 ? 1.0 : n == 2 ? y : y2;
-    p2 = n <= 2 ? 1.0 : y;
+    // s to f shells without a loop (the lanes of a wave hold different powers): y^n as
+    // f1 f2 f3 with f_k = y when n >= k, else 1, three selects; the same products as the
+    // loop ((y y) y), so the same values
+    const double f1 = n >= 1 ? y : 1.0, f2 = n >= 2 ? y : 1.0, f3 = n >= 3 ? y : 1.0;
+    const double f23 = f2 * f3;
+    p0 = f1 * f23;
+    p1 = f23;
+    p2 = f3;
     return;
   }
   p0 = 1.0;
