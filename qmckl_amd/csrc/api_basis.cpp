@@ -226,7 +226,9 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     d.mo_pad = pad;
     d.k_pad = kpad;
     up(d.coef_t, ct.data(), 8 * ct.size());
-    // the operand layouts of the fused kernel: per (group of G column tiles, 32-AO chunk)
+    // the operand layouts of the fused kernel: per (group of G column tiles, 32-AO chunk),
+    // row k scaled by ao_factor[k] (the fused kernel's AO values leave the factor out:
+    // sum_k (f_k P_k) C_kj = sum_k P_k (f_k C_kj))
     const int64_t nct = pad / 16, nchunk = kpad / AO_MO_KC;
     const int G = (int)std::min<int64_t>(AO_MO_MAX_G, nct);
     const int64_t ngroup = (nct + G - 1) / G, NG16 = 16 * G;
@@ -238,8 +240,9 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     for (int64_t ch = 0; ch < nchunk; ++ch)
       for (int kk = 0; kk < AO_MO_KC; ++kk) {
         const int64_t k = ch * AO_MO_KC + kk;
+        const double fk = k < Nao ? t.ao_factor[k] : 0.0;
         for (int64_t col = 0; col < pad; ++col) {
-          const double x = ct[k * pad + col];
+          const double x = ct[k * pad + col] * fk;
           const int64_t g = col / NG16, n = col - g * NG16;
           const size_t blk = (size_t)(g * nchunk + ch);
           b64[(blk * AO_MO_KC + kk) * NG16 + n] = x;

@@ -50,7 +50,8 @@ struct DevConst {
   DevBuf nucl_coord;  // [3][nucl]
   DevBuf coef_t;      // MO coefficients transposed [k_pad][mo_pad] (FP64, zero-padded)
   // C_t in the per-chunk operand layouts of the fused AO->MO kernel, [group][chunk][...]:
-  // FP64 / FP32 [32][16 G], BF16X3 [3][16 G][AO_MO_BF_LD] (zero-padded)
+  // FP64 / FP32 [32][16 G], BF16X3 [3][16 G][AO_MO_BF_LD] (zero-padded), row k scaled by
+  // ao_factor[k]
   DevBuf coef_ch64, coef_ch32, coef_chbf;
   int64_t mo_pad = 0, k_pad = 0;
   int ao_mo_g = 1, ao_mo_ngroup = 1;  // column tiles per pass of the fused kernel, passes
@@ -232,8 +233,9 @@ hipError_t launch_split_bf16x3(const double* src, int64_t n, uint16_t* dst, int6
 // desc_off[c+1]) (doubles, an even count):
 //   [ns, na, np, 0], na nuclei [x, y, z, range], ns shells [nucleus + 256 nprim, first prim],
 //   np primitives [alpha, coef], AO_MO_KC AOs [shell + 256 (a | b << 8 | c << 16), factor]
-// (indices local to the chunk; AOs past ao_num have factor 0).  B holds C_t chunk by chunk
-// in the operand layout of the mode (see DevConst), b_chunk_bytes per (group, chunk); the MO
+// (indices local to the chunk; AOs past ao_num have factor 0; the kernel does not read the
+// factor).  B holds C_t chunk by chunk in the operand layout of the mode (see DevConst), its
+// row k scaled by ao_factor[k] (zero past ao_num), b_chunk_bytes per (group, chunk); the MO
 // columns are taken G tiles of 16 at a time, ngroup groups.  C: [points][5][ldc], columns < N.
 constexpr int AO_MO_KC = 32;
 constexpr int AO_MO_MAX_G = 8;  // column tiles per pass

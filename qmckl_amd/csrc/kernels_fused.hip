@@ -101,11 +101,11 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
 }
 
 // AO VGL of AO kk of the chunk at point (px, py, pz), whose shells' radial VGL are staged at
-// rad, `stride` doubles per shell; org/qmckl_ao.org:8530-8565.
+// rad, `stride` doubles per shell; org/qmckl_ao.org:8530-8565, without the normalisation
+// factor ao_factor[k], which the chunk operands of C_t carry (row k scaled by it).
 __device__ __forceinline__ void chunk_ao_pt(const Chunk& c, int kk, double px, double py, double pz,
                                             const double* rad, int stride, double r[5]) {
   const double code = c.ao[2 * kk];
-  const double f = c.ao[2 * kk + 1];
   const uint32_t u = (uint32_t)code;
   const int s = u & 255, pw = u >> 8;
   const int nl = (int)c.shl[2 * s] & 255;
@@ -114,11 +114,11 @@ __device__ __forceinline__ void chunk_ao_pt(const Chunk& c, int kk, double px, d
   monomial_vgl(x + 1.e-20, y + 1.e-20, z + 1.e-20, pw & 0xff, (pw >> 8) & 0xff, (pw >> 16) & 0xff, pv);
   const double* sh = rad + stride * s;
   const double s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3], s4 = sh[4];
-  r[0] = pv[0] * s0 * f;
-  r[1] = (pv[1] * s0 + pv[0] * s1) * f;
-  r[2] = (pv[2] * s0 + pv[0] * s2) * f;
-  r[3] = (pv[3] * s0 + pv[0] * s3) * f;
-  r[4] = (pv[4] * s0 + pv[0] * s4 + 2.0 * (pv[1] * s1 + pv[2] * s2 + pv[3] * s3)) * f;
+  r[0] = pv[0] * s0;
+  r[1] = pv[1] * s0 + pv[0] * s1;
+  r[2] = pv[2] * s0 + pv[0] * s2;
+  r[3] = pv[3] * s0 + pv[0] * s3;
+  r[4] = pv[4] * s0 + pv[0] * s4 + 2.0 * (pv[1] * s1 + pv[2] * s2 + pv[3] * s3);
 }
 // The kernel.  TP = 16 points per workgroup (5 row tiles, point-major rows 5 p + c), KC = 32
 // AOs per chunk.  The 5 x G products of a chunk (row tile r, column tile j) go to the waves
