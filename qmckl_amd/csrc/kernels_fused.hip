@@ -78,7 +78,9 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
   const double code = c.shl[2 * s];
   const int nl = (int)code & 255, np = (int)code >> 8, p0 = (int)c.shl[2 * s + 1];
   const double r2 = x * x + y * y + z * z;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+  // per primitive: e = c e^(-alpha r^2) and t = -2 alpha e; the gradient is (sum t) (x, y, z)
+  // and the Laplacian 3 sum t - 2 sum t alpha r^2, applied once per shell
+  double v0 = 0, st = 0, st2 = 0;
   if (!(r2 > cutoff * c.nuc[4 * nl + 3])) {
     for (int i = 0; i < np; ++i) {
       const double al = c.prm[2 * (p0 + i)];
@@ -87,17 +89,15 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
       const double e = c.prm[2 * (p0 + i) + 1] * exp_neg_deg<DEG>(-ar2);
       const double two_a = -2.0 * al * e;
       v0 += e;
-      v1 += two_a * x;
-      v2 += two_a * y;
-      v3 += two_a * z;
-      v4 += two_a * (3.0 - 2.0 * ar2);
+      st += two_a;
+      st2 += two_a * ar2;
     }
   }
   r[0] = v0;
-  r[1] = v1;
-  r[2] = v2;
-  r[3] = v3;
-  r[4] = v4;
+  r[1] = st * x;
+  r[2] = st * y;
+  r[3] = st * z;
+  r[4] = 3.0 * st - 2.0 * st2;
 }
 
 // AO VGL of AO kk of the chunk at point (px, py, pz), whose shells' radial VGL are staged at
