@@ -75,3 +75,20 @@ def n2():
     )
     x = d[p + "elec_coord"].reshape(W * Ne, 3).T.copy()
     return s, x, W
+
+
+def high_l(walk_num=16, seed=23):
+    """Three nuclei carrying s to h shells (l = 0..5: 2s 1p 1d 1f 1g 1h, 57 cartesian AOs
+    each) with AO normalisation factors drawn in [0.5, 2]: the AO polynomial beyond the f
+    shells and the normalisation in every AO->MO product mode.  5 + 5 electrons, 5 MOs.
+    Returns (System, coord_T [3][W*Ne], walk_num)."""
+    from qmckl_amd import system as S
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    coord = S._place_nuclei(rng, 3, 4.0, 1.8)
+    basis = S._basis(rng, ["X"] * 3, {"X": (2, 1, 1, 1, 1, 1)})
+    basis["ao_factor"] = rng.uniform(0.5, 2.0, basis["ao_factor"].size)
+    mo = S._orthonormal_rows(rng, 5, basis["ao_factor"].size)
+    s = System(nucl_charge=np.array([7.0, 7.0, 7.0]), nucl_coord=coord, up=5, dn=5, mo_coef=mo, precision=53,
+               name="high-l", **basis, **S._jastrow(rng, [0, 0, 0], 1))
+    return s, S.electron_coords(rng, s, walk_num), walk_num

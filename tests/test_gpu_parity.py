@@ -90,6 +90,24 @@ def test_mo_vgl_fp64(name, W):
     q.context_destroy(ctx)
 
 
+@pytest.mark.parametrize("mode,bound", [(q.MO_FP64, 1e-12), (q.MO_FP32, 2e-5), (q.MO_BF16X3, 2e-6)])
+def test_ao_mo_vgl_high_l(mode, bound):
+    """s to h shells (the AO polynomial's power loop beyond f) with AO factors off 1 (folded
+    into the fused kernel's C_t operands): the fused AO->MO product in each mode, then the
+    AO getter, against the oracle."""
+    s, x, W = systems.high_l()
+    ctx = make_ctx(s, x, W, jastrow=False)
+    q.check(ctx, q.lib().qmckl_amd_set_mo_precision(ctx, mode))
+    P = W * s.elec_num
+    nmo = s.mo_num_kept
+    ao_ref = oracle.ao_vgl(s, x)
+    mo = q.get_array(ctx, "qmckl_get_mo_basis_mo_vgl", P * 5 * nmo).reshape(P, 5, nmo)
+    vgl_close(mo, oracle.mo_vgl(oracle.kept_mo_coef(s), ao_ref), bound, f"mo_vgl high-l mode {mode}")
+    ao = q.get_array(ctx, "qmckl_get_ao_basis_ao_vgl", P * 5 * s.ao_num).reshape(P, 5, s.ao_num)
+    vgl_close(ao, ao_ref, 1e-12, "ao_vgl high-l")
+    q.context_destroy(ctx)
+
+
 def test_mo_vgl_after_ao_getter_reuses_resident_ao():
     s, x, W = S.workload("small", walk_num=64)
     ctx = make_ctx(s, x, W, jastrow=False)
@@ -102,10 +120,10 @@ def test_mo_vgl_after_ao_getter_reuses_resident_ao():
 
 
 @pytest.mark.parametrize("mode,bound", [(q.MO_FP32, 2e-5), (q.MO_BF16X3, 2e-6)])
-@pytest.mark.parametrize("name,W", [("small", 256), ("medium", 8)])
+@pytest.mark.parametrize("name,W", [("chbrclf", 2), ("small", 256), ("medium", 8)])
 def test_mo_vgl_split_precision(mode, bound, name, W):
     """Split-precision products stay within `bound` * max|component| of the FP64 result."""
-    s, x, W = S.workload(name, walk_num=W)
+    s, x, W = systems.chbrclf() if name == "chbrclf" else S.workload(name, walk_num=W)
     ctx = make_ctx(s, x, W, jastrow=False)
     q.check(ctx, q.lib().qmckl_amd_set_mo_precision(ctx, mode))
     P = W * s.elec_num

@@ -27,7 +27,11 @@ from qmckl_amd import system as S
 def _ao_closed_form(s, x):
     """AO values [P][ao] from the definition: ao_factor * x^a y^b z^c * shell_factor *
     sum_p coefficient_p prim_factor_p exp(-alpha_p r^2), cartesian order a = l..0,
-    b = l-a..0 for each shell (org/qmckl_ao.org, polynomial order of :6270-6300)."""
+    b = l-a..0 for each shell (org/qmckl_ao.org, polynomial order of :6270-6300), with the
+    primitives where alpha r^2 > -log(2^(1 - precision)) left out, the screening rule of
+    org/qmckl_ao.org:118-120 (its prim_factor grows as alpha^((2l+3)/4), so beyond the
+    f shells a screened tight primitive is not negligible against its shell)."""
+    cutoff = -np.log(2.0 ** (1 - s.precision))
     P = x.shape[1]
     out = np.zeros((P, s.ao_num))
     k = 0
@@ -40,7 +44,8 @@ def _ao_closed_form(s, x):
             p0, np_ = int(s.shell_prim_index[sh]), int(s.shell_prim_num[sh])
             rad = np.zeros(P)
             for p in range(p0, p0 + np_):
-                rad += s.coefficient[p] * s.prim_factor[p] * np.exp(-s.exponent[p] * r2)
+                ar2 = s.exponent[p] * r2
+                rad += np.where(ar2 > cutoff, 0.0, s.coefficient[p] * s.prim_factor[p] * np.exp(-ar2))
             rad *= s.shell_factor[sh]
             for pa in range(l, -1, -1):
                 for pb in range(l - pa, -1, -1):
@@ -57,20 +62,29 @@ def _points(s, n, seed):
     return np.ascontiguousarray(c + rng.normal(0.0, 0.8, (3, n)))
 
 
-@pytest.mark.parametrize("name", ["small", "medium"])
+def _ao_system(name):
+    """A synthetic workload, or the s-to-h-shell system with AO factors off 1."""
+    if name == "high_l":
+        import systems
+
+        return systems.high_l(walk_num=1)[0]
+    return S.workload(name, walk_num=1)[0]
+
+
+@pytest.mark.parametrize("name", ["small", "medium", "high_l"])
 def test_ao_values_match_closed_form(name):
-    s, _, _ = S.workload(name, walk_num=1)
+    s = _ao_system(name)
     s.precision = 53
     x = _points(s, 24, 1)
     got = oracle.ao_vgl(s, x)[:, 0, :]
     ref = _ao_closed_form(s, x)
-    # the oracle drops primitives with alpha r^2 > -log(2^-52): below 1e-15 of each shell
     scale = np.abs(ref).max()
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * scale)
 
 
-def test_ao_gradient_and_laplacian_match_finite_differences():
-    s, _, _ = S.workload("small", walk_num=1)
+@pytest.mark.parametrize("name", ["small", "high_l"])
+def test_ao_gradient_and_laplacian_match_finite_differences(name):
+    s = _ao_system(name)
     x = _points(s, 12, 2)
     vgl = oracle.ao_vgl(s, x)
     h = 1e-4
