@@ -100,20 +100,14 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
   r[4] = 3.0 * st - 2.0 * st2;
 }
 
-// AO VGL of AO kk of the chunk at point (px, py, pz), whose shells' radial VGL are staged at
-// rad, `stride` doubles per shell; org/qmckl_ao.org:8530-8565, without the normalisation
-// factor ao_factor[k], which the chunk operands of C_t carry (row k scaled by it).
-__device__ __forceinline__ void chunk_ao_pt(const Chunk& c, int kk, double px, double py, double pz,
-                                            const double* rad, int stride, double r[5]) {
-  const double code = c.ao[2 * kk];
-  const uint32_t u = (uint32_t)code;
-  const int s = u & 255, pw = u >> 8;
-  const int nl = (int)c.shl[2 * s] & 255;
-  const double x = px - c.nuc[4 * nl], y = py - c.nuc[4 * nl + 1], z = pz - c.nuc[4 * nl + 2];
+// AO VGL of an AO with powers (pa, pb, pc) at displacement (x, y, z) from its nucleus, its
+// shell's radial VGL at rs; org/qmckl_ao.org:8530-8565, without the normalisation factor
+// ao_factor[k], which the chunk operands of C_t carry (row k scaled by it).
+__device__ __forceinline__ void chunk_ao_vgl(double x, double y, double z, int pa, int pb, int pc, const double* rs,
+                                             double r[5]) {
   double pv[5];
-  monomial_vgl(x + 1.e-20, y + 1.e-20, z + 1.e-20, pw & 0xff, (pw >> 8) & 0xff, (pw >> 16) & 0xff, pv);
-  const double* sh = rad + stride * s;
-  const double s0 = sh[0], s1 = sh[1], s2 = sh[2], s3 = sh[3], s4 = sh[4];
+  monomial_vgl(x + 1.e-20, y + 1.e-20, z + 1.e-20, pa, pb, pc, pv);
+  const double s0 = rs[0], s1 = rs[1], s2 = rs[2], s3 = rs[3], s4 = rs[4];
   r[0] = pv[0] * s0;
   r[1] = pv[1] * s0 + pv[0] * s1;
   r[2] = pv[2] * s0 + pv[0] * s2;
@@ -129,6 +123,8 @@ constexpr int TP = 16;
 constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row = 5 p + c)
 constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
 constexpr int SH = 6;          // doubles per staged (point, shell): 5 radial VGL + padding
+constexpr int QS = 64 * NWAVE / KC;  // points between the two of a thread in the AO step
+static_assert((64 * NWAVE) % KC == 0 && TP % QS == 0, "AO step: whole points per pass");
 constexpr int KP16 = AO_MO_BF_LD;
 // Row strides of the FP64 / FP32 operand tiles, so that the operand reads A[16 r + li][4 s
 // + lg] of a wave cover the 64 four-byte LDS banks evenly.  Doubles: lane (li, lg) starts at
@@ -251,11 +247,18 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
       for (int k = 0; k < 5; ++k) o[k] = r[k];
     }
     __syncthreads();
-    // 2. AO VGL of the chunk into the operand tile, row = 5 p + c
-    for (int t = tid; t < TP * KC; t += 64 * NWAVE) {
-      const int q = t / KC, kk = t - q * KC;
+    // 2. AO VGL of the chunk into the operand tile, row = 5 p + c: a thread keeps AO
+    //    kk = tid mod KC (its descriptor decoded once per chunk) for the points
+    //    q = tid / KC + QS m
+    const int kk = tid & (KC - 1);
+    const uint32_t au = (uint32_t)c.ao[2 * kk];
+    const int ash = au & 255, apw = au >> 8, anl = (int)c.shl[2 * ash] & 255;
+    const int pa = apw & 0xff, pb = (apw >> 8) & 0xff, pc = (apw >> 16) & 0xff;
+    const double nxa = c.nuc[4 * anl], nya = c.nuc[4 * anl + 1], nza = c.nuc[4 * anl + 2];
+    for (int q = tid / KC; q < TP; q += QS) {
       double r[5];
-      chunk_ao_pt(c, kk, xt[3 * q], xt[3 * q + 1], xt[3 * q + 2], sh + q * A.ns_max * SH, SH, r);
+      chunk_ao_vgl(xt[3 * q] - nxa, xt[3 * q + 1] - nya, xt[3 * q + 2] - nza, pa, pb, pc,
+                   sh + (q * A.ns_max + ash) * SH, r);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         if constexpr (MODE == 0) {
