@@ -1015,16 +1015,9 @@ struct EenResLds {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Copies n doubles from global memory to LDS with the gfx950 global-to-LDS loads, 16 bytes
-// per lane (src and dst 16-byte aligned, n even); the waves of the block take 64-lane slices
-// in turn.  The copy is asynchronous: wait (s_waitcnt) and synchronise before reading dst.
-__device__ __forceinline__ void lds_copy16(double* dst, const double* src, int n, int wave, int lane) {
-  for (int base = wave * 128; base < n; base += EQ_WAVES * 128) {
-    const int t = base + 2 * lane;
-    if (t < n) __builtin_amdgcn_global_load_lds(src + t, (lds_void*)(dst + base), 16, 0, 0);
-  }
-}
-// The same, 4 bytes per lane, for any 8-byte aligned src and dst.
+// Copies n doubles from global memory to LDS with the gfx950 global-to-LDS loads, 4 bytes
+// per lane, for any 8-byte aligned src and dst; the waves of the block take 64-lane slices in
+// turn.  The copy is asynchronous: wait (s_waitcnt) and synchronise before reading dst.
 __device__ __forceinline__ void lds_copy4(double* dst, const double* src, int n, int wave, int lane) {
   const float* s = reinterpret_cast<const float*>(src);
   float* d = reinterpret_cast<float*>(dst);
@@ -1316,7 +1309,6 @@ __device__ __forceinline__ void een_fold_unit(const EenResCtx& x, int il, const 
 template <int C, int K, int Q, int NQ>
 __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int gcol, const double (&X)[NQ][5],
                                                  double (&out)[5]) {
-  constexpr int K1 = C - 1;
   const int i = min(x.i0 + il, x.Ne - 1);
   double gp[C + 1];
   gp[0] = 1.0;
