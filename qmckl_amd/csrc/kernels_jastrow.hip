@@ -1184,13 +1184,22 @@ constexpr int een_single_q(int C, int p) { return C - een_single_k(C, p); }
 // rows 4-7 (h = 0, 1): lane l = 16 m + 4 b + n feeds A(i = 4 h + n, j = 4 s + m) and
 // B(j = 4 s + m, a = 4 b + n), the B element of the full tile, and holds row 4 h + lg,
 // column li of the result, so that half the matrix work of a full tile covers its rows.
-enum { EEN_FULL = 0, EEN_MINI = 1, EEN_ROWS = 2 };
+// MINI2: the last nucleus tile holds at most 2 nuclei and the group is a pair (q, q + 1):
+// the four columns of the 4x4x4 products are (nucleus n mod 2, power q + n / 2), so one
+// product per component covers both units; lane n < 2 folds its own column (q) with its
+// partner's l ^ 2 (q + 1), and lanes n >= 2 fold against the padding nuclei 2, 3 of the
+// tile, whose coefficients are zero.
+enum { EEN_FULL = 0, EEN_MINI = 1, EEN_ROWS = 2, EEN_MINI2 = 3 };
 template <int MODE>
 struct EenAccT {
   using type = d4;
 };
 template <>
 struct EenAccT<EEN_MINI> {
+  using type = double;
+};
+template <>
+struct EenAccT<EEN_MINI2> {
   using type = double;
 };
 template <>
@@ -1239,10 +1248,13 @@ __device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][5][2], const
 template <int K, int Q, int NQ, int MODE>
 __device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const double* jr, const double* gt,
                                              const double (&xi)[3], int nsteps, int li, int lg) {
-  constexpr bool MINI = MODE == EEN_MINI;
+  constexpr bool MINI = MODE == EEN_MINI || MODE == EEN_MINI2;
+  constexpr bool MINI2 = MODE == EEN_MINI2;
+  static_assert(!MINI2 || NQ == 1, "MINI2 holds both units of a pair in one accumulator");
   const double* pj = jr + lg * JREC;
   const double* pi = pj + li;
-  const double* pg = gt + lg * 16 + (MINI ? (li & 3) : li);
+  const double* pg = gt + lg * 16 + (MINI2 ? (li & 1) : MINI ? (li & 3) : li);
+  const bool qup = MINI2 && (li & 2) != 0;  // MINI2: this column holds power q + 1
   // the next step's operands are read before this step's products are issued
   double nGe = pi[0], nri = pi[16], ng = pg[0], nxj = pj[32], nyj = pj[33], nzj = pj[34];
   for (int st = 0; st < nsteps; ++st) {
@@ -1258,6 +1270,7 @@ __device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const d
     for (int k = 2; k <= K; ++k) Gk *= Ge;
 #pragma unroll
     for (int q = 2; q <= Q; ++q) gq *= g;
+    if constexpr (MINI2) gq = qup ? gq * g : gq;
     // g_e^k w with w = (1, u_x, u_y, u_z, 1/r), u = (r_i - r_j) / r, as g_e^k / r times
     // (r_i - r_j); u and 1/r are 0 for j == i (1/r is stored 0 there)
     const double t = Gk * ri;
@@ -1363,6 +1376,21 @@ __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (
       een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * h, x.li, X, ra[h]);
     }
     return;
+  } else if constexpr (MODE == EEN_MINI2) {
+    double acc[1][5] = {{0.0, 0.0, 0.0, 0.0, 0.0}};
+    een_res_pass<K, Q, 1, EEN_MINI2>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+    double X[2][5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      X[0][c] = acc[0][c];
+      X[1][c] = dpp_f64<DPP_XOR2>(acc[0][c]);  // column n ^ 2: power q + 1 of the same nucleus
+    }
+    double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    een_res_fold_row<C, K, Q, 2>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, out);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ra[r][c] += (r == (x.li >> 2)) ? out[c] : 0.0;
   } else {
   EenAcc<MODE> acc[NQ][5];
 #pragma unroll
@@ -1410,7 +1438,13 @@ __device__ __forceinline__ void een_res_dispatch(int g, int mode, const EenResCt
       constexpr int K = G < NPR ? een_pair_k(C, G) : een_single_k(C, G - NPR);
       constexpr int Q = G < NPR ? een_pair_q(C, G) : een_single_q(C, G - NPR);
       constexpr int NQ = G < NPR ? 2 : 1;
-      if (mode == EEN_MINI)
+      if constexpr (NQ == 2) {
+        if (mode == EEN_MINI2) {
+          een_res_group<C, K, Q, NQ, EEN_MINI2>(x, xi, ra);
+          return;
+        }
+      }
+      if (mode == EEN_MINI || mode == EEN_MINI2)
         een_res_group<C, K, Q, NQ, EEN_MINI>(x, xi, ra);
       else if (mode == EEN_ROWS)
         een_res_group<C, K, Q, NQ, EEN_ROWS>(x, xi, ra);
@@ -1476,14 +1510,14 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     // the least loaded wave (lowest index on ties): the pairs of every full nucleus tile,
     // then its single units, then, when the last tile holds at most 4 nuclei, its pairs
     // and single units on the 4x4x4 products (about a third of the work of a full-tile
-    // unit).  A second table serves the electron tiles of at most 8 rows, whose full
+    // unit; a quarter for a pair when that tile holds at most 2 nuclei).  A second table serves the electron tiles of at most 8 rows, whose full
     // nucleus tiles run on the 4x4x4 products by rows (about two thirds of the work).
     const int Nn = (int)v.nucl_num, NA = gn_na(Nn), NA16 = 16 * NA;
     const EenResLds L(gn_jp((int)v.elec_num), NA16, C, DIM);
     lds_copy4(lds + L.nuc, v.nuc_pad, 4 * NA16, wave, threadIdx.x & 63);
     lds_copy4(lds + L.cfl, v.cfull_pad, DIM * NA16, wave, threadIdx.x & 63);
     if (threadIdx.x == 0) {
-      const bool mini = Nn - 16 * (NA - 1) <= 4;
+      const bool mini = Nn - 16 * (NA - 1) <= 4, mini2 = Nn - 16 * (NA - 1) <= 2;
       const int NAF = mini ? NA - 1 : NA;
       const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n3 = n1 + (mini ? NPR + NSG : 0);
       int* own = reinterpret_cast<int*>(lds + L.own);
@@ -1491,7 +1525,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
         int load[EQ_WAVES];
         for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
         for (int it = 0; it < n3; ++it) {
-          const int weight = it < n0 ? (rows ? 8 : 12) : it < n1 ? (rows ? 4 : 6) : it < n1 + NPR ? 4 : 2;
+          const int weight = it < n0 ? (rows ? 8 : 12) : it < n1 ? (rows ? 4 : 6) : it < n1 + NPR ? (mini2 ? 3 : 4) : 2;
           int owner = 0;
           for (int wv = 1; wv < EQ_WAVES; ++wv)
             if (load[wv] < load[owner]) owner = wv;
@@ -1522,7 +1556,8 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     double* red = lds + L.red;
     const double* coord = v.coord;
     const int64_t cld = v.coord_ld;
-    const bool mini = Nn - 16 * (NA - 1) <= 4;  // nuclei in the last tile
+    const bool mini = Nn - 16 * (NA - 1) <= 4;   // nuclei in the last tile
+    const bool mini2 = Nn - 16 * (NA - 1) <= 2;  // pairs of units in one 4x4x4 product
     const int NAF = mini ? NA - 1 : NA;         // tiles on the 16x16x4 products
     const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n2 = n1 + (mini ? NPR : 0), n3 = n2 + (mini ? NSG : 0);
     const int w32 = item / ntiles;
@@ -1703,7 +1738,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
         at = NA - 1, g = NPR + it - n2;
       }
       const bool mu = it >= n1;  // the last nucleus tile on the 4x4x4 products
-      const int mode = mu ? EEN_MINI : rows ? EEN_ROWS : EEN_FULL;
+      const int mode = mu ? (mini2 ? EEN_MINI2 : EEN_MINI) : rows ? EEN_ROWS : EEN_FULL;
       x.gt = gch + at * JP * 16;
       x.a = 16 * at + (mu ? (gli & 3) : gli);
       x.nx = nuc[x.a];
