@@ -150,6 +150,24 @@ __host__ __device__ __forceinline__ int gn_jp(int64_t Ne) { return (int)((Ne + 3
 // every gradient, Laplacian and e-e-n term is multiplied by an exact zero (dx = dy = dz = 0,
 // ed = em = 0), which leaves the sums unchanged.  Returns g_e(r_ij) = e^(-kappa r_ij) and
 // 1/r_ij, or 0 and 0 for j == i.
+// |r| and 1/|r| from r2 = |r|^2 with one reciprocal square root: v_rsq_f64 refined by two
+// Newton steps y <- y (3/2 - r2 y^2 / 2), within a few ulp of the correctly rounded values,
+// instead of a square root and a division; outside [1e-200, 1e200] (coincident points
+// included) the library square root and a division.
+__device__ __forceinline__ void dist_and_inverse(double r2, double& d, double& rinv) {
+  if (r2 >= 1.e-200 && r2 <= 1.e200) {
+    double y = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    y = y * __builtin_fma(-h, y * y, 1.5);
+    y = y * __builtin_fma(-h, y * y, 1.5);
+    rinv = y;
+    d = r2 * y;
+  } else {
+    d = sqrt(r2);
+    rinv = 1.0 / d;
+  }
+}
+
 struct EeAcc {
   double fee = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
 };
@@ -181,7 +199,8 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   const double* b = v.b_vector;
   const int bord = (int)v.bord, up = (int)v.up_num;
   const bool use = j != i;
-  const double d = use ? sqrt(dx * dx + dy * dy + dz * dz) : 1.0;
+  double d = 1.0, rinv = 1.0;
+  if (use) dist_and_inverse(dx * dx + dy * dy + dz * dz, d, rinv);
   const double e = exp_neg_deg<13, true>(-kee * d);
   const double x = (1.0 - e) * kinv;  // rescaled distance, org/qmckl_distance.org:975-985
   double pq, dq, hq;  // sum_{k>=2} b_k x^k, its derivative, half its second derivative
@@ -199,7 +218,7 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
     s.fee += pq;
   }
   const double dd = d > 1.e-20 ? d : 1.e-20;
-  const double dinv = 1.0 / dd;
+  const double dinv = d > 1.e-20 ? rinv : 1.e20;  // 1 / dd
   const double ed = use ? ((d > 1.e-20) ? e : exp_neg_deg<13, true>(-kee * dd)) : 0.0;
   const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
   const double gl4 = (2.0 * dinv - kee) * ed;
@@ -220,8 +239,6 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
   s.g1 += dq * gy;
   s.g2 += dq * gz;
   s.g3 += dq * gl4 + 2.0 * hq * gc2;
-  double rinv = dinv;  // 1 / d, the same quotient unless d <= 1e-20
-  if (!(d > 1.e-20)) rinv = 1.0 / d;
   const double ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
   const double em = use ? e : 0.0;
   double gk = 1.0;
@@ -245,9 +262,8 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
 struct EnAcc {
   double fen = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;
 };
-// Returns 1 / d (the caller's inverse distance).
-__device__ __forceinline__ double en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
-                                          double e, EnAcc& s) {
+__device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
+                                        double rinv, double e, EnAcc& s) {
   const int aord = (int)v.aord;
   const double* av = v.a_vector + t * (aord + 1);
   const double ka = v.kappa_en[t];
@@ -259,7 +275,7 @@ __device__ __forceinline__ double en_pair(const JasDevView& v, int t, double dx,
   s.fen += av[0] * x * invd - v.asymp_jasa[t];
   s.fen += pq;
   const double dd = d > 1.e-20 ? d : 1.e-20;
-  const double dinv = 1.0 / dd;
+  const double dinv = d > 1.e-20 ? rinv : 1.e20;  // 1 / dd
   const double ed = (d > 1.e-20) ? e : exp_neg_deg<13, true>(-ka * dd);
   const double gx = dx * dinv * ed, gy = dy * dinv * ed, gz = dz * dinv * ed;
   const double gl4 = (2.0 * dinv - ka) * ed;
@@ -273,9 +289,6 @@ __device__ __forceinline__ double en_pair(const JasDevView& v, int t, double dx,
   s.h1 += dq * gy;
   s.h2 += dq * gz;
   s.h3 += dq * gl4 + 2.0 * hq * gc2;
-  double rinv = dinv;  // the same quotient unless d <= 1e-20
-  if (!(d > 1.e-20)) rinv = 1.0 / d;
-  return rinv;
 }
 
 // ------------------------------------------------------------------------------------
@@ -331,10 +344,11 @@ __global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, dou
     for (int a = 0; a < Nn; ++a) {
       const int t = (int)v.type_nucl[a];
       const double dx = xi - v.nucl_coord[a], dy = yi - v.nucl_coord[a + Nn], dz = zi - v.nucl_coord[a + 2 * Nn];
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      double d, rinv;
+      dist_and_inverse(dx * dx + dy * dy + dz * dz, d, rinv);
       const double e = exp_neg_deg<13, true>(-v.kappa_en[t] * d);
       gN[gn_index(w, a, i, NAg, JPg)] = e;  // g_a(r_i), reused by the e-e-n kernels
-      en_pair(v, t, dx, dy, dz, d, e, sn);
+      en_pair(v, t, dx, dy, dz, d, rinv, e, sn);
     }
     fen += sn.fen;
     double* oh = o.fen_gl + w * 4 * Ne;
@@ -1654,8 +1668,9 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
         double rinv = 0.0;
         if (i < Ne && a < Nn) {
           const double dx = xi - nuc[a], dy = yi - nuc[NA16 + a], dz = zi - nuc[2 * NA16 + a];
-          const double d = sqrt(dx * dx + dy * dy + dz * dz);
-          rinv = en_pair(v, (int)v.type_nucl[a], dx, dy, dz, d, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
+          double d;
+          dist_and_inverse(dx * dx + dy * dy + dz * dz, d, rinv);
+          en_pair(v, (int)v.type_nucl[a], dx, dy, dz, d, rinv, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
         }
         enr[il * NA16 + a] = rinv;
       }
