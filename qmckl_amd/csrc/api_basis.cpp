@@ -366,8 +366,7 @@ namespace {
 
 qmckl_exit_code copy_out(Context* c, qmckl_context context, const void* d_src, void* dst, size_t bytes,
                          const char* fn) {
-  hipError_t e = hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  const hipError_t e = copy_to_host(c, dst, d_src, bytes);
   if (e != hipSuccess) return hip_fail(context, e, fn);
   return QMCKL_SUCCESS;
 }

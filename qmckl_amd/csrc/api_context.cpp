@@ -1,10 +1,13 @@
 // Context, error record, numerical precision, points, nuclei and electrons.
 // Host side of the C-ABI; device work is delegated to kernels_*.hip.
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <set>
+#include <thread>
 
 #include "internal.hpp"
 
@@ -122,6 +125,65 @@ static void drain_profile(Context* c) {
   c->prof_pending.clear();
 }
 
+namespace {
+
+constexpr size_t kStageBytes = size_t(32) << 20;  // per pinned staging buffer
+constexpr size_t kStagedMin = size_t(16) << 20;   // smaller copies go straight to dst
+
+// memcpy of n bytes split over up to 8 host threads (one for n < 4 MiB)
+void host_copy(char* dst, const char* src, size_t n) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = n < (size_t(4) << 20) ? 1 : (int)std::min<unsigned>(8u, hw ? hw : 1u);
+  if (T == 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t part = ((n + T - 1) / T + 63) & ~size_t(63);
+  std::thread th[8];
+  for (int t = 1; t < T; ++t) {
+    const size_t o = (size_t)t * part;
+    if (o < n) th[t] = std::thread([=] { std::memcpy(dst + o, src + o, std::min(part, n - o)); });
+  }
+  std::memcpy(dst, src, std::min(part, n));
+  for (int t = 1; t < T; ++t)
+    if (th[t].joinable()) th[t].join();
+}
+
+}  // namespace
+
+hipError_t copy_to_host(Context* c, void* dst, const void* d_src, size_t bytes) {
+  hipError_t e = hipSuccess;
+  if (bytes < kStagedMin) {
+    e = hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+  }
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    if (!c->stage[b]) e = hipHostMalloc(&c->stage[b], kStageBytes, hipHostMallocDefault);
+    if (e == hipSuccess && !c->stage_ev[b]) e = hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) return e;
+  const char* src = static_cast<const char*>(d_src);
+  char* out = static_cast<char*>(dst);
+  const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+  // chunk k goes to stage[k & 1]; that buffer's previous chunk (k - 2) was copied out to
+  // dst in the iteration before, so it is free when chunk k is enqueued
+  for (size_t k = 0; k <= nch && e == hipSuccess; ++k) {
+    if (k < nch) {
+      const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+      e = hipMemcpyAsync(c->stage[k & 1], src + off, len, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(c->stage_ev[k & 1], c->stream);
+    }
+    if (e == hipSuccess && k > 0) {
+      const size_t off = (k - 1) * kStageBytes, len = std::min(kStageBytes, bytes - off);
+      e = hipEventSynchronize(c->stage_ev[(k - 1) & 1]);
+      if (e == hipSuccess) host_copy(out + off, static_cast<const char*>(c->stage[(k - 1) & 1]), len);
+    }
+  }
+  if (e != hipSuccess) (void)hipStreamSynchronize(c->stream);  // no copy left in flight
+  return e;
+}
+
 static void release_device(Context* c) {
   drain_profile(c);
   DevConst& d = c->dc;
@@ -136,6 +198,12 @@ static void release_device(Context* c) {
                     &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->d_deriv) b.release();
+  for (int b = 0; b < 2; ++b) {
+    if (c->stage[b]) (void)hipHostFree(c->stage[b]);
+    if (c->stage_ev[b]) (void)hipEventDestroy(c->stage_ev[b]);
+    c->stage[b] = nullptr;
+    c->stage_ev[b] = nullptr;
+  }
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   c->own_stream = nullptr;
   c->stream = nullptr;
@@ -702,8 +770,7 @@ qmckl_exit_code qmckl_get_electron_coord(const qmckl_context context, const char
 
 static qmckl_exit_code copy_out(Context* c, qmckl_context context, const void* d_src, double* dst, int64_t n,
                                 const char* fn) {
-  hipError_t e = hipMemcpyAsync(dst, d_src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  const hipError_t e = copy_to_host(c, dst, d_src, sizeof(double) * n);
   if (e != hipSuccess) return hip_fail(context, e, fn);
   return QMCKL_SUCCESS;
 }

@@ -146,8 +146,7 @@ qmckl_exit_code jas_ready(Context* c, qmckl_context handle, const char* fn) {
 }
 
 qmckl_exit_code copy_out(Context* c, qmckl_context context, const void* d_src, double* dst, int64_t n, const char* fn) {
-  hipError_t e = hipMemcpyAsync(dst, d_src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  const hipError_t e = copy_to_host(c, dst, d_src, sizeof(double) * n);
   if (e != hipSuccess) return hip_fail(context, e, fn);
   return QMCKL_SUCCESS;
 }

@@ -134,6 +134,9 @@ struct Context {
   int device = -1;
   hipStream_t stream = nullptr;      // stream in use
   hipStream_t own_stream = nullptr;  // created by the context
+  // pinned staging of large device-to-host copies (copy_to_host), allocated on first use
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   DevConst dc;
   DevBuf d_coord;  // [3][point_num]
   uint64_t d_coord_date = 0;
@@ -192,6 +195,11 @@ qmckl_exit_code fail(qmckl_context context, qmckl_exit_code code, const char* fn
 qmckl_exit_code device_ready(Context* c, qmckl_context handle, const char* fn);
 qmckl_exit_code upload_coords(Context* c, qmckl_context handle, const char* fn);
 qmckl_exit_code hip_fail(qmckl_context handle, hipError_t e, const char* fn);
+// Device-to-host copy of `bytes` from d_src (on c's device) into the caller's host memory,
+// synchronous.  Below 16 MiB one hipMemcpyAsync on c->stream; above, through c's two
+// pinned 32 MiB staging buffers: the device-to-host copy of chunk k + 1 overlaps the host
+// threads' copy of chunk k into dst.
+hipError_t copy_to_host(Context* c, void* dst, const void* d_src, size_t bytes);
 
 // providers
 qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn);

@@ -299,6 +299,31 @@ def test_coordinate_update_recomputes():
 
 
 # ---------------------------------------------------------------- device entry points
+def test_large_host_getters_through_pinned_staging():
+    """Host getters above 16 MiB go through two pinned 32 MiB staging buffers in chunks: an
+    86 MB MO VGL (3 chunks, both buffers) and a 1 MB GL, against the device entry points
+    read back directly, bit for bit, twice (the staging buffers are reused)."""
+    s, x, W = S.workload("medium", walk_num=192)
+    ctx = make_ctx(s, x, W)
+    L = q.lib()
+    Ne, nmo = s.elec_num, s.mo_num_kept
+    n = W * Ne * 5 * nmo
+    assert 8 * n > 2 * 32 * 2**20
+    mo = q.DeviceBuffer(8 * n)
+    gl = q.DeviceBuffer(8 * W * 4 * Ne)
+    xd = q.DeviceBuffer.from_array(x)
+    for _ in range(2):
+        host_mo = q.get_array(ctx, "qmckl_get_mo_basis_mo_vgl", n)
+        host_gl = q.get_array(ctx, "qmckl_get_jastrow_champ_gl", W * 4 * Ne)
+        q.check(ctx, L.qmckl_amd_set_electron_coord_device(ctx, b"T", W, xd.ptr, x.size))
+        q.check(ctx, L.qmckl_amd_get_mo_basis_mo_vgl_device(ctx, mo.ptr, n))
+        q.check(ctx, L.qmckl_amd_get_jastrow_champ_gl_device(ctx, gl.ptr, W * 4 * Ne))
+        q.synchronize(ctx)
+        np.testing.assert_array_equal(mo.to_array(n), host_mo)
+        np.testing.assert_array_equal(gl.to_array(W * 4 * Ne), host_gl)
+    q.context_destroy(ctx)
+
+
 def test_device_entry_points_match_host_getters():
     s, x, W = S.workload("small", walk_num=32)
     ctx = make_ctx(s, x, W)
