@@ -130,7 +130,9 @@ namespace {
 constexpr size_t kStageBytes = size_t(32) << 20;  // per pinned staging buffer
 constexpr size_t kStagedMin = size_t(16) << 20;   // smaller copies go straight to dst
 
-// memcpy of n bytes split over up to 8 host threads (one for n < 4 MiB)
+// memcpy of n bytes split over up to 8 host threads (one for n < 4 MiB); a slice whose
+// thread cannot be started is copied by the calling thread, so nothing throws across the
+// C-ABI
 void host_copy(char* dst, const char* src, size_t n) {
   const unsigned hw = std::thread::hardware_concurrency();
   const int T = n < (size_t(4) << 20) ? 1 : (int)std::min<unsigned>(8u, hw ? hw : 1u);
@@ -142,7 +144,13 @@ void host_copy(char* dst, const char* src, size_t n) {
   std::thread th[8];
   for (int t = 1; t < T; ++t) {
     const size_t o = (size_t)t * part;
-    if (o < n) th[t] = std::thread([=] { std::memcpy(dst + o, src + o, std::min(part, n - o)); });
+    if (o >= n) continue;
+    const size_t len = std::min(part, n - o);
+    try {
+      th[t] = std::thread([=] { std::memcpy(dst + o, src + o, len); });
+    } catch (...) {
+      std::memcpy(dst + o, src + o, len);
+    }
   }
   std::memcpy(dst, src, std::min(part, n));
   for (int t = 1; t < T; ++t)
