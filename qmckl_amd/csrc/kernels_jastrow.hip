@@ -665,18 +665,20 @@ __device__ __forceinline__ void een_mfma_pass(d4 (&acc)[5], const double* pdG, c
   for (int st = 0; st < nsteps; ++st) {
     const int jj = 4 * st + lg;
     const double Ge = pdG[jj * EM + li], ri = pdR[jj * EM + li], g = gch[jj * 16 + li];
-    const double ux = (xi[0] - xj[jj]) * ri, uy = (xi[1] - xj[JSP + jj]) * ri, uz = (xi[2] - xj[2 * JSP + jj]) * ri;
+    const double dx = xi[0] - xj[jj], dy = xi[1] - xj[JSP + jj], dz = xi[2] - xj[2 * JSP + jj];
     double Gk = Ge, gq = g;
 #pragma unroll
     for (int k = 2; k <= K; ++k) Gk *= Ge;
 #pragma unroll
     for (int q = 2; q <= Q; ++q) gq *= g;
-    // w = (1, u_x, u_y, u_z, 1/r); u and 1/r are 0 for j == i (1/r is stored 0 there)
+    // g_e^k w with w = (1, u_x, u_y, u_z, 1/r), u = (r_i - r_j) / r, as g_e^k / r times
+    // (r_i - r_j); u and 1/r are 0 for j == i (1/r is stored 0 there)
+    const double t = Gk * ri;
     acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk, gq, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ux, gq, acc[1], 0, 0, 0);
-    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uy, gq, acc[2], 0, 0, 0);
-    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * uz, gq, acc[3], 0, 0, 0);
-    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Gk * ri, gq, acc[4], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(t * dx, gq, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(t * dy, gq, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(t * dz, gq, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(t, gq, acc[4], 0, 0, 0);
   }
 }
 
