@@ -903,9 +903,9 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
             const double dx = xt[il * 3] - coord[w * Ne + j];
             const double dy = xt[il * 3 + 1] - coord[w * Ne + j + cld];
             const double dz = xt[il * 3 + 2] - coord[w * Ne + j + 2 * cld];
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            ri = 1.0 / d;
-            Ge = exp(-kee * d);
+            double d;
+            dist_and_inverse(dx * dx + dy * dy + dz * dz, d, ri);
+            Ge = exp_neg_deg<13, true>(-kee * d);
           }
           pdG[t] = Ge;  // [jj][il]
           pdR[t] = ri;
