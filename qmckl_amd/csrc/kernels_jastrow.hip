@@ -911,8 +911,8 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
           pdR[t] = ri;
         }
       }
-      for (int t = tid; t < 16 * jn4; t += NT) {
-        const int jj = t % jn4, al = t / jn4;
+      for (int t = tid; t < 16 * jn4; t += NT) {  // consecutive threads: consecutive doubles
+        const int jj = t >> 4, al = t & 15;
         const int a = a0 + al;
         gch[jj * 16 + al] = (a < Nn && jj < jn) ? gN[gn_index(w, a, j0 + jj, gn_na(Nn), gn_jp(Ne))] : 0.0;
       }
