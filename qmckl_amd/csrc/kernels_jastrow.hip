@@ -35,6 +35,7 @@ namespace qmckl_amd {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kPairBlock = 1024;  // k_jas_pair: up to 16 waves per walker, one electron i per thread
 
 // ---- (l, k, m) of the c-coefficients, org/qmckl_jastrow_champ.org:12000-12040 ----------
 constexpr int dimc(int C) {
@@ -296,10 +297,10 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
 // R0 scratch layout: [W][Ne][cord-1][5]; gN: see gn_index.
 // ------------------------------------------------------------------------------------
 // C (= cord, 0..8) is a template parameter so that only the C - 1 orders of the q = 0 sums
-// are held in registers.  The block has 64 ceil(Ne / 64) threads (at most 256), one
+// are held in registers.  The block has 64 ceil(Ne / 64) threads (at most kPairBlock), one
 // electron i per thread.
 template <int C>
-__global__ void __launch_bounds__(kBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
+__global__ void __launch_bounds__(kPairBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
                                                      double* __restrict__ gN, double* __restrict__ Sg) {
   constexpr int K1 = C >= 2 ? C - 1 : 0;
   extern __shared__ double lds[];
@@ -2116,8 +2117,8 @@ bool een_fused(const JasDevView& v) {
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s) {
   if (een_fused(v)) return hipSuccess;  // done by the resident e-e-n kernel
   const ScratchView sv = scratch_view(v, o);
-  const int pair_threads = (int)std::min<int64_t>(kBlock, 64 * ((v.elec_num + 63) / 64));
-  const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kBlock / 64);
+  const int pair_threads = (int)std::min<int64_t>(kPairBlock, 64 * ((v.elec_num + 63) / 64));
+  const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kPairBlock / 64);
   if (lds_pair > 160 * 1024) return hipErrorInvalidValue;
   switch (v.cord) {
 #define PAIR_CASE(CC)                                                                                         \
