@@ -71,7 +71,7 @@ typedef int32_t qmckl_exit_code;
 #define QMCKL_MAX_FUN_LEN 256
 #define QMCKL_MAX_MSG_LEN 1024
 
-/* ---- Context: org/qmckl_context.org:221, 253, 302, 428, 467, 609, 648, 769 */
+/* ---- Context: org/qmckl_context.org:221, 253, 302, 428, 467, 609, 648, 695, 769 */
 qmckl_context qmckl_context_check(const qmckl_context context);
 qmckl_exit_code qmckl_context_touch(const qmckl_context context);
 qmckl_context qmckl_context_create(void);
@@ -80,6 +80,7 @@ void qmckl_unlock(qmckl_context context);
 qmckl_context qmckl_context_copy(const qmckl_context context);
 qmckl_exit_code qmckl_context_get_num_threads(const qmckl_context context, int32_t* const num_threads);
 qmckl_exit_code qmckl_context_set_num_threads(const qmckl_context context, const int32_t num_threads);
+qmckl_exit_code qmckl_context_unset_num_threads(const qmckl_context context);
 qmckl_exit_code qmckl_context_destroy(const qmckl_context context);
 
 /* ---- Errors: org/qmckl_error.org:241, 433, 493, 550, 606, 660 */
@@ -311,6 +312,9 @@ qmckl_exit_code qmckl_get_jastrow_champ_factor_een(qmckl_context context, double
                                                    const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_factor_een_gl(qmckl_context context, double* const factor_een_gl,
                                                       const int64_t size_max);
+/* org/qmckl_jastrow_champ.org:14403 */
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_grad(qmckl_context context, double* const factor_een_grad,
+                                                        const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_value(qmckl_context context, double* const value, const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_gl(qmckl_context context, double* const gl, const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_grad(qmckl_context context, double* const grad, const int64_t size_max);
@@ -409,8 +413,11 @@ qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, dou
  * `stream` (a hipStream_t; NULL: a stream of the batch's own); qmckl_amd_batch_replay
  * launches that graph on it.  Each replay reads the
  * current contents of the d_coord[i] buffers.  The contexts' streams are set to the batch's
- * branch streams until qmckl_amd_batch_destroy; the contexts must outlive the batch, must not
- * change sizes, and kernel profiling must be off. */
+ * branch streams until qmckl_amd_batch_destroy; the contexts must outlive the batch, and kernel
+ * profiling must be off.  qmckl_amd_batch_replay returns QMCKL_FAILURE without launching
+ * anything when a context has changed since the recording: its walker, electron or MO count,
+ * its MO precision, any constant data (a setter other than the coordinates), or the address or
+ * size of any of its device buffers (a getter that allocated or grew one). */
 qmckl_exit_code qmckl_amd_batch_record(const qmckl_context* contexts, const int64_t n, void* stream,
                                        const double* const* d_coord, double* const* d_mo_vgl,
                                        double* const* d_value, double* const* d_gl, int64_t* const batch);

@@ -21,6 +21,32 @@ namespace {
 
 constexpr int kBranches = 4;
 
+// What a recorded graph depends on in one context: its sizes, its constant data and the
+// address and size of every device buffer its kernels read or write.  A replay is refused
+// when any of these has changed since the recording (a setter, a new walker count, a
+// buffer that grew and moved).
+struct Snapshot {
+  int64_t walk_num = 0, elec_num = 0, mo_num = 0;
+  uint64_t const_date = 0;
+  int32_t mo_precision = 0;
+  std::vector<std::pair<void*, size_t>> buffers;
+
+  static Snapshot of(Context* c) {
+    Snapshot s;
+    s.walk_num = c->walk_num;
+    s.elec_num = c->elec_num;
+    s.mo_num = c->mo_num;
+    s.const_date = c->const_date;
+    s.mo_precision = c->mo_precision;
+    for (DevBuf* b : device_buffers(c)) s.buffers.emplace_back(b->ptr, b->bytes);
+    return s;
+  }
+  bool operator==(const Snapshot& o) const {
+    return walk_num == o.walk_num && elec_num == o.elec_num && mo_num == o.mo_num &&
+           const_date == o.const_date && mo_precision == o.mo_precision && buffers == o.buffers;
+  }
+};
+
 struct Batch {
   hipStream_t origin = nullptr;
   bool own_origin = false;  // created here when the caller passes no stream
@@ -30,6 +56,7 @@ struct Batch {
   hipGraphExec_t exec = nullptr;
   int nbranch = 0;
   std::vector<qmckl_context> contexts;
+  std::vector<Snapshot> recorded;  // per context, taken after the capture
 };
 
 std::mutex g_batch_mutex;
@@ -133,6 +160,7 @@ qmckl_exit_code qmckl_amd_batch_record(const qmckl_context* contexts, const int6
     destroy_batch(b);
     return rc != QMCKL_SUCCESS ? rc : hip_fail(contexts[0], e, fn);
   }
+  for (int64_t i = 0; i < n; ++i) b->recorded.push_back(Snapshot::of(ctx_of(contexts[i])));
   std::lock_guard<std::mutex> g(g_batch_mutex);
   const int64_t h = g_next_batch++;
   g_batches[h] = b;
@@ -148,9 +176,17 @@ qmckl_exit_code qmckl_amd_batch_replay(const int64_t batch) {
     if (it == g_batches.end()) return QMCKL_INVALID_ARG_1;
     b = it->second;
   }
-  // the graph reads and writes the contexts' device buffers: they must all still exist
-  for (qmckl_context ctx : b->contexts)
-    if (!ctx_of(ctx)) return QMCKL_INVALID_CONTEXT;
+  // The graph reads and writes the contexts' device buffers with the sizes and parameters
+  // of the recording: they must all still exist, unchanged.
+  for (size_t i = 0; i < b->contexts.size(); ++i) {
+    Context* c = ctx_of(b->contexts[i]);
+    if (!c) return QMCKL_INVALID_CONTEXT;
+    std::lock_guard<std::recursive_mutex> g(c->mutex);
+    if (!(Snapshot::of(c) == b->recorded[i]))
+      return fail(b->contexts[i], QMCKL_FAILURE, "qmckl_amd_batch_replay",
+                  "the context changed since the batch was recorded (sizes, parameters or device "
+                  "buffers): record the batch again");
+  }
   return hipGraphLaunch(b->exec, b->origin) == hipSuccess ? QMCKL_SUCCESS : QMCKL_FAILURE;
 }
 

@@ -192,10 +192,9 @@ hipError_t copy_to_host(Context* c, void* dst, const void* d_src, size_t bytes) 
   return e;
 }
 
-static void release_device(Context* c) {
-  drain_profile(c);
+std::vector<DevBuf*> device_buffers(Context* c) {
   DevConst& d = c->dc;
-  DevBuf* bufs[] = {&d.shell_nucl, &d.shell_l, &d.shell_prim_start, &d.shell_prim_count,
+  std::vector<DevBuf*> bufs = {&d.shell_nucl, &d.shell_l, &d.shell_prim_start, &d.shell_prim_count,
                     &d.shell_ao_start, &d.prim_expo, &d.prim_coef, &d.ao_shell, &d.ao_pow,
                     &d.ao_factor, &d.nucl_range, &d.nucl_coord, &d.coef_t, &d.coef_ch64,
                     &d.coef_ch32, &d.coef_chbf, &d.ao_desc, &d.ao_desc_off, &d.jas_a, &d.jas_b,
@@ -204,8 +203,13 @@ static void release_device(Context* c) {
                     &c->d_en_dist, &c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl,
                     &c->d_feen_gl, &c->d_jvalue, &c->d_jgl, &c->d_jscratch, &c->d_jint,
                     &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c};
-  for (DevBuf* b : bufs) b->release();
-  for (DevBuf& b : c->d_deriv) b.release();
+  for (DevBuf& b : c->d_deriv) bufs.push_back(&b);
+  return bufs;
+}
+
+static void release_device(Context* c) {
+  drain_profile(c);
+  for (DevBuf* b : device_buffers(c)) b->release();
   for (int b = 0; b < 2; ++b) {
     if (c->stage[b]) (void)hipHostFree(c->stage[b]);
     if (c->stage_ev[b]) (void)hipEventDestroy(c->stage_ev[b]);
@@ -283,9 +287,13 @@ qmckl_context qmckl_context_copy(const qmckl_context context) {
   n->asymp_jasb[1] = c->asymp_jasb[1];
   if (c->point_on_device && c->point_num > 0) {
     // Bring device-only coordinates back to the host so the copy is self-contained.
+    // The device coordinates were written on c->stream (a non-blocking stream, or a batch
+    // replay ordered on it): read them back on that stream, then wait for it.
     n->point_coord.resize(3 * c->point_num);
-    if (hipMemcpy(n->point_coord.data(), c->d_coord.ptr, sizeof(double) * 3 * c->point_num,
-                  hipMemcpyDeviceToHost) != hipSuccess) {
+    hipError_t e = hipMemcpyAsync(n->point_coord.data(), c->d_coord.ptr, sizeof(double) * 3 * c->point_num,
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
       delete n;
       return QMCKL_NULL_CONTEXT;
     }
@@ -310,6 +318,15 @@ qmckl_exit_code qmckl_context_set_num_threads(const qmckl_context context, const
   if (!c) return QMCKL_INVALID_CONTEXT;
   if (num_threads <= 0) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_context_set_num_threads", "num_threads <= 0");
   c->num_threads = num_threads;  // host threads are not used by this implementation
+  return QMCKL_SUCCESS;
+}
+
+// org/qmckl_context.org:690-719: back to the default thread count of a new context
+qmckl_exit_code qmckl_context_unset_num_threads(const qmckl_context context) {
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  c->num_threads = Context::kDefaultNumThreads;
   return QMCKL_SUCCESS;
 }
 
@@ -813,7 +830,7 @@ qmckl_exit_code qmckl_get_electron_en_distance(qmckl_context context, double* co
   return copy_out(c, context, c->d_en_dist.ptr, distance, n, fn);
 }
 
-// org/qmckl_electron.org:1022-1192: V_ee[walk] = sum_{i<j} 1/r_ij (from the device distances)
+// org/qmckl_electron.org:1022-1192: V_ee[walk] = sum_{i<j, r_ij > 1e-5} 1/r_ij (from the device distances)
 qmckl_exit_code qmckl_get_electron_ee_potential(qmckl_context context, double* const ee_potential) {
   const char* fn = "qmckl_get_electron_ee_potential";
   Context* c = ctx_of(context);
@@ -828,13 +845,16 @@ qmckl_exit_code qmckl_get_electron_ee_potential(qmckl_context context, double* c
   for (int64_t w = 0; w < W; ++w) {
     double e = 0.0;
     for (int64_t j = 0; j < Ne; ++j)
-      for (int64_t i = 0; i < j; ++i) e += 1.0 / d[(w * Ne + j) * Ne + i];
+      for (int64_t i = 0; i < j; ++i) {
+        const double r = d[(w * Ne + j) * Ne + i];
+        if (std::fabs(r) > 1e-5) e += 1.0 / r;  // coincident electrons are left out (:1172)
+      }
     ee_potential[w] = e;
   }
   return QMCKL_SUCCESS;
 }
 
-// org/qmckl_electron.org:1477-1658: V_en[walk] = -sum_{i,a} Z_a / r_ia
+// org/qmckl_electron.org:1477-1658: V_en[walk] = -sum_{i,a, r_ia > 1e-6} Z_a / r_ia
 qmckl_exit_code qmckl_get_electron_en_potential(qmckl_context context, double* const en_potential) {
   const char* fn = "qmckl_get_electron_en_potential";
   Context* c = ctx_of(context);
@@ -849,7 +869,10 @@ qmckl_exit_code qmckl_get_electron_en_potential(qmckl_context context, double* c
   for (int64_t w = 0; w < W; ++w) {
     double e = 0.0;
     for (int64_t i = 0; i < Ne; ++i)
-      for (int64_t a = 0; a < Nn; ++a) e -= c->nucl_charge[a] / d[(w * Ne + i) * Nn + a];
+      for (int64_t a = 0; a < Nn; ++a) {
+        const double r = d[(w * Ne + i) * Nn + a];
+        if (std::fabs(r) > 1e-6) e -= c->nucl_charge[a] / r;  // an electron on a nucleus is left out (:1638)
+      }
     en_potential[w] = e;
   }
   return QMCKL_SUCCESS;

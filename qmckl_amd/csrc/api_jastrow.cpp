@@ -811,6 +811,27 @@ qmckl_exit_code qmckl_get_jastrow_champ_grad(qmckl_context context, double* cons
   return QMCKL_SUCCESS;
 }
 
+// org/qmckl_jastrow_champ.org:14403, 14448-14475: factor_een_grad[walk][3][elec], the
+// gradient rows of factor_een_gl (the same e-e-n sums, without the Laplacian)
+qmckl_exit_code qmckl_get_jastrow_champ_factor_een_grad(qmckl_context context, double* const out,
+                                                        const int64_t size_max) {
+  const char* fn = "qmckl_get_jastrow_champ_factor_een_grad";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "Null pointer");
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  const int64_t W = c->walk_num, Ne = c->elec_num;
+  if (size_max < W * 3 * Ne) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small. Expected 3*walk_num*elec_num");
+  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  hipError_t e = hipMemcpy2DAsync(out, sizeof(double) * 3 * Ne, c->d_feen_gl.ptr, sizeof(double) * 4 * Ne,
+                                  sizeof(double) * 3 * Ne, W, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
 /* ---- device-resident results */
 qmckl_exit_code qmckl_amd_get_jastrow_champ_value_device(qmckl_context context, double* const d_value, const int64_t size_max) {
   const char* fn = "qmckl_amd_get_jastrow_champ_value_device";

@@ -75,7 +75,10 @@ struct Context {
   std::recursive_mutex mutex;
   uint64_t date = 1;        // bumped by every setter
   uint64_t const_date = 1;  // bumped by every setter except the coordinate setters
-  int32_t num_threads = 1;
+  // host threads of the reference's OpenMP loops; the evaluation runs on the GPU, so this is
+  // only recorded (the host copies of large results pick their own thread count)
+  static constexpr int32_t kDefaultNumThreads = 1;
+  int32_t num_threads = kDefaultNumThreads;
 
   // error record, org/qmckl_error.org:433-470
   qmckl_exit_code err_code = QMCKL_SUCCESS;
@@ -190,6 +193,8 @@ inline void touch_const(Context* c) {
   c->const_date = c->date;
 }
 Context* ctx_of(qmckl_context context);  // nullptr if not a live context
+// Every device allocation a context owns (the constant tables, inputs, results, scratch).
+std::vector<DevBuf*> device_buffers(Context* c);
 qmckl_exit_code fail(qmckl_context context, qmckl_exit_code code, const char* fn, const char* msg);
 // Makes the device, stream and constant tables ready.
 qmckl_exit_code device_ready(Context* c, qmckl_context handle, const char* fn);

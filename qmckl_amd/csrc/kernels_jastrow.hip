@@ -36,6 +36,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kPairBlock = 1024;  // k_jas_pair: up to 16 waves per walker, one electron i per thread
+// Threads per k_jas_pair<C> block at most: 1024 limits a thread to 128 VGPRs, enough for the
+// q = 0 sums of orders up to 6; orders 7 and 8 hold more and take 256 threads (no spill).
+constexpr int pair_block(int C) { return C <= 6 ? kPairBlock : 256; }
 
 // ---- (l, k, m) of the c-coefficients, org/qmckl_jastrow_champ.org:12000-12040 ----------
 constexpr int dimc(int C) {
@@ -297,10 +300,10 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
 // R0 scratch layout: [W][Ne][cord-1][5]; gN: see gn_index.
 // ------------------------------------------------------------------------------------
 // C (= cord, 0..8) is a template parameter so that only the C - 1 orders of the q = 0 sums
-// are held in registers.  The block has 64 ceil(Ne / 64) threads (at most kPairBlock), one
+// are held in registers.  The block has 64 ceil(Ne / 64) threads (at most pair_block(C)), one
 // electron i per thread.
 template <int C>
-__global__ void __launch_bounds__(kPairBlock) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
+__global__ void __launch_bounds__(pair_block(C)) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
                                                      double* __restrict__ gN, double* __restrict__ Sg) {
   constexpr int K1 = C >= 2 ? C - 1 : 0;
   extern __shared__ double lds[];
@@ -2117,7 +2120,7 @@ bool een_fused(const JasDevView& v) {
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s) {
   if (een_fused(v)) return hipSuccess;  // done by the resident e-e-n kernel
   const ScratchView sv = scratch_view(v, o);
-  const int pair_threads = (int)std::min<int64_t>(kPairBlock, 64 * ((v.elec_num + 63) / 64));
+  const int pair_threads = (int)std::min<int64_t>(pair_block((int)v.cord), 64 * ((v.elec_num + 63) / 64));
   const size_t lds_pair = sizeof(double) * (3 * v.elec_num + kPairBlock / 64);
   if (lds_pair > 160 * 1024) return hipErrorInvalidValue;
   switch (v.cord) {

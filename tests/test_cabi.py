@@ -129,3 +129,19 @@ def test_multi_gpu_request_is_refused():
     r = subprocess.run(["python", "bench.py", "--gpus", "2"], capture_output=True, text=True,
                        cwd=q.HERE + "/..")
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_c99_program_against_the_header(tmp_path):
+    """A C99 caller (tests/c_abi_host.c) compiles against include/qmckl.h with
+    `gcc -std=c99 -pedantic -Werror`, links libqmckl_amd.so and passes its host-side checks."""
+    import os
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    libdir = os.path.dirname(q.LIB_PATH)
+    exe = str(tmp_path / "c_abi_host")
+    subprocess.run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror",
+                    "-I", os.path.join(here, "..", "include"), os.path.join(here, "c_abi_host.c"),
+                    "-o", exe, "-L", libdir, "-lqmckl_amd", f"-Wl,-rpath,{libdir}", "-lm"],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
