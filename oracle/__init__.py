@@ -53,8 +53,8 @@ def lib():
         L.oracle_cutoff.restype = _D
         L.oracle_cutoff.argtypes = [_I32]
         L.oracle_prim_vgl.argtypes = [_I, _I, _P, _P, _P, _I32, _I, _P, _P]
-        L.oracle_shell_vgl.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I, _P, _P]
-        L.oracle_ao_vgl.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I, _P, _P]
+        L.oracle_shell_vgl.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I, _P, _P]
+        L.oracle_ao_vgl.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I, _P, _P]
         L.oracle_mo_vgl.argtypes = [_I, _I, _I, _P, _P, _P]
         L.oracle_ee_distance.argtypes = [_I, _I, _P, _P]
         L.oracle_en_distance.argtypes = [_I, _I, _I, _P, _P, _P]
@@ -81,6 +81,7 @@ def _basis_args(s):
         _c(s.nucl_coord), _c(s.nucleus_index, np.int64), _c(s.nucleus_shell_num, np.int64),
         _c(s.shell_ang_mom, np.int32), _c(s.shell_prim_num, np.int64), _c(s.shell_prim_index, np.int64),
         _c(s.shell_factor), _c(s.exponent), _c(s.coefficient), _c(s.prim_factor),
+        _c(np.zeros(s.shell_num) if getattr(s, "r_power", None) is None else s.r_power, np.int32),
     ]
 
 

@@ -19,6 +19,7 @@
  * the tests under tests/.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -46,7 +47,7 @@ static void finalize_basis(int64_t nucl_num, int64_t shell_num, int64_t prim_num
                            const int32_t *shell_ang_mom, const int64_t *shell_prim_num,
                            const int64_t *shell_prim_index, const double *shell_factor,
                            const double *exponent, const double *coefficient,
-                           const double *prim_factor, basis_extra *b) {
+                           const double *prim_factor, const int32_t *r_power, basis_extra *b) {
   b->coef_normalized = (double *)calloc(prim_num, sizeof(double));
   b->nucleus_range = (double *)calloc(nucl_num, sizeof(double));
   b->nucleus_max_ang_mom = (int32_t *)calloc(nucl_num, sizeof(int32_t));
@@ -64,7 +65,9 @@ static void finalize_basis(int64_t nucl_num, int64_t shell_num, int64_t prim_num
       b->ao_index[s] = k;
       k += (l + 1) * (l + 2) / 2;
       for (int64_t p = shell_prim_index[s]; p < shell_prim_index[s] + shell_prim_num[s]; ++p) {
-        const double range = 1.0 / exponent[p]; /* r_power = 0 for Gaussians */
+        /* :3195-3205: range = (r_power + 1) / alpha; r_power defaults to 0 (:2982-3008) */
+        const double n_val = (double)((r_power ? r_power[s] : 0) + 1);
+        const double range = n_val / exponent[p];
         if (range > b->nucleus_range[a]) b->nucleus_range[a] = range;
       }
     }
@@ -196,10 +199,12 @@ int oracle_shell_vgl(int64_t nucl_num, int64_t shell_num, int64_t prim_num, cons
                      const int32_t *shell_ang_mom, const int64_t *shell_prim_num,
                      const int64_t *shell_prim_index, const double *shell_factor,
                      const double *exponent, const double *coefficient, const double *prim_factor,
-                     int32_t precision, int64_t point_num, const double *coord, double *shell_vgl) {
+                     const int32_t *r_power, int32_t precision, int64_t point_num, const double *coord,
+                     double *shell_vgl) {
   basis_extra b;
   finalize_basis(nucl_num, shell_num, prim_num, nucleus_index, nucleus_shell_num, shell_ang_mom,
-                 shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor, &b);
+                 shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor,
+                 r_power, &b);
   const double cutoff = oracle_cutoff(precision);
   for (int64_t ip = 0; ip < point_num; ++ip)
     shell_vgl_point(coord[ip], coord[ip + point_num], coord[ip + 2 * point_num], nucl_num, shell_num,
@@ -216,11 +221,12 @@ int oracle_ao_vgl(int64_t nucl_num, int64_t shell_num, int64_t prim_num, int64_t
                   const int64_t *nucleus_shell_num, const int32_t *shell_ang_mom,
                   const int64_t *shell_prim_num, const int64_t *shell_prim_index,
                   const double *shell_factor, const double *exponent, const double *coefficient,
-                  const double *prim_factor, const double *ao_factor, int32_t precision,
-                  int64_t point_num, const double *coord, double *ao_vgl) {
+                  const double *prim_factor, const int32_t *r_power, const double *ao_factor,
+                  int32_t precision, int64_t point_num, const double *coord, double *ao_vgl) {
   basis_extra b;
   finalize_basis(nucl_num, shell_num, prim_num, nucleus_index, nucleus_shell_num, shell_ang_mom,
-                 shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor, &b);
+                 shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor,
+                 r_power, &b);
   const double cutoff = oracle_cutoff(precision);
   int32_t lmax_all = 0;
   for (int64_t a = 0; a < nucl_num; ++a)
@@ -447,7 +453,10 @@ int oracle_jastrow(int64_t walk_num, int64_t elec_num, int64_t up_num, int64_t n
   for (int64_t n = 0; n < dim_c; ++n)
     for (int64_t a = 0; a < Nn; ++a) cfull[n * Nn + a] = c_vector[type_nucl_vector[a] * dim_c + n];
 
-#pragma omp parallel
+  /* Fewer walkers than threads: one thread walks the walkers and the whole team shares the
+     P-tensor sums of each (see below); otherwise one walker per thread. */
+  const int inner = W < omp_get_max_threads();
+#pragma omp parallel if (!inner)
   {
     double *xe = (double *)malloc(sizeof(double) * 3 * Ne);
     double *rr = (double *)malloc(sizeof(double) * Ne * Ne);       /* ee distance [j][i] */
@@ -460,7 +469,6 @@ int oracle_jastrow(int64_t walk_num, int64_t elec_num, int64_t up_num, int64_t n
     double *geg = (double *)malloc(sizeof(double) * (C + 1) * Ne * 4 * Ne); /* [k][j][4][i] */
     double *gn = (double *)malloc(sizeof(double) * (C + 1) * Nn * Ne);      /* [q][a][i] */
     double *gng = (double *)malloc(sizeof(double) * (C + 1) * Nn * 4 * Ne); /* [q][a][4][i] */
-    double *gnt = (double *)malloc(sizeof(double) * Ne * (C + 1) * Nn);     /* [m][q][a] */
     const int64_t ntc = (C > 0 ? C : 1) * (C + 1) * Nn * Ne;
     double *tc = (double *)malloc(sizeof(double) * ntc);           /* [k][q][a][i] */
     double *dtc = (double *)malloc(sizeof(double) * ntc * 4);      /* [k][q][a][4][i] */
@@ -643,25 +651,26 @@ int oracle_jastrow(int64_t walk_num, int64_t elec_num, int64_t up_num, int64_t n
         }
 
       /* --- e-e-n: P tensors (tmp_c, dtmp_c) -------------------------------------- */
-      for (int64_t m = 0; m < Ne; ++m)
-        for (int64_t q = 0; q <= C; ++q)
-          for (int64_t a = 0; a < Nn; ++a) gnt[(m * (C + 1) + q) * Nn + a] = gn[(q * Nn + a) * Ne + m];
+      /* tmp_c[k][q][a][i] = sum_m een_e[k][m][i] een_n[q][a][m] and dtmp_c likewise with
+         een_e_gl, each element summed over m in ascending order (q > c - k stay zero).
+         The (k, a) blocks are independent; with fewer walkers than threads the team shares
+         them. */
       memset(tc, 0, sizeof(double) * ntc);
       memset(dtc, 0, sizeof(double) * ntc * 4);
+#pragma omp parallel for collapse(2) schedule(dynamic) if (inner)
       for (int64_t k = 0; k < C; ++k)
-        for (int64_t i = 0; i < Ne; ++i)
-          for (int64_t m = 0; m < Ne; ++m) {
-            const double g = ge[(k * Ne + m) * Ne + i];
-            double dg[4];
-            for (int c = 0; c < 4; ++c) dg[c] = geg[((k * Ne + m) * 4 + c) * Ne + i];
-            const double *gm = gnt + m * (C + 1) * Nn;
-            for (int64_t q = 0; q <= C - k; ++q)
-              for (int64_t a = 0; a < Nn; ++a) {
-                const double r = gm[q * Nn + a];
-                tc[((k * (C + 1) + q) * Nn + a) * Ne + i] += g * r;
-                double *o = dtc + (((k * (C + 1) + q) * Nn + a) * 4) * Ne + i;
-                for (int c = 0; c < 4; ++c) o[c * Ne] += dg[c] * r;
-              }
+        for (int64_t a = 0; a < Nn; ++a)
+          for (int64_t q = 0; q <= C - k; ++q) {
+            double *t = tc + ((k * (C + 1) + q) * Nn + a) * Ne;
+            double *dt = dtc + (((k * (C + 1) + q) * Nn + a) * 4) * Ne;
+            for (int64_t m = 0; m < Ne; ++m) {
+              const double r = gn[(q * Nn + a) * Ne + m];
+              const double *g = ge + (k * Ne + m) * Ne;
+              const double *dg = geg + ((k * Ne + m) * 4) * Ne;
+              for (int64_t i = 0; i < Ne; ++i) t[i] += g[i] * r;
+              for (int c = 0; c < 4; ++c)
+                for (int64_t i = 0; i < Ne; ++i) dt[c * Ne + i] += dg[c * Ne + i] * r;
+            }
           }
 
       /* --- J_eeN and its GL ------------------------------------------------------ */
@@ -730,7 +739,7 @@ int oracle_jastrow(int64_t walk_num, int64_t elec_num, int64_t up_num, int64_t n
       if (dtmp_c) memcpy(dtmp_c + w * ntc * 4, dtc, sizeof(double) * ntc * 4);
     }
     free(xe); free(rr); free(fr); free(fg); free(R); free(fn); free(fng);
-    free(ge); free(geg); free(gn); free(gng); free(gnt); free(tc); free(dtc);
+    free(ge); free(geg); free(gn); free(gng); free(tc); free(dtc);
     free(gee); free(gen); free(geen);
   }
   free(asymp_jasa);

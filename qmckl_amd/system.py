@@ -58,6 +58,7 @@ class System:
     spin_independent: int = 0
     precision: int = 53
     keep_mo: np.ndarray = None  # optional select_mo mask
+    r_power: np.ndarray = None  # optional int32 [shell] (qmckl_set_ao_basis_r_power); default 0
     name: str = ""
     meta: dict = field(default_factory=dict)
 
@@ -142,6 +143,9 @@ def load(ctx, s: System, jastrow=True, mo=True):
     ]:
         a = conv(arr); keep.append(a)
         c(getattr(L, "qmckl_set_ao_basis_" + name)(ctx, ptr(a), a.size), "set_ao_basis_" + name)
+    if s.r_power is not None:
+        a = i32(s.r_power); keep.append(a)
+        c(L.qmckl_set_ao_basis_r_power(ctx, ptr(a), a.size), "set_ao_basis_r_power")
     c(L.qmckl_set_ao_basis_ao_num(ctx, s.ao_num), "set_ao_basis_ao_num")
     a = f64(s.ao_factor); keep.append(a)
     c(L.qmckl_set_ao_basis_ao_factor(ctx, ptr(a), a.size), "set_ao_basis_ao_factor")

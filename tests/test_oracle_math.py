@@ -34,12 +34,19 @@ def _ao_closed_form(s, x):
     cutoff = -np.log(2.0 ** (1 - s.precision))
     P = x.shape[1]
     out = np.zeros((P, s.ao_num))
+    rp = np.zeros(s.shell_num, np.int64) if s.r_power is None else np.asarray(s.r_power, np.int64)
     k = 0
     for a in range(s.nucl_num):
         R = s.nucl_coord[:, a][:, None]
         d = x - R
         r2 = (d * d).sum(0)
-        for sh in range(s.nucleus_index[a], s.nucleus_index[a] + s.nucleus_shell_num[a]):
+        shells = range(s.nucleus_index[a], s.nucleus_index[a] + s.nucleus_shell_num[a])
+        # nucleus range: max over its primitives of (r_power + 1) / alpha
+        # (org/qmckl_ao.org:3195-3205); points beyond r^2 > cutoff * range see no AO of it
+        rng_a = max([0.0] + [(rp[sh] + 1) / s.exponent[p] for sh in shells
+                             for p in range(s.shell_prim_index[sh], s.shell_prim_index[sh] + s.shell_prim_num[sh])])
+        far = r2 > cutoff * rng_a
+        for sh in shells:
             l = int(s.shell_ang_mom[sh])
             p0, np_ = int(s.shell_prim_index[sh]), int(s.shell_prim_num[sh])
             rad = np.zeros(P)
@@ -47,6 +54,7 @@ def _ao_closed_form(s, x):
                 ar2 = s.exponent[p] * r2
                 rad += np.where(ar2 > cutoff, 0.0, s.coefficient[p] * s.prim_factor[p] * np.exp(-ar2))
             rad *= s.shell_factor[sh]
+            rad[far] = 0.0
             for pa in range(l, -1, -1):
                 for pb in range(l - pa, -1, -1):
                     pc = l - pa - pb
@@ -80,6 +88,31 @@ def test_ao_values_match_closed_form(name):
     ref = _ao_closed_form(s, x)
     scale = np.abs(ref).max()
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * scale)
+
+
+def test_ao_values_with_r_power_match_closed_form():
+    """r_power enters the nucleus range (r_power + 1) / alpha: r_power = -1 on every shell of
+    the first nucleus gives it range 0 (its AOs vanish at every point off the nucleus),
+    larger powers on the second nucleus widen its range (no change: the primitive screening
+    already stops within the r_power = 0 range)."""
+    s = _ao_system("small")
+    s.precision = 53
+    rp = np.zeros(s.shell_num, np.int32)
+    n0 = slice(s.nucleus_index[0], s.nucleus_index[0] + s.nucleus_shell_num[0])
+    n1 = slice(s.nucleus_index[1], s.nucleus_index[1] + s.nucleus_shell_num[1])
+    rp[n0] = -1
+    rp[n1] = 2
+    s.r_power = rp
+    x = _points(s, 24, 3)
+    got = oracle.ao_vgl(s, x)
+    ref = _ao_closed_form(s, x)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got[:, 0, :], ref, rtol=1e-12, atol=1e-14 * scale)
+    nao0 = sum((l + 1) * (l + 2) // 2 for l in s.shell_ang_mom[n0])
+    assert np.all(got[:, :, :nao0] == 0.0) and np.abs(got[:, 0, nao0:]).max() > 0.0
+    s.r_power = None
+    base = oracle.ao_vgl(s, x)
+    np.testing.assert_array_equal(got[:, :, nao0:], base[:, :, nao0:])
 
 
 @pytest.mark.parametrize("name", ["small", "high_l"])
@@ -159,6 +192,45 @@ def _een_brute_force(s, x, W):
     return J
 
 
+def _ee_brute_force(s, x, W):
+    """J_ee from its definition (org/qmckl_jastrow_champ.org:2340-2380, 3582-3650): over
+    pairs i < j with f = (1 - exp(-kappa r_ij)) / kappa,
+    s_ij b0 f / (1 + b1 f) + sum_{k>=2} b_k f^k - (s_ij b0 / kappa / (1 + b1 / kappa)
+    + sum_{k>=2} b_k / kappa^k), where s_ij = 1/2 for a same-spin pair unless the Jastrow is
+    spin independent, else 1 (the shift makes each pair term vanish as r -> infinity)."""
+    Ne, up, kap, b = s.elec_num, s.up, s.rescale_ee, s.b_vector
+    xw = x.reshape(3, W, Ne)
+    J = np.zeros(W)
+    for w in range(W):
+        for j in range(Ne):
+            for i in range(j):
+                f = (1.0 - np.exp(-kap * np.linalg.norm(xw[:, w, i] - xw[:, w, j]))) / kap
+                same = (j < up) or (i >= up)
+                sij = 1.0 if (s.spin_independent == 1 or not same) else 0.5
+                term = sij * b[0] * f / (1.0 + b[1] * f) + sum(b[k] * f ** k for k in range(2, s.bord + 1))
+                shift = sij * b[0] / kap / (1.0 + b[1] / kap) + sum(b[k] / kap ** k for k in range(2, s.bord + 1))
+                J[w] += term - shift
+    return J
+
+
+@pytest.mark.parametrize("spin_independent", [0, 1])
+def test_ee_value_matches_brute_force(spin_independent):
+    s, x, W = S.jastrow_workload(3, 4, 3, 5, 5, 5, walk_num=3)
+    s.spin_independent = spin_independent
+    got = oracle.jastrow(s, x, W, want=("factor_ee",))["factor_ee"]
+    np.testing.assert_allclose(got, _ee_brute_force(s, x, W), rtol=1e-12, atol=1e-12)
+
+
+def test_ee_gl_spin_independent_matches_finite_differences():
+    s, x, W = S.jastrow_workload(3, 3, 2, 5, 5, 5, walk_num=2)
+    s.spin_independent = 1
+    got = oracle.jastrow(s, x, W, want=("factor_ee_gl",))["factor_ee_gl"]
+    fd = _fd_gl(s, x, W, "factor_ee")
+    for c, tol in ((0, 1e-6), (1, 1e-6), (2, 1e-6), (3, 1e-4)):
+        scale = max(np.abs(got[:, c]).max(), 1e-3)
+        assert np.abs(fd[:, c] - got[:, c]).max() <= tol * scale, f"component {c}"
+
+
 @pytest.mark.parametrize("cfg", [(3, 3, 2, 5, 5, 5), (4, 4, 3, 3, 4, 3), (2, 3, 3, 5, 5, 7)])
 def test_een_value_matches_brute_force(cfg):
     s, x, W = S.jastrow_workload(*cfg, walk_num=3)
@@ -213,3 +285,40 @@ def test_jastrow_value_and_gl_identities():
     for c, tol in ((0, 1e-6), (1, 1e-6), (2, 1e-6), (3, 1e-4)):
         scale = np.abs(r["gl"][:, c]).max()
         assert np.abs(fd[:, c] - r["gl"][:, c]).max() <= tol * scale, f"gl component {c}"
+
+
+def test_rescaled_distance_gl_match_finite_differences():
+    """ee_rescaled_gl[w][j][i][:] and en_rescaled_gl[w][a][i][:] are the gradient and
+    Laplacian, in the coordinates of electron i, of the rescaled distances
+    f(r) = (1 - exp(-kappa r)) / kappa of ee_rescaled[w][j][i] and en_rescaled[w][a][i]
+    (org/qmckl_jastrow_champ.org:3077-3120, 6601-6700; org/qmckl_distance.org:1345-1509)."""
+    s, x, W = S.jastrow_workload(3, 3, 2, 5, 5, 5, walk_num=2)
+    Ne = s.elec_num
+    keys = ("ee_rescaled", "en_rescaled", "ee_rescaled_gl", "en_rescaled_gl")
+    r = oracle.jastrow(s, x, W, want=keys)
+    xw = x.reshape(3, W, Ne)
+    h1, h2 = 1e-5, 1e-3
+
+    def at(y):
+        o = oracle.jastrow(s, np.ascontiguousarray(y.reshape(3, W * Ne)), W, want=keys[:2])
+        return o["ee_rescaled"], o["en_rescaled"]
+
+    for i in range(Ne):
+        lap_ee = -6.0 * r["ee_rescaled"][:, :, i]
+        lap_en = -6.0 * r["en_rescaled"][:, :, i]
+        for d in range(3):
+            v = {}
+            for h, sg in ((h1, 1), (h1, -1), (h2, 1), (h2, -1)):
+                y = xw.copy()
+                y[d, :, i] += sg * h
+                v[(h, sg)] = at(y)
+            fd_ee = (v[(h1, 1)][0] - v[(h1, -1)][0]) / (2 * h1)
+            fd_en = (v[(h1, 1)][1] - v[(h1, -1)][1]) / (2 * h1)
+            off = np.arange(Ne) != i
+            np.testing.assert_allclose(fd_ee[:, off, i], r["ee_rescaled_gl"][:, off, i, d], atol=1e-8)
+            np.testing.assert_allclose(fd_en[:, :, i], r["en_rescaled_gl"][:, :, i, d], atol=1e-8)
+            lap_ee = lap_ee + v[(h2, 1)][0][:, :, i] + v[(h2, -1)][0][:, :, i]
+            lap_en = lap_en + v[(h2, 1)][1][:, :, i] + v[(h2, -1)][1][:, :, i]
+        off = np.arange(Ne) != i
+        np.testing.assert_allclose((lap_ee / h2 ** 2)[:, off], r["ee_rescaled_gl"][:, off, i, 3], atol=2e-5)
+        np.testing.assert_allclose(lap_en / h2 ** 2, r["en_rescaled_gl"][:, :, i, 3], atol=2e-5)
