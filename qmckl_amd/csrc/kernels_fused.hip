@@ -16,10 +16,12 @@
 // Operand precisions:
 //   FP64   v_mfma_f64_16x16x4_f64, FP64 accumulation.
 //   FP32   v_mfma_f32_16x16x4_f32; FP32 sums of each chunk added into FP64.
-//   BF16X3 both operands as three bfloat16 terms (the AO values by an exact truncation split
-//          of float(x), C_t by a rounded split of the FP64 value); per chunk the six leading
-//          cross products (smallest first) through v_mfma_f32_16x16x32_bf16 into FP32, then
-//          added into FP64.
+//   BF16X3 both operands as three bfloat16 terms, each rounded to nearest from the FP64
+//          remainder of the terms before it (about 26 significant bits per operand); per
+//          chunk the six leading cross products through v_mfma_f32_16x16x32_bf16, the
+//          leading one (hi x hi) into an FP32 sum of its own and the five smaller ones
+//          (smallest first) into a second, so that the FP32 rounding of the large sum is not
+//          compounded by five more; both sums are then added into FP64.
 // Why one shared tile, not a per-wave layout: the AO evaluation dominates the vector work,
 // and evaluating each AO value once per workgroup at 2-3 waves per SIMD beats layouts that
 // keep 25 FP64 accumulator tiles per wave in registers (one wave per SIMD) by 15-30 % at
@@ -149,16 +151,16 @@ struct TileA<2> {
   static constexpr size_t bytes = sizeof(uint16_t) * 3 * ROWS * KP16;
 };
 
-// six leading cross products of a (3 planes) x b (3 planes), smallest first, into FP32
-__device__ __forceinline__ f4 bf16x3_product(const bf8 (&a)[3], const bf8 (&b)[3]) {
-  f4 q = f4{0.f, 0.f, 0.f, 0.f};
-  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], q, 0, 0, 0);
-  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], q, 0, 0, 0);
-  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], q, 0, 0, 0);
-  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], q, 0, 0, 0);
-  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], q, 0, 0, 0);
-  q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], q, 0, 0, 0);
-  return q;
+// six leading cross products of a (3 planes) x b (3 planes) in FP32: the leading one
+// (hi x hi) into `hi`, the five smaller ones, smallest first, into `lo`
+__device__ __forceinline__ void bf16x3_product(const bf8 (&a)[3], const bf8 (&b)[3], f4& hi, f4& lo) {
+  const f4 z = f4{0.f, 0.f, 0.f, 0.f};
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], z, 0, 0, 0);
+  hi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], z, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], lo, 0, 0, 0);
 }
 
 // Product of the operand tile's row tile r with column tile j of the chunk's C_t, added
@@ -193,9 +195,10 @@ __device__ __forceinline__ void tile_product(d4& acc, const void* at, const void
       a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(A + pl * ROWS * KP16 + (16 * r + li) * KP16 + 8 * lg));
       b[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(B + ((pl * NG16) + 16 * j + li) * KP16 + 8 * lg));
     }
-    const f4 q = bf16x3_product(a, b);
+    f4 hi, lo;
+    bf16x3_product(a, b, hi, lo);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] += (double)q[k];
+    for (int k = 0; k < 4; ++k) acc[k] += (double)hi[k] + (double)lo[k];
   }
 }
 
@@ -267,7 +270,7 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
           static_cast<float*>(at)[(5 * q + k) * LDA32 + kk] = (float)r[k];
         } else {
           uint16_t h, m, l;
-          split3_trunc(r[k], h, m, l);
+          split3(r[k], h, m, l);
           uint16_t* T = static_cast<uint16_t*>(at);
           const int idx = (5 * q + k) * KP16 + kk;
           T[idx] = h;

@@ -216,10 +216,10 @@ def _jastrow_gpu(ctx, W, Ne):
     }
 
 
-def _compare_jastrow(got, ref, tag):
+def _compare_jastrow(got, ref, tag, factor_tol=1e-12):
     for k in ("factor_ee", "factor_en", "factor_een"):
         err = np.abs(got[k] - ref[k]) / np.maximum(1.0, np.abs(ref[k]))
-        assert err.max() <= 1e-12, f"{tag} {k}: {err.max():.3e}"
+        assert err.max() <= factor_tol, f"{tag} {k}: {err.max():.3e}"
     # value = exp(J): relative error ~ |dJ|; J < -100 is flushed to exactly 0 by both, and
     # exp(J) overflows to +inf in both above J = 709.78
     J = ref["factor_ee"] + ref["factor_en"] + ref["factor_een"]
