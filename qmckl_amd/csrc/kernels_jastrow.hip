@@ -2077,17 +2077,24 @@ __global__ void k_en_distance(const double* __restrict__ coord, int64_t P, const
 
 // Scratch of the Jastrow kernels: gN, R0, Jt and Sg, each starting on a 256-byte boundary
 // (the e-e-n kernel copies gN into LDS 16 bytes per lane).
+bool een_fused(const JasDevView& v);
+
+// Regions of the Jastrow scratch, in doubles, each 256-byte aligned.  The pair kernel's
+// outputs (gN, R0, Sg) exist only when it runs; the resident e-e-n kernel, which does its
+// work, writes only the per-tile sums Jt and JtP.  Every kernel writes each scratch element
+// before any kernel reads it, so the scratch needs no clearing.
 struct ScratchLayout {
   int64_t gN, R0, Jt, Sg, JtP, total;
   explicit ScratchLayout(const JasDevView& v) {
     auto up = [](int64_t n) { return (n + 31) & ~int64_t(31); };
     const int64_t C = v.cord, W = v.walk_num;
+    const bool fused = een_fused(v), pair = !fused && C >= 2;
     gN = 0;
-    R0 = up(gN + W * 16 * gn_na(v.nucl_num) * gn_jp(v.elec_num));
-    Jt = up(R0 + (C >= 2 ? W * v.elec_num * (C - 1) * 5 : 0));
+    R0 = up(gN + (fused ? 0 : W * 16 * gn_na(v.nucl_num) * gn_jp(v.elec_num)));
+    Jt = up(R0 + (pair ? W * v.elec_num * (C - 1) * 5 : 0));
     Sg = up(Jt + W * een_shape(v).ntiles);
-    JtP = up(Sg + (C >= 2 ? W * v.nucl_num * (C - 1) : 0));
-    total = up(JtP + 2 * W * een_shape(v).ntiles);
+    JtP = up(Sg + (pair ? W * v.nucl_num * (C - 1) : 0));
+    total = up(JtP + (fused ? 2 * W * een_shape(v).ntiles : 0));
   }
 };
 
