@@ -426,6 +426,23 @@ qmckl_exit_code qmckl_amd_batch_replay(const int64_t batch);
 qmckl_exit_code qmckl_amd_batch_synchronize(const int64_t batch);
 qmckl_exit_code qmckl_amd_batch_destroy(const int64_t batch);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The Jastrow value
+ * e^J [walk_num] and GL [walk_num][4][elec_num] of walk_num walkers at the device coordinates
+ * d_coord ('T' layout [3][walk_num*elec_num]), with the context's parameters, into the device
+ * buffers d_value and d_gl; the same kernels and results as qmckl_amd_get_jastrow_champ_value_device
+ * and _gl_device.  It calls no allocating routine: every intermediate goes to a 64 MiB static
+ * device array of the library (one per GPU, held until the process ends), and walk_num larger
+ * than that scratch holds answers QMCKL_INVALID_ARG_2.  The context's Jastrow tables must
+ * already be on the device and current (any device evaluation of its Jastrow uploads them;
+ * otherwise QMCKL_NOT_PROVIDED).  The context's own coordinates and cached results are not
+ * touched.  Runs on the context's stream; calls that overlap on one GPU share the scratch
+ * and race. */
+qmckl_exit_code qmckl_amd_get_jastrow_champ_device_noalloc(qmckl_context context, const int64_t walk_num,
+                                                           const double* d_coord, const int64_t coord_size,
+                                                           double* const d_value, double* const d_gl,
+                                                           const int64_t gl_size);
+
 /* Kernel timing: when profiling is on, every kernel launch of the context is bracketed by
  * HIP events on the context's stream.  qmckl_amd_get_kernel_times waits for the stream,
  * returns the accumulated milliseconds and launch counts per slot and resets them.

@@ -413,6 +413,15 @@ qmckl_exit_code get_deriv(qmckl_context context, int what, double* out, int64_t 
 
 namespace qmckl_amd {
 
+bool jastrow_view_resident(Context* c, int64_t walk_num, const double* coord, JasDevView& v) {
+  if (c->stream == nullptr || c->device < 0 || c->dc.jas_date < c->const_date) return false;
+  v = jas_view(c);
+  v.walk_num = walk_num;
+  v.coord = coord;
+  v.coord_ld = walk_num * c->elec_num;
+  return true;
+}
+
 // Value, factors and their gradients/Laplacians of all walkers in one pass over the
 // device; org/qmckl_jastrow_champ.org:16481-16560 (value) and :16887-16960 (gl).
 qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn) {

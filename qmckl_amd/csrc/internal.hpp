@@ -293,6 +293,10 @@ struct JasOut {
   double* scratch;                    // partial sums, size jastrow_scratch_doubles()
 };
 int64_t jastrow_scratch_doubles(const JasDevView& v);
+// The Jastrow view of c for walk_num walkers at the device coordinates coord
+// [3][walk_num * elec_num], on the device tables of c; false (and nothing uploaded or
+// allocated) when those tables are not on the device or older than c's constant data.
+bool jastrow_view_resident(Context* c, int64_t walk_num, const double* coord, JasDevView& v);
 // The three kernels of one evaluation, in this order (same stream).
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s);
