@@ -142,6 +142,58 @@ static void split3_host(double x, uint16_t h[3]) {
   h[2] = bf16_rne_host((float)r2);
 }
 
+// Order of the AOs in the fused AO->MO kernel's 32-AO chunks (perm[position] = AO index).
+// The product sums over every AO, so any order gives the MO VGL (up to the order of the FP64
+// sums across chunks).  The kernel skips the chunks whose shells are all screened at every
+// point of a workgroup's tile, and a tile holds 16 points near one another (the same
+// electron of 16 walkers): so shells that vanish at the same points go together.  A shell
+// vanishes beyond r^2 = cutoff * min(range of its nucleus, 1 / smallest exponent); the
+// shells are taken compact ones first (that radius at most the median), then diffuse ones,
+// each group in the Morton order of their nuclei (10 bits per axis over the nuclei's box),
+// whole shells, their AOs in the reference order.
+static std::vector<int64_t> fused_ao_order(const Context* c) {
+  const AoTables& t = c->tables;
+  const int64_t S = t.shell_num, Nn = t.nucl_num;
+  const double cutoff = -log(ldexp(1.0, 1 - c->precision));
+  std::vector<double> r2(S);
+  for (int64_t s = 0; s < S; ++s) {
+    double amin = INFINITY;
+    for (int q = t.shell_prim_start[s]; q < t.shell_prim_start[s] + t.shell_prim_count[s]; ++q)
+      amin = std::min(amin, t.prim_expo[q]);
+    r2[s] = cutoff * std::min(t.nucl_range[t.shell_nucl[s]], 1.0 / amin);
+  }
+  std::vector<double> sorted_r2(r2);
+  std::nth_element(sorted_r2.begin(), sorted_r2.begin() + S / 2, sorted_r2.end());
+  const double median = S ? sorted_r2[S / 2] : 0.0;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t a = 0; a < Nn; ++a)
+    for (int d3 = 0; d3 < 3; ++d3) {
+      lo[d3] = std::min(lo[d3], c->nucl_coord[d3 * Nn + a]);
+      hi[d3] = std::max(hi[d3], c->nucl_coord[d3 * Nn + a]);
+    }
+  std::vector<uint64_t> morton(Nn, 0);
+  for (int64_t a = 0; a < Nn; ++a)
+    for (int d3 = 0; d3 < 3; ++d3) {
+      const double span = hi[d3] - lo[d3];
+      const uint64_t qd = span > 0 ? (uint64_t)((c->nucl_coord[d3 * Nn + a] - lo[d3]) / span * 1023.0) : 0;
+      for (int b = 0; b < 10; ++b) morton[a] |= ((qd >> b) & 1u) << (3 * b + d3);
+    }
+  std::vector<int64_t> order(S);
+  for (int64_t s = 0; s < S; ++s) order[s] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    const bool cx = r2[x] <= median, cy = r2[y] <= median;
+    if (cx != cy) return cx;
+    return morton[t.shell_nucl[x]] < morton[t.shell_nucl[y]];
+  });
+  std::vector<int64_t> perm;
+  perm.reserve(t.ao_num);
+  for (int64_t s : order) {
+    const int64_t n = (int64_t)(t.shell_l[s] + 1) * (t.shell_l[s] + 2) / 2;
+    for (int64_t k = 0; k < n; ++k) perm.push_back(t.shell_ao_start[s] + k);
+  }
+  return perm;
+}
+
 // Uploads the AO tables and nuclear coordinates (once per context date change).
 static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* fn) {
   qmckl_exit_code rc = device_ready(c, handle, fn);
@@ -167,39 +219,59 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
   up(d.ao_factor, t.ao_factor.data(), 8 * t.ao_factor.size());
   up(d.nucl_range, t.nucl_range.data(), 8 * t.nucl_range.size());
   up(d.nucl_coord, c->nucl_coord.data(), 8 * c->nucl_coord.size());
+  const std::vector<int64_t> perm = fused_ao_order(c);
   {
-    // Descriptors of the 32-AO chunks for the fused AO->MO kernel (layout: AoMoArgs).
+    // Descriptors of the 32-AO chunks for the fused AO->MO kernel (layout: AoMoArgs), the
+    // AOs taken in the order `perm`: each chunk lists the shells of its AOs (in order of first
+    // appearance), their nuclei and primitives, with chunk-local indices.
     const int64_t Nao = t.ao_num, nchunk = (Nao + AO_MO_KC - 1) / AO_MO_KC, Nn = t.nucl_num;
     std::vector<double> desc;
     std::vector<int64_t> off(1, 0);
-    int dmax = 0;
+    int dmax = 0, nsmax = 1;
     for (int64_t ch = 0; ch < nchunk; ++ch) {
-      const int64_t k0 = ch * AO_MO_KC, k1 = std::min<int64_t>(k0 + AO_MO_KC, Nao);
-      const int s_lo = t.ao_shell[k0], s_hi = t.ao_shell[k1 - 1];
-      const int a_lo = t.shell_nucl[s_lo], a_hi = t.shell_nucl[s_hi];
-      const int p_lo = t.shell_prim_start[s_lo], p_hi = t.shell_prim_start[s_hi] + t.shell_prim_count[s_hi];
-      const int ns = s_hi - s_lo + 1, na = a_hi - a_lo + 1, np = p_hi - p_lo;
+      std::vector<int> shells, nuclei, shell_local(AO_MO_KC, 0);
+      for (int kk = 0; kk < AO_MO_KC; ++kk) {
+        const int64_t pos = ch * AO_MO_KC + kk;
+        if (pos >= Nao) break;
+        const int sh = t.ao_shell[perm[pos]];
+        auto it = std::find(shells.begin(), shells.end(), sh);
+        shell_local[kk] = (int)(it - shells.begin());
+        if (it == shells.end()) shells.push_back(sh);
+        if (std::find(nuclei.begin(), nuclei.end(), t.shell_nucl[sh]) == nuclei.end())
+          nuclei.push_back(t.shell_nucl[sh]);
+      }
+      int np = 0;
+      for (int sh : shells) np += t.shell_prim_count[sh];
       const size_t at = desc.size();
-      desc.insert(desc.end(), {(double)ns, (double)na, (double)np, 0.0});
-      for (int a = a_lo; a <= a_hi; ++a)
+      desc.insert(desc.end(), {(double)shells.size(), (double)nuclei.size(), (double)np, 0.0});
+      for (int a : nuclei)
         desc.insert(desc.end(), {c->nucl_coord[a], c->nucl_coord[Nn + a], c->nucl_coord[2 * Nn + a],
                                  t.nucl_range[a]});
-      for (int s = s_lo; s <= s_hi; ++s)
-        desc.insert(desc.end(), {(double)(t.shell_nucl[s] - a_lo + 256 * t.shell_prim_count[s]),
-                                 (double)(t.shell_prim_start[s] - p_lo)});
-      for (int q = p_lo; q < p_hi; ++q) desc.insert(desc.end(), {t.prim_expo[q], t.prim_coef[q]});
-      for (int64_t k = k0; k < k0 + AO_MO_KC; ++k) {
-        if (k < Nao)
-          desc.insert(desc.end(), {(double)(t.ao_shell[k] - s_lo) + 256.0 * (double)t.ao_pow[k], t.ao_factor[k]});
+      int p_first = 0;
+      for (int sh : shells) {
+        const int nl = (int)(std::find(nuclei.begin(), nuclei.end(), t.shell_nucl[sh]) - nuclei.begin());
+        desc.insert(desc.end(), {(double)(nl + 256 * t.shell_prim_count[sh]), (double)p_first});
+        p_first += t.shell_prim_count[sh];
+      }
+      for (int sh : shells)
+        for (int q = t.shell_prim_start[sh]; q < t.shell_prim_start[sh] + t.shell_prim_count[sh]; ++q)
+          desc.insert(desc.end(), {t.prim_expo[q], t.prim_coef[q]});
+      for (int kk = 0; kk < AO_MO_KC; ++kk) {
+        const int64_t pos = ch * AO_MO_KC + kk;
+        if (pos < Nao)
+          desc.insert(desc.end(), {(double)shell_local[kk] + 256.0 * (double)t.ao_pow[perm[pos]],
+                                   t.ao_factor[perm[pos]]});
         else
           desc.insert(desc.end(), {0.0, 0.0});
       }
       if (desc.size() % 2) desc.push_back(0.0);
       dmax = std::max<int>(dmax, (int)(desc.size() - at));
+      nsmax = std::max<int>(nsmax, (int)shells.size());
       off.push_back((int64_t)desc.size());
     }
     d.ao_nchunk = (int)nchunk;
     d.ao_desc_max = dmax;
+    d.ao_ns_max = nsmax;
     up(d.ao_desc, desc.data(), 8 * desc.size());
     up(d.ao_desc_off, off.data(), 8 * off.size());
   }
@@ -239,10 +311,12 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     std::vector<uint16_t> bbf((size_t)(ngroup * nchunk) * 3 * NG16 * AO_MO_BF_LD, 0);
     for (int64_t ch = 0; ch < nchunk; ++ch)
       for (int kk = 0; kk < AO_MO_KC; ++kk) {
-        const int64_t k = ch * AO_MO_KC + kk;
-        const double fk = k < Nao ? t.ao_factor[k] : 0.0;
+        // row kk of chunk ch is AO perm[32 ch + kk] (zero past ao_num)
+        const int64_t pos = ch * AO_MO_KC + kk;
+        const int64_t k = pos < Nao ? perm[pos] : Nao;
+        const double fk = pos < Nao ? t.ao_factor[k] : 0.0;
         for (int64_t col = 0; col < pad; ++col) {
-          const double x = ct[k * pad + col] * fk;
+          const double x = pos < Nao ? ct[k * pad + col] * fk : 0.0;
           const int64_t g = col / NG16, n = col - g * NG16;
           const size_t blk = (size_t)(g * nchunk + ch);
           b64[(blk * AO_MO_KC + kk) * NG16 + n] = x;
@@ -338,10 +412,15 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     a.coord_ld = P;
     a.p0 = 0;
     a.np = P;
+    // tiles of the same electron of consecutive walkers (points close to one another) when
+    // the points are the walkers' electrons; else consecutive points
+    const bool walkers = c->walk_num > 1 && c->walk_num * c->elec_num == P;
+    a.tw = walkers ? c->walk_num : 1;
+    a.te = walkers ? c->elec_num : P;
     a.desc = d.ao_desc.as<double>();
     a.desc_off = d.ao_desc_off.as<int64_t>();
     a.desc_max = d.ao_desc_max;
-    a.ns_max = c->tables.chunk_shells_max;
+    a.ns_max = d.ao_ns_max;
     a.nchunk = d.ao_nchunk;
     a.G = d.ao_mo_g;
     a.ngroup = d.ao_mo_ngroup;

@@ -57,7 +57,7 @@ struct DevConst {
   int ao_mo_g = 1, ao_mo_ngroup = 1;  // column tiles per pass of the fused kernel, passes
   // 32-AO chunk descriptors of the fused kernel (AoMoArgs), offsets in doubles
   DevBuf ao_desc, ao_desc_off;
-  int ao_desc_max = 0, ao_nchunk = 0;
+  int ao_desc_max = 0, ao_nchunk = 0, ao_ns_max = 1;
   DevBuf jas_a, jas_b, jas_cfull, jas_type, jas_kappa_en, jas_asymp_a;
   DevBuf jas_cfull_pad, jas_nuc_pad;  // nucleus tables zero-padded to whole 16-nucleus tiles
   DevBuf prim_nucl_c, prim_expo_c;  // caller-order primitives (primitive VGL getter)
@@ -256,6 +256,10 @@ constexpr int AO_MO_BF_LD = 40;  // bf16 row of a BF16X3 C_t chunk: 32 AOs + 8 p
 struct AoMoArgs {
   const double* coord;
   int64_t coord_ld, p0, np;
+  // the points are taken in the order p' = i tw + w of the point p = w te + i (tw te points:
+  // tw walkers of te electrons), so that a workgroup's 16 points are the same electron of
+  // consecutive walkers; tw = 1 keeps the points' own order.  p0, np select p'.
+  int64_t tw, te;
   const double* desc;
   const int64_t* desc_off;
   int desc_max, ns_max, nchunk;

@@ -3,9 +3,13 @@
 //
 // A workgroup of 4 waves owns TP = 16 points, i.e. the 80 rows (point, component) of the
 // product MO[p][c][:] = sum_k AO[p][c][k] C_t[k][:] (row 5 p + c), and G <= 8 column tiles of
-// 16 MOs (grid.y walks the groups of G tiles).  The AOs are walked in chunks of KC = 32:
+// 16 MOs (grid.y walks the groups of G tiles).  The 16 points are the same electron of 16
+// consecutive walkers (AoMoArgs::tw, te), which the workloads draw around the same nucleus,
+// so that the screening is coherent across the tile.  The AOs are walked in chunks of
+// KC = 32, grouped by the host so that the shells of a chunk vanish at the same places:
 //   1. the radial VGL of every shell of the chunk at the 16 points (FP64, screened exactly as
-//      the reference) into LDS;
+//      the reference) into LDS; when every primitive of every shell is screened at every
+//      point, the chunk's AO values are exact zeros and steps 2 and 3 are skipped;
 //   2. the 16 x 32 AO VGL values of the chunk (5 components each), assembled in FP64, into
 //      an LDS operand tile in the precision of the product;
 //   3. each wave multiplies the tile by its column tiles of C_t on matrix cores.
@@ -74,8 +78,9 @@ __device__ __forceinline__ Chunk chunk_view(const double* d) {
 
 // Radial VGL of shell s of the chunk at displacement (x, y, z) from its nucleus, screened as
 // shell_radial (org/qmckl_ao.org:4769-4868); e^x by the degree-DEG series of exp_neg_deg.
+// Returns whether any primitive passed the screening (else r is exactly zero).
 template <int DEG>
-__device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, double y, double z, double cutoff,
+__device__ __forceinline__ bool chunk_radial(const Chunk& c, int s, double x, double y, double z, double cutoff,
                                              double r[5]) {
   const double code = c.shl[2 * s];
   const int nl = (int)code & 255, np = (int)code >> 8, p0 = (int)c.shl[2 * s + 1];
@@ -83,11 +88,13 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
   // per primitive: e = c e^(-alpha r^2) and t = -2 alpha e; the gradient is (sum t) (x, y, z)
   // and the Laplacian 3 sum t - 2 sum t alpha r^2, applied once per shell
   double v0 = 0, st = 0, st2 = 0;
+  bool live = false;
   if (!(r2 > cutoff * c.nuc[4 * nl + 3])) {
     for (int i = 0; i < np; ++i) {
       const double al = c.prm[2 * (p0 + i)];
       const double ar2 = al * r2;
       if (ar2 > cutoff) continue;
+      live = true;
       const double e = c.prm[2 * (p0 + i) + 1] * exp_neg_deg<DEG>(-ar2);
       const double two_a = -2.0 * al * e;
       v0 += e;
@@ -100,6 +107,7 @@ __device__ __forceinline__ void chunk_radial(const Chunk& c, int s, double x, do
   r[2] = st * y;
   r[3] = st * z;
   r[4] = 3.0 * st - 2.0 * st2;
+  return live;
 }
 
 // AO VGL of an AO with powers (pa, pb, pc) at displacement (x, y, z) from its nucleus, its
@@ -210,17 +218,24 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // LDS: descriptor buffers [2][desc_max] (addresses computed from `lds`, so that every
   // access compiles to an LDS instruction), then the points, radial staging and tile
-  double* xt = lds + 2 * A.desc_max;  // [TP][3]
-  double* sh = xt + 3 * TP;           // [TP][ns_max][SH]
-  void* at = sh + TP * A.ns_max * SH;  // operand tile (TileA<MODE>)
+  double* xt = lds + 2 * A.desc_max;                 // [TP][3]
+  int64_t* pidx = reinterpret_cast<int64_t*>(xt + 3 * TP);  // [TP] point of each tile row
+  double* sh = xt + 4 * TP;                          // [TP][ns_max][SH]
+  void* at = sh + TP * A.ns_max * SH;                // operand tile (TileA<MODE>)
   const int group = blockIdx.y;
-  const int64_t tp0 = A.p0 + (int64_t)blockIdx.x * TP;
+  const int64_t tp0 = A.p0 + (int64_t)blockIdx.x * TP;  // first position p' of the tile
   const int ntp = (int)min((int64_t)TP, A.p0 + A.np - tp0);
   const char* bsrc = static_cast<const char*>(A.B) + (int64_t)group * A.nchunk * A.b_chunk_bytes;
   const bool has1 = wave < NMAIN, has2 = G > 5 && wave + 4 < G;
+  if (tid < TP) {
+    // position p' = i tw + w holds point p = w te + i
+    const int64_t pp = tp0 + tid, i = pp / A.tw;
+    pidx[tid] = tid < ntp ? (pp - i * A.tw) * A.te + i : 0;
+  }
+  __syncthreads();
   for (int t = tid; t < TP * 3; t += 64 * NWAVE) {
     const int q = t / 3, d = t - 3 * q;
-    xt[t] = q < ntp ? A.coord[tp0 + q + d * A.coord_ld] : 0.0;
+    xt[t] = q < ntp ? A.coord[pidx[q] + d * A.coord_ld] : 0.0;
   }
   // column tile `wave` (all row tiles); then column tile wave + 4 (G > 5) or the tiles
   // (wave, 4), (wave + 4, 4) (G = 5)
@@ -238,18 +253,22 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
                8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave, lane);
     const Chunk c = chunk_view(lds + (ch & 1) * A.desc_max);
     // 1. radial VGL of the chunk's shells at the 16 points
+    int live = 0;
     for (int t = tid; t < TP * c.ns; t += 64 * NWAVE) {
       const int q = t / c.ns, s = t - q * c.ns;
       const int nl = (int)c.shl[2 * s] & 255;
       const double x = xt[3 * q] - c.nuc[4 * nl], y = xt[3 * q + 1] - c.nuc[4 * nl + 1],
                    z = xt[3 * q + 2] - c.nuc[4 * nl + 2];
       double r[5];
-      chunk_radial<MODE == 0 ? 13 : 9>(c, s, x, y, z, A.cutoff, r);
+      const bool lv = chunk_radial<MODE == 0 ? 13 : 9>(c, s, x, y, z, A.cutoff, r);
+      if (q < ntp) live |= (int)lv;
       double* o = sh + (q * A.ns_max + s) * SH;
 #pragma unroll
       for (int k = 0; k < 5; ++k) o[k] = r[k];
     }
-    __syncthreads();
+    // every shell of the chunk screened at every point of the tile: all its AO values are
+    // exact zeros and add nothing to the products (rows past the tile's points are zero too)
+    if (!__syncthreads_or(live)) continue;
     // 2. AO VGL of the chunk into the operand tile, row = 5 p + c: a thread keeps AO
     //    kk = tid mod KC (its descriptor decoded once per chunk) for the points
     //    q = tid / KC + QS m
@@ -303,7 +322,8 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int row = 16 * r + (MODE == 0 ? lg + 4 * k : 4 * lg + k);
-      if (row < 5 * ntp) A.C[(tp0 * 5 + row) * A.ldc + col] = v[k];
+      const int q = row / 5;
+      if (row < 5 * ntp) A.C[(pidx[q] * 5 + (row - 5 * q)) * A.ldc + col] = v[k];
     }
   };
   if (has1)
@@ -328,13 +348,20 @@ hipError_t launch_kernel(K kernel, const AoMoArgs& a, dim3 grid, dim3 block, siz
 
 template <int MODE, int G>
 hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
-  static bool attr = false;
-  if (!attr) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ao_mo_tile<MODE, G>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  // dynamic LDS up to the CU's 160 KiB less the kernel's static LDS (the block-wide OR of
+  // the chunk screening uses a little)
+  static size_t max_dyn = 0;
+  if (max_dyn == 0) {
+    const void* k = reinterpret_cast<const void*>(&k_ao_mo_tile<MODE, G>);
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, k);
     if (e != hipSuccess) return e;
-    attr = true;
+    const size_t dyn = 160 * 1024 - fa.sharedSizeBytes;
+    e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (e != hipSuccess) return e;
+    max_dyn = dyn;
   }
+  if (lds > max_dyn) return hipErrorInvalidValue;
   const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
   return launch_kernel(k_ao_mo_tile<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
 }
@@ -357,14 +384,14 @@ hipError_t launch_mode(const AoMoArgs& a, size_t lds, hipStream_t s) {
 }  // namespace
 
 size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
-  return sizeof(double) * (2 * (size_t)a.desc_max + 3 * TP + (size_t)TP * a.ns_max * SH) +
+  return sizeof(double) * (2 * (size_t)a.desc_max + 4 * TP + (size_t)TP * a.ns_max * SH) +
          (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
 }
 
 hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s) {
   if (a.np <= 0) return hipSuccess;
   if (a.G < 1 || a.G > AO_MO_MAX_G || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
-      a.b_chunk_bytes % 16)
+      a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te)
     return hipErrorInvalidValue;
   const size_t lds = ao_mo_lds_bytes(a, mode);
   if (lds > 160 * 1024) return hipErrorInvalidValue;

