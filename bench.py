@@ -325,6 +325,23 @@ def bench_derivatives(args, q, S, L):
         for k, v in step(it).items():
             totals[k] += v
     t = (time.perf_counter() - t0) / args.steps
+
+    # the adjoint: L = sum_w [ln Psi_J + E_L surrogate] and its gradient in a, b, c and the
+    # nuclear coordinates, one call per step after new coordinates (forward + adjoint kernels)
+    from qmckl_amd.autograd import jastrow_adjoint
+
+    def adj_step(it):
+        xx = xs[it % 2]
+        q.check(ctx, L.qmckl_set_electron_coord(ctx, b"T", W, q.ptr(xx), xx.size))
+        return jastrow_adjoint(ctx, 1.0, 1.0)
+
+    for it in range(args.warmup):
+        adj_step(it)
+    t0 = time.perf_counter()
+    for it in range(args.steps):
+        adj_step(it)
+    ta = (time.perf_counter() - t0) / args.steps
+    log(f"adjoint: {ta * 1e3:.3f} ms/step")
     q.context_destroy(ctx)
     out = {
         "metric": "walker-derivative-evals/sec (Jastrow a/b/c parameter derivatives with GL, and forces with GL)",
@@ -334,6 +351,9 @@ def bench_derivatives(args, q, S, L):
         "config": {"workload": f"small: 2 nuclei, {Ne} electrons, Jastrow 5/5/5, {W} walkers, all 12 derivative "
                                f"and force getters per step (host buffers)", "walkers": W},
         "getter_ms_per_step": {k: v / args.steps * 1e3 for k, v in totals.items()},
+        "adjoint": {"value": W / ta, "ms_per_step": ta * 1e3,
+                    "what": "L = sum_w [ln Psi_J + E_L surrogate] and dL/d(a, b, c, R) by the adjoint kernel "
+                            "(qmckl_amd_get_jastrow_champ_adjoint: forward pass, adjoint, walker sum, host copy)"},
     }
     print(json.dumps(out))
 

@@ -426,6 +426,20 @@ qmckl_exit_code qmckl_amd_batch_replay(const int64_t batch);
 qmckl_exit_code qmckl_amd_batch_synchronize(const int64_t batch);
 qmckl_exit_code qmckl_amd_batch_destroy(const int64_t batch);
 
+/* Differentiation (no counterpart in the reference): L = sum_w [weight_j ln Psi_J(w) +
+ * weight_el E_L(w)] over the context's walkers, with the Jastrow-only local-energy surrogate
+ * E_L = -1/2 sum_i (lap_i J + |grad_i J|^2) + V_ee + V_en + V_nn (J = ln Psi_J), and its exact
+ * gradient by a hand-written adjoint kernel: grad_a [type_nucl_num][aord_num+1], grad_b
+ * [bord_num+1], grad_c [type_nucl_num][dim_c_vector] (the layouts of the a, b and c vectors) and
+ * grad_nucl [nucl_num][3] (d/dR of the nuclear coordinates).  Needs 1 <= aord_num, bord_num <= 8.
+ * Host buffers; synchronous. */
+qmckl_exit_code qmckl_amd_get_jastrow_champ_adjoint(qmckl_context context, const double weight_j,
+                                                    const double weight_el, double* const value,
+                                                    double* const grad_a, const int64_t size_a,
+                                                    double* const grad_b, const int64_t size_b,
+                                                    double* const grad_c, const int64_t size_c,
+                                                    double* const grad_nucl, const int64_t size_nucl);
+
 /* No counterpart in the reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The Jastrow value
  * e^J [walk_num] and GL [walk_num][4][elec_num] of walk_num walkers at the device coordinates

@@ -104,6 +104,7 @@ qmckl_exit_code upload_jastrow(Context* c, qmckl_context handle, const char* fn)
   up(d.jas_kappa_en, c->rescale_en.data(), 8 * c->rescale_en.size());
   up(d.jas_asymp_a, c->asymp_jasa.data(), 8 * c->asymp_jasa.size());
   up(d.nucl_coord, c->nucl_coord.data(), 8 * c->nucl_coord.size());
+  up(d.jas_charge, c->nucl_charge.data(), 8 * c->nucl_charge.size());
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   d.jas_date = c->const_date;
   return QMCKL_SUCCESS;
@@ -838,6 +839,84 @@ qmckl_exit_code qmckl_get_jastrow_champ_factor_een_grad(qmckl_context context, d
                                   sizeof(double) * 3 * Ne, W, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(context, e, fn);
+  return QMCKL_SUCCESS;
+}
+
+// Gradient of L = sum_w [wJ ln Psi_J + wL E_L^surr] (E_L^surr = -1/2 sum_i (lap_i J +
+// |grad_i J|^2) + V_ee + V_en + V_nn) in the Jastrow parameters and nuclear coordinates, by
+// the adjoint kernel of kernels_adjoint.hip on the forward pass's factor GLs.  No
+// counterpart in the reference.
+qmckl_exit_code qmckl_amd_get_jastrow_champ_adjoint(qmckl_context context, const double weight_j,
+                                                    const double weight_el, double* const value,
+                                                    double* const grad_a, const int64_t size_a,
+                                                    double* const grad_b, const int64_t size_b,
+                                                    double* const grad_c, const int64_t size_c,
+                                                    double* const grad_nucl, const int64_t size_nucl) {
+  const char* fn = "qmckl_amd_get_jastrow_champ_adjoint";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  const int64_t T = c->type_nucl_num, Nn = c->nucl_num, W = c->walk_num;
+  const int64_t na = T * (c->aord + 1), nb = c->bord + 1, nc = c->cord >= 2 ? T * c->dim_c : 0;
+  if (value == nullptr) return fail(context, QMCKL_INVALID_ARG_4, fn, "NULL pointer");
+  if (grad_a == nullptr || size_a < na) return fail(context, QMCKL_INVALID_ARG_6, fn, "grad_a: type_nucl_num*(aord_num+1)");
+  if (grad_b == nullptr || size_b < nb) return fail(context, QMCKL_INVALID_ARG_8, fn, "grad_b: bord_num+1");
+  if ((nc > 0 && grad_c == nullptr) || size_c < nc)
+    return fail(context, QMCKL_INVALID_ARG_10, fn, "grad_c: type_nucl_num*dim_c_vector");
+  if (grad_nucl == nullptr || size_nucl < 3 * Nn) return fail(context, QMCKL_INVALID_ARG_12, fn, "grad_nucl: 3*nucl_num");
+  if (c->aord < 1 || c->bord < 1 || c->aord > 8 || c->bord > 8)
+    return fail(context, QMCKL_FAILURE, fn, "the adjoint needs 1 <= aord_num, bord_num <= 8");
+  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  AdjArgs a;
+  a.v = jas_view(c);
+  a.ntype = (int)T;
+  a.fee = c->d_fee.as<double>();
+  a.fen = c->d_fen.as<double>();
+  a.feen = c->d_feen.as<double>();
+  a.fee_gl = c->d_fee_gl.as<double>();
+  a.fen_gl = c->d_fen_gl.as<double>();
+  a.feen_gl = c->d_feen_gl.as<double>();
+  a.charge = c->dc.jas_charge.as<double>();
+  a.wJ = weight_j;
+  a.wL = weight_el;
+  a.nout = adjoint_outputs(a.v, a.ntype);
+  hipError_t e = c->d_deriv[DB_PART].reserve(sizeof(double) * W * a.nout);
+  if (e == hipSuccess) e = c->d_deriv[DB_OUT].reserve(sizeof(double) * a.nout);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  c->deriv_date[DB_PART] = c->deriv_date[DB_OUT] = 0;  // no longer the derivative getters' results
+  a.out = c->d_deriv[DB_PART].as<double>();
+  e = launch_jastrow_adjoint(a, c->stream);
+  if (e == hipSuccess)
+    e = launch_sum_walkers(a.out, W, a.nout, 1.0, c->d_deriv[DB_OUT].as<double>(), c->stream);
+  std::vector<double> h(a.nout);
+  if (e == hipSuccess) e = copy_to_host(c, h.data(), c->d_deriv[DB_OUT].ptr, sizeof(double) * a.nout);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
+  // V_nn = sum_{a<b} Z_a Z_b / R_ab (org/qmckl_nucleus.org:1040-1076), once per walker
+  double vnn = 0.0;
+  std::vector<double> dvnn(3 * Nn, 0.0);
+  for (int64_t p = 0; p < Nn; ++p)
+    for (int64_t q = 0; q < p; ++q) {
+      double d[3], r2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        d[k] = c->nucl_coord[k * Nn + p] - c->nucl_coord[k * Nn + q];
+        r2 += d[k] * d[k];
+      }
+      const double r = std::sqrt(r2), zz = c->nucl_charge[p] * c->nucl_charge[q];
+      vnn += zz / r;
+      for (int k = 0; k < 3; ++k) {
+        dvnn[3 * p + k] -= zz * d[k] / (r2 * r);
+        dvnn[3 * q + k] += zz * d[k] / (r2 * r);
+      }
+    }
+  const double wn = weight_el * (double)W;
+  *value = h[0] + wn * vnn;
+  int64_t o = 1;
+  for (int64_t k = 0; k < na; ++k) grad_a[k] = h[o++];
+  for (int64_t k = 0; k < nb; ++k) grad_b[k] = h[o++];
+  for (int64_t k = 0; k < nc; ++k) grad_c[k] = h[o++];
+  for (int64_t k = 0; k < 3 * Nn; ++k) grad_nucl[k] = h[o++] + wn * dvnn[k];
   return QMCKL_SUCCESS;
 }
 

@@ -60,6 +60,7 @@ struct DevConst {
   int ao_desc_max = 0, ao_nchunk = 0, ao_ns_max = 1;
   DevBuf jas_a, jas_b, jas_cfull, jas_type, jas_kappa_en, jas_asymp_a;
   DevBuf jas_cfull_pad, jas_nuc_pad;  // nucleus tables zero-padded to whole 16-nucleus tiles
+  DevBuf jas_charge;                  // nuclear charges (adjoint: V_en)
   DevBuf prim_nucl_c, prim_expo_c;  // caller-order primitives (primitive VGL getter)
   uint64_t date = 0;      // context date of the basis upload
   uint64_t jas_date = 0;  // context date of the Jastrow upload
@@ -305,6 +306,23 @@ bool jastrow_view_resident(Context* c, int64_t walk_num, const double* coord, Ja
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_t s);
+// Gradient of L = sum_w [wJ J_w + wL E_w] (E_w = -1/2 sum_i (lap_i J + |grad_i J|^2) + V_ee
+// + V_en; V_nn is added by the caller) in the Jastrow parameters and nuclear coordinates,
+// per walker into out [W][nout]: [0] the walker's value, then d/da [type][aord+1], d/db
+// [bord+1], d/dc [type][dim_c], d/dR [nucl][3] (kernels_adjoint.hip).  The factors and
+// factor GLs of the same coordinates must be current.
+struct AdjArgs {
+  JasDevView v;
+  int ntype;
+  const double *fee, *fen, *feen;           // [W]
+  const double *fee_gl, *fen_gl, *feen_gl;  // [W][4][Ne]
+  const double* charge;                     // [nucl]
+  double wJ, wL;
+  double* out;
+  int nout;
+};
+int adjoint_outputs(const JasDevView& v, int ntype);
+hipError_t launch_jastrow_adjoint(const AdjArgs& a, hipStream_t s);
 // Intermediates with the reference layouts; kind selects which one.
 enum JasInter {
   J_EE_RESCALED = 0, J_EE_RESCALED_GL, J_EN_RESCALED, J_EN_RESCALED_GL,
