@@ -25,6 +25,7 @@
 #define QMCKL_AMD_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -364,6 +365,53 @@ qmckl_exit_code qmckl_get_forces_jastrow_een_g(qmckl_context context, double* co
                                                const int64_t size_max);
 qmckl_exit_code qmckl_get_forces_jastrow_een_l(qmckl_context context, double* const forces_jastrow_een_l,
                                                const int64_t size_max);
+
+/* ---- Single-electron moves of the CHAMP Jastrow: org/qmckl_jastrow_champ_single.org:452,
+ * 589, 629, 904, 1240, 1572, 1868, 2741, 3058, 3682, 4350, 5219, 5513, 5881, 6229, 7088, 7381,
+ * 7754, 8081, 8906.  qmckl_set_single_point takes the new position of electron num (0-based)
+ * for every walker, [walk_num][3] ('N') or [3][walk_num] ('T'); the getters return, for that
+ * move, the distances and rescaled distances of the new position and the changes of the e-e,
+ * e-n and e-e-n factors and of their gradients and Laplacians (delta_ee_gl, delta_en_gl
+ * [walk_num][elec_num][4]; delta_een_gl [walk_num][4][elec_num]; delta_een_g the moved
+ * electron's gradient alone).  qmckl_get_jastrow_champ_single_accept moves the electron. */
+qmckl_exit_code qmckl_set_single_point(qmckl_context context, const char transp, const int64_t num,
+                                       const double* coord, const int64_t size_max);
+qmckl_exit_code qmckl_single_touch(const qmckl_context context);
+qmckl_exit_code qmckl_get_single_electron_ee_distance(qmckl_context context, double* const distance,
+                                                      const int64_t size_max);
+qmckl_exit_code qmckl_get_single_electron_en_distance(qmckl_context context, double* distance,
+                                                      const int64_t size_max);
+qmckl_exit_code qmckl_get_een_rescaled_single_e(qmckl_context context, double* const distance_rescaled,
+                                                const int64_t size_max);
+qmckl_exit_code qmckl_get_een_rescaled_single_n(qmckl_context context, double* const distance_rescaled,
+                                                const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_een(qmckl_context context, double* const delta_een,
+                                                   const int64_t size_max);
+qmckl_exit_code qmckl_get_een_rescaled_single_n_gl(qmckl_context context, double* const distance_rescaled,
+                                                   const int64_t size_max);
+qmckl_exit_code qmckl_get_een_rescaled_single_e_gl(qmckl_context context, double* const distance_rescaled,
+                                                   const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_een_gl(qmckl_context context, double* const delta_een_gl,
+                                                      const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_een_g(qmckl_context context, double* const delta_een_g,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_get_ee_rescaled_single(qmckl_context context, double* const distance_rescaled,
+                                             const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_ee(qmckl_context context, double* const delta_ee,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_ee_rescaled_single_gl(qmckl_context context, double* const distance_rescaled_gl,
+                                                const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_ee_gl(qmckl_context context, double* const delta_ee_gl,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_get_en_rescaled_single(qmckl_context context, double* const distance_rescaled,
+                                             const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_en(qmckl_context context, double* const delta_en,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_en_rescaled_single_gl(qmckl_context context, double* distance_rescaled_gl,
+                                                const size_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_en_gl(qmckl_context context, double* const delta_en_gl,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_get_jastrow_champ_single_accept(qmckl_context context);
 
 /* ======================================================================================
  * Extensions without a counterpart in the reference (device residency, streams, MO GEMM

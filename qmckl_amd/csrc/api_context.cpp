@@ -202,7 +202,8 @@ std::vector<DevBuf*> device_buffers(Context* c) {
                     &c->d_coord, &c->d_ao_vgl, &c->d_ao_work, &c->d_mo_vgl, &c->d_ee_dist,
                     &c->d_en_dist, &c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl,
                     &c->d_feen_gl, &c->d_jvalue, &c->d_jgl, &c->d_jscratch, &c->d_jint,
-                    &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c, &d.jas_charge};
+                    &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c, &d.jas_charge,
+                                &c->d_single_coord, &c->d_single_out};
   for (DevBuf& b : c->d_deriv) bufs.push_back(&b);
   return bufs;
 }

@@ -414,6 +414,29 @@ qmckl_exit_code get_deriv(qmckl_context context, int what, double* out, int64_t 
 
 namespace qmckl_amd {
 
+// The Jastrow view of c's walkers, its device tables and coordinates uploaded.
+qmckl_exit_code jastrow_view_ready(Context* c, qmckl_context handle, const char* fn, JasDevView& v) {
+  qmckl_exit_code rc = jas_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  v = jas_view(c);
+  return QMCKL_SUCCESS;
+}
+
+// The device buffers of c's factors, GLs and values, as the Jastrow kernels take them.
+JasOut jastrow_out(Context* c) {
+  JasOut o;
+  o.fee = c->d_fee.as<double>();
+  o.fen = c->d_fen.as<double>();
+  o.feen = c->d_feen.as<double>();
+  o.fee_gl = c->d_fee_gl.as<double>();
+  o.fen_gl = c->d_fen_gl.as<double>();
+  o.feen_gl = c->d_feen_gl.as<double>();
+  o.value = c->d_jvalue.as<double>();
+  o.gl = c->d_jgl.as<double>();
+  o.scratch = c->d_jscratch.as<double>();
+  return o;
+}
+
 bool jastrow_view_resident(Context* c, int64_t walk_num, const double* coord, JasDevView& v) {
   if (c->stream == nullptr || c->device < 0 || c->dc.jas_date < c->const_date) return false;
   v = jas_view(c);

@@ -170,6 +170,13 @@ struct Context {
   uint64_t jint_date = 0;
   int jint_kind = -1;
 
+  // single-electron move, org/qmckl_jastrow_champ_single.org:440-586
+  int64_t single_num = -1;
+  std::vector<double> single_coord;  // [walk_num][3]
+  uint64_t single_date = 0;          // bumped by qmckl_set_single_point and qmckl_single_touch
+  DevBuf d_single_coord, d_single_out;
+  uint64_t single_out_date = 0, single_out_ctx_date = 0;  // single_date and date of d_single_out
+
   // kernel timing (profiling extension)
   bool profiling = false;
   std::vector<ProfEvent> prof_pending;
@@ -302,6 +309,10 @@ int64_t jastrow_scratch_doubles(const JasDevView& v);
 // [3][walk_num * elec_num], on the device tables of c; false (and nothing uploaded or
 // allocated) when those tables are not on the device or older than c's constant data.
 bool jastrow_view_resident(Context* c, int64_t walk_num, const double* coord, JasDevView& v);
+// The Jastrow view of c's walkers after uploading its tables and coordinates.
+qmckl_exit_code jastrow_view_ready(Context* c, qmckl_context handle, const char* fn, JasDevView& v);
+// c's device factor, GL and value buffers (valid after provide_jastrow).
+JasOut jastrow_out(Context* c);
 // The three kernels of one evaluation, in this order (same stream).
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s);
@@ -323,6 +334,30 @@ struct AdjArgs {
 };
 int adjoint_outputs(const JasDevView& v, int ntype);
 hipError_t launch_jastrow_adjoint(const AdjArgs& a, hipStream_t s);
+// Single-electron moves (kernels_single.hip): outputs for the move of electron `num` of
+// every walker to coord [W][3] (device), in the layouts of the reference's getters
+// (org/qmckl_jastrow_champ_single.org): distances and rescaled distances of the new
+// position (ee [W][Ne], en [W][Nn], their GLs [W][Ne][4], [W][Nn][4]), een_rescaled
+// single e [W][C+1][Ne] / e_gl [W][C+1][4][Ne] / n [W][C+1][Nn] / n_gl [W][C+1][Nn][4], and
+// the changes of the factors [W] and of their GLs (ee, en: [W][Ne][4]; een: [W][4][Ne]; een_g:
+// the moved electron's gradient alone, [W][4][Ne]).
+struct SingleOut {
+  double *ee_dist, *en_dist, *ee_resc, *ee_resc_gl, *en_resc, *en_resc_gl;
+  double *een_e, *een_e_gl, *een_n, *een_n_gl;
+  double *d_ee, *d_en, *d_een, *d_ee_gl, *d_en_gl, *d_een_gl, *d_een_g;
+};
+struct SingleArgs {
+  int64_t num;
+  const double* coord;  // [W][3]
+  SingleOut out;
+};
+int64_t single_out_doubles(const JasDevView& v);
+SingleOut single_out_view(const JasDevView& v, double* base);
+hipError_t launch_jastrow_single(const JasDevView& v, const SingleArgs& s, hipStream_t st);
+// Moves electron num of every walker in coord [3][W*Ne] and adds the changes to the
+// factors and factor GLs of o, then forms its value and GL again.
+hipError_t launch_jastrow_single_accept(const JasDevView& v, const SingleArgs& s, const JasOut& o, double* coord,
+                                        hipStream_t st);
 // Intermediates with the reference layouts; kind selects which one.
 enum JasInter {
   J_EE_RESCALED = 0, J_EE_RESCALED_GL, J_EN_RESCALED, J_EN_RESCALED_GL,
