@@ -413,6 +413,14 @@ qmckl_exit_code qmckl_get_jastrow_champ_single_en_gl(qmckl_context context, doub
                                                      const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_single_accept(qmckl_context context);
 
+/* ---- TREXIO input (org/qmckl_trexio.org:1104-1190) ---------------------------------------
+ * Sets the electron numbers, nuclei, AO basis and MO coefficients of `context` from the TREXIO
+ * file `file_name` (at most size_max characters).  The TEXT back end (a directory of
+ * <group>.txt files) is read without libtrexio; any other path fails with QMCKL_INVALID_ARG_2,
+ * as the reference does when no back end opens the file.  Inconsistent index data return
+ * QMCKL_FAILURE.  An invalid context returns QMCKL_INVALID_CONTEXT (the reference returns 0). */
+qmckl_exit_code qmckl_trexio_read(const qmckl_context context, const char* file_name, const int64_t size_max);
+
 /* ======================================================================================
  * Extensions without a counterpart in the reference (device residency, streams, MO GEMM
  * precision).  Device pointers are HIP device addresses on the context's device.

@@ -116,6 +116,7 @@ def check(ctx, rc, fn="qmckl"):
 # ---- SWIG-style layer --------------------------------------------------------------
 
 QMCKL_SUCCESS = 0
+QMCKL_FAILURE = 101
 QMCKL_INVALID_CONTEXT = 103
 QMCKL_NOT_PROVIDED = 106
 QMCKL_ALREADY_SET = 108
@@ -169,6 +170,13 @@ def get_scalar(ctx, name, ctype=ctypes.c_int64):
 
 def synchronize(ctx):
     _call(ctx, "qmckl_amd_synchronize")
+
+
+def trexio_read(ctx, file_name):
+    """qmckl_trexio_read (org/qmckl_trexio.org:1104): electron numbers, nuclei, AO basis and MO
+    coefficients from a TREXIO file of the TEXT back end (a directory of <group>.txt files)."""
+    name = os.fsencode(file_name)
+    _call(ctx, "qmckl_trexio_read", name, len(name))
 
 
 # ---- device buffers over the HIP runtime the library itself uses --------------------

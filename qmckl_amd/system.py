@@ -176,6 +176,35 @@ def load(ctx, s: System, jastrow=True, mo=True):
         c(L.qmckl_set_jastrow_champ_rescale_factor_ee(ctx, float(s.rescale_ee)), "rescale_factor_ee")
 
 
+def from_context(ctx, name=""):
+    """The System held by a context whose nucleus, electron, AO-basis and MO-basis groups are set
+    (for instance by qmckl_trexio_read), read back through the getters.  The Jastrow fields are
+    zero parameters of orders 5/5/5 over one nucleus type."""
+    from . import get_array, get_scalar
+
+    i64 = lambda fn, n: get_array(ctx, "qmckl_get_" + fn, n, np.int64)
+    f64 = lambda fn, n: get_array(ctx, "qmckl_get_" + fn, n)
+    Nn = get_scalar(ctx, "qmckl_get_nucleus_num")
+    shell = get_scalar(ctx, "qmckl_get_ao_basis_shell_num")
+    prim = get_scalar(ctx, "qmckl_get_ao_basis_prim_num")
+    ao = get_scalar(ctx, "qmckl_get_ao_basis_ao_num")
+    mo = get_scalar(ctx, "qmckl_get_mo_basis_mo_num")
+    coord = np.empty(3 * Nn)
+    check(ctx, lib().qmckl_get_nucleus_coord(ctx, b"T", ptr(coord), coord.size), "get_nucleus_coord")
+    return System(
+        nucl_charge=f64("nucleus_charge", Nn), nucl_coord=coord.reshape(3, Nn),
+        up=get_scalar(ctx, "qmckl_get_electron_up_num"), dn=get_scalar(ctx, "qmckl_get_electron_down_num"),
+        shell_ang_mom=get_array(ctx, "qmckl_get_ao_basis_shell_ang_mom", shell, np.int32),
+        shell_prim_num=i64("ao_basis_shell_prim_num", shell), shell_prim_index=i64("ao_basis_shell_prim_index", shell),
+        nucleus_index=i64("ao_basis_nucleus_index", Nn), nucleus_shell_num=i64("ao_basis_nucleus_shell_num", Nn),
+        shell_factor=f64("ao_basis_shell_factor", shell), exponent=f64("ao_basis_exponent", prim),
+        coefficient=f64("ao_basis_coefficient", prim), prim_factor=f64("ao_basis_prim_factor", prim),
+        ao_factor=f64("ao_basis_ao_factor", ao), mo_coef=f64("mo_basis_coefficient", mo * ao).reshape(mo, ao),
+        r_power=get_array(ctx, "qmckl_get_ao_basis_r_power", shell, np.int32),
+        type_nucl_vector=np.zeros(Nn, np.int64), a_vector=np.zeros((1, 6)), b_vector=np.zeros(6),
+        c_vector=np.zeros((1, dim_c_vector(5))), rescale_en=np.ones(1), name=name)
+
+
 def set_coords(ctx, s: System, coord_T, walk_num):
     """qmckl_set_electron_coord with coordinates given as [3][walk_num*elec_num]."""
     a = np.ascontiguousarray(coord_T, dtype=np.float64)
