@@ -421,6 +421,85 @@ qmckl_exit_code qmckl_get_jastrow_champ_single_accept(qmckl_context context);
  * QMCKL_FAILURE.  An invalid context returns QMCKL_INVALID_CONTEXT (the reference returns 0). */
 qmckl_exit_code qmckl_trexio_read(const qmckl_context context, const char* file_name, const int64_t size_max);
 
+/* ---- Slater determinants (org/under_development/qmckl_determinant.org:344-556, 569-640,
+ * 1308-1495).  The reference does not build this module; names and argument lists are its own.
+ * mo_index_alpha/beta: [det_num][walk_num][up_num / down_num], 1-based MO indices, set after
+ * the walkers (qmckl_set_electron_coord).  Getters without size_max write (like the reference)
+ * det_vgl [det][walk][5][n][n] (component c of MO mo_index[d][w][m] - 1 at electron e, [e][m]
+ * inner), det_inv_matrix and det_adj_matrix [det][walk][n][n] (the inverse of that value
+ * matrix, NaN when it is singular, and det times it), det [det][walk]. */
+qmckl_exit_code qmckl_set_determinant_type(const qmckl_context context, const char t);
+qmckl_exit_code qmckl_set_determinant_det_num_alpha(const qmckl_context context, const int64_t det_num_alpha);
+qmckl_exit_code qmckl_set_determinant_det_num_beta(const qmckl_context context, const int64_t det_num_beta);
+qmckl_exit_code qmckl_set_determinant_mo_index_alpha(const qmckl_context context, const int64_t* mo_index_alpha,
+                                                     const int64_t size_max);
+qmckl_exit_code qmckl_set_determinant_mo_index_beta(const qmckl_context context, const int64_t* mo_index_beta,
+                                                    const int64_t size_max);
+bool qmckl_determinant_provided(const qmckl_context context);
+qmckl_exit_code qmckl_get_det_vgl_alpha(qmckl_context context, double* const det_vgl_alpha);
+qmckl_exit_code qmckl_get_det_vgl_beta(qmckl_context context, double* const det_vgl_beta);
+qmckl_exit_code qmckl_get_det_inv_matrix_alpha(qmckl_context context, double* const det_inv_matrix_alpha);
+qmckl_exit_code qmckl_get_det_inv_matrix_beta(qmckl_context context, double* const det_inv_matrix_beta);
+qmckl_exit_code qmckl_get_det_adj_matrix_alpha(qmckl_context context, double* const det_adj_matrix_alpha);
+qmckl_exit_code qmckl_get_det_adj_matrix_beta(qmckl_context context, double* const det_adj_matrix_beta);
+qmckl_exit_code qmckl_get_det_alpha(qmckl_context context, double* const det_value_alpha);
+qmckl_exit_code qmckl_get_det_beta(qmckl_context context, double* const det_value_beta);
+
+/* ---- Local energy of one determinant per spin (org/under_development/qmckl_local_energy.org:
+ * 203-1713): kinetic -1/2 sum_e lap_e D / D, potential V_ee + V_en + V_nn, local energy their
+ * sum, all [walk_num]; drift vector 2 grad_e D / D [walk_num][elec_num][3].  QMCKL_FAILURE
+ * when det_num_alpha or det_num_beta is not 1. */
+bool qmckl_local_energy_provided(const qmckl_context context);
+qmckl_exit_code qmckl_get_kinetic_energy(qmckl_context context, double* const kinetic_energy);
+qmckl_exit_code qmckl_get_potential_energy(qmckl_context context, double* const potential_energy);
+qmckl_exit_code qmckl_get_local_energy(qmckl_context context, double* const local_energy, const int64_t size_max);
+qmckl_exit_code qmckl_get_drift_vector(qmckl_context context, double* const drift_vector);
+
+/* ---- Sherman-Morrison-Woodbury (org/under_development/qmckl_sherman_morrison_woodbury.org:
+ * 299-347, 980-1036, 1623-1668, 2269-2314, 2896-2945).  Slater_inv [Dim][LDS]; Updates
+ * [N_updates][LDS]; Updates_index 1-based: update l adds Updates[l] to column
+ * Updates_index[l] - 1 of the Slater matrix.  determinant may be NULL.  A breakdown (a
+ * denominator or 2x2 / 3x3 determinant below `breakdown`) returns QMCKL_FAILURE with
+ * Slater_inv and *determinant unchanged.  The _doc names are the same computation.  Each call
+ * runs on the context's GPU; qmckl_amd_sm_update_device batches many inverses. */
+qmckl_exit_code qmckl_sm_naive(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                               const uint64_t N_updates, const double* Updates, const uint64_t* Updates_index,
+                               const double breakdown, double* Slater_inv, double* determinant);
+qmckl_exit_code qmckl_sm_naive_doc(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                   const uint64_t N_updates, const double* Updates, const uint64_t* Updates_index,
+                                   const double breakdown, double* Slater_inv, double* determinant);
+qmckl_exit_code qmckl_sm_splitting(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                   const uint64_t N_updates, const double* Updates, const uint64_t* Updates_index,
+                                   const double breakdown, double* Slater_inv, double* determinant);
+qmckl_exit_code qmckl_sm_splitting_doc(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                       const uint64_t N_updates, const double* Updates,
+                                       const uint64_t* Updates_index, const double breakdown, double* Slater_inv,
+                                       double* determinant);
+/* The halves of the updates that break down are appended to later_updates [.][LDS] and
+ * later_index from position *later on, and *later is advanced. */
+qmckl_exit_code qmckl_sm_splitting_core(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                        const uint64_t N_updates, const double* Updates,
+                                        const uint64_t* Updates_index, const double breakdown, double* Slater_inv,
+                                        double* later_updates, uint64_t* later_index, uint64_t* later,
+                                        double* determinant);
+qmckl_exit_code qmckl_sm_splitting_core_doc(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                            const uint64_t N_updates, const double* Updates,
+                                            const uint64_t* Updates_index, const double breakdown,
+                                            double* Slater_inv, double* later_updates, uint64_t* later_index,
+                                            uint64_t* later, double* determinant);
+qmckl_exit_code qmckl_woodbury_2x2(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                   const double* Updates, const uint64_t* Updates_index, const double breakdown,
+                                   double* Slater_inv, double* determinant);
+qmckl_exit_code qmckl_woodbury_2x2_doc(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                       const double* Updates, const uint64_t* Updates_index, const double breakdown,
+                                       double* Slater_inv, double* determinant);
+qmckl_exit_code qmckl_woodbury_3x3(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                   const double* Updates, const uint64_t* Updates_index, const double breakdown,
+                                   double* Slater_inv, double* determinant);
+qmckl_exit_code qmckl_woodbury_3x3_doc(const qmckl_context context, const uint64_t LDS, const uint64_t Dim,
+                                       const double* Updates, const uint64_t* Updates_index, const double breakdown,
+                                       double* Slater_inv, double* determinant);
+
 /* ======================================================================================
  * Extensions without a counterpart in the reference (device residency, streams, MO GEMM
  * precision).  Device pointers are HIP device addresses on the context's device.
@@ -521,6 +600,22 @@ qmckl_exit_code qmckl_amd_get_jastrow_champ_device_noalloc(qmckl_context context
 qmckl_exit_code qmckl_amd_set_profiling(qmckl_context context, const int32_t on);
 qmckl_exit_code qmckl_amd_get_kernel_times(qmckl_context context, double* const ms, int64_t* const counts,
                                            const int64_t size_max);
+
+/* Batched Sherman-Morrison-Woodbury on n_mat device-resident inverses (no counterpart in the
+ * reference: its kernels take one matrix).  d_slater_inv [n_mat][Dim][LDS], d_updates
+ * [n_mat][N][LDS], d_updates_index [n_mat][N] (1-based), d_determinant [n_mat] (or NULL),
+ * d_status [n_mat]: 0 applied, 1 breakdown, 2 an index outside 1..Dim; a matrix and its
+ * determinant are written only when its status is 0.  N is 2 or 3 for the Woodbury kinds.
+ * Ordered on the context's stream: call qmckl_amd_synchronize before reading the results. */
+#define QMCKL_AMD_SM_NAIVE ((int32_t)0)
+#define QMCKL_AMD_SM_SPLITTING ((int32_t)1)
+#define QMCKL_AMD_WOODBURY_2X2 ((int32_t)3)
+#define QMCKL_AMD_WOODBURY_3X3 ((int32_t)4)
+qmckl_exit_code qmckl_amd_sm_update_device(qmckl_context context, const int32_t kind, const int64_t n_mat,
+                                           const uint64_t LDS, const uint64_t Dim, const uint64_t N_updates,
+                                           const double* d_updates, const uint64_t* d_updates_index,
+                                           const double breakdown, double* d_slater_inv, double* d_determinant,
+                                           int32_t* d_status);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ _CTYPES = {
     "qmckl_context": ctypes.c_int64,
     "qmckl_exit_code": ctypes.c_int32,
     "int64_t": ctypes.c_int64,
+    "uint64_t": ctypes.c_uint64,
     "size_t": ctypes.c_size_t,
     "int32_t": ctypes.c_int32,
     "int": ctypes.c_int,

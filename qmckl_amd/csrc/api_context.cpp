@@ -203,7 +203,9 @@ std::vector<DevBuf*> device_buffers(Context* c) {
                     &c->d_en_dist, &c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl,
                     &c->d_feen_gl, &c->d_jvalue, &c->d_jgl, &c->d_jscratch, &c->d_jint,
                     &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c, &d.jas_charge,
-                                &c->d_single_coord, &c->d_single_out};
+                                &c->d_single_coord, &c->d_single_out, &c->d_det_idx, &c->d_det_value,
+                                &c->d_det_inv, &c->d_det_work, &c->d_det_out, &c->d_det_drift,
+                                &c->d_det_kin, &c->d_energy, &c->d_pot_charge, &c->d_smw};
   for (DevBuf& b : c->d_deriv) bufs.push_back(&b);
   return bufs;
 }
@@ -283,7 +285,12 @@ qmckl_context qmckl_context_copy(const qmckl_context context) {
   CP(aord); CP(bord); CP(cord); CP(dim_c); CP(type_nucl_num); CP(spin_independent);
   CP(type_nucl_vector); CP(a_vector); CP(b_vector); CP(c_vector); CP(rescale_en); CP(rescale_ee);
   CP(jas_uninit); CP(jas_provided); CP(asymp_jasa);
+  CP(single_num); CP(single_coord); CP(single_date);
+  CP(det_type); CP(det_num_alpha); CP(det_num_beta); CP(mo_index_alpha); CP(mo_index_beta);
+  CP(det_uninit); CP(det_set_date);
 #undef CP
+  n->mo_index_walk[0] = c->mo_index_walk[0];
+  n->mo_index_walk[1] = c->mo_index_walk[1];
   n->asymp_jasb[0] = c->asymp_jasb[0];
   n->asymp_jasb[1] = c->asymp_jasb[1];
   if (c->point_on_device && c->point_num > 0) {
@@ -831,52 +838,29 @@ qmckl_exit_code qmckl_get_electron_en_distance(qmckl_context context, double* co
   return copy_out(c, context, c->d_en_dist.ptr, distance, n, fn);
 }
 
-// org/qmckl_electron.org:1022-1192: V_ee[walk] = sum_{i<j, r_ij > 1e-5} 1/r_ij (from the device distances)
-qmckl_exit_code qmckl_get_electron_ee_potential(qmckl_context context, double* const ee_potential) {
-  const char* fn = "qmckl_get_electron_ee_potential";
+// org/qmckl_electron.org:1022-1192: V_ee[walk] = sum_{i<j, r_ij > 1e-5} 1/r_ij, and
+// org/qmckl_electron.org:1477-1658: V_en[walk] = -sum_{i,a, r_ia > 1e-6} Z_a / r_ia
+// (coincident particles left out, :1172, :1638), one workgroup per walker on the device.
+static qmckl_exit_code get_potential(qmckl_context context, double* out, int which, const char* fn) {
   Context* c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT;
-  if (ee_potential == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "null pointer");
+  if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "null pointer");
   std::lock_guard<std::recursive_mutex> g(c->mutex);
-  if (!c->elec_provided || c->walk_num <= 0) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
-  const int64_t W = c->walk_num, Ne = c->elec_num;
-  std::vector<double> d(W * Ne * Ne);
-  qmckl_exit_code rc = qmckl_get_electron_ee_distance(context, d.data(), (int64_t)d.size());
+  if (!c->elec_provided || c->walk_num <= 0 || (which == 4 && !c->nucl_provided))
+    return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  qmckl_exit_code rc = provide_energy(c, context, fn, false);
   if (rc != QMCKL_SUCCESS) return rc;
-  for (int64_t w = 0; w < W; ++w) {
-    double e = 0.0;
-    for (int64_t j = 0; j < Ne; ++j)
-      for (int64_t i = 0; i < j; ++i) {
-        const double r = d[(w * Ne + j) * Ne + i];
-        if (std::fabs(r) > 1e-5) e += 1.0 / r;  // coincident electrons are left out (:1172)
-      }
-    ee_potential[w] = e;
-  }
+  const hipError_t e = copy_to_host(c, out, c->d_energy.as<double>() + which * c->walk_num, sizeof(double) * c->walk_num);
+  if (e != hipSuccess) return hip_fail(context, e, fn);
   return QMCKL_SUCCESS;
 }
 
-// org/qmckl_electron.org:1477-1658: V_en[walk] = -sum_{i,a, r_ia > 1e-6} Z_a / r_ia
+qmckl_exit_code qmckl_get_electron_ee_potential(qmckl_context context, double* const ee_potential) {
+  return get_potential(context, ee_potential, 3, "qmckl_get_electron_ee_potential");
+}
+
 qmckl_exit_code qmckl_get_electron_en_potential(qmckl_context context, double* const en_potential) {
-  const char* fn = "qmckl_get_electron_en_potential";
-  Context* c = ctx_of(context);
-  if (!c) return QMCKL_INVALID_CONTEXT;
-  if (en_potential == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "null pointer");
-  std::lock_guard<std::recursive_mutex> g(c->mutex);
-  if (!c->elec_provided || c->walk_num <= 0 || !c->nucl_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
-  const int64_t W = c->walk_num, Ne = c->elec_num, Nn = c->nucl_num;
-  std::vector<double> d(W * Ne * Nn);
-  qmckl_exit_code rc = qmckl_get_electron_en_distance(context, d.data(), (int64_t)d.size());
-  if (rc != QMCKL_SUCCESS) return rc;
-  for (int64_t w = 0; w < W; ++w) {
-    double e = 0.0;
-    for (int64_t i = 0; i < Ne; ++i)
-      for (int64_t a = 0; a < Nn; ++a) {
-        const double r = d[(w * Ne + i) * Nn + a];
-        if (std::fabs(r) > 1e-6) e -= c->nucl_charge[a] / r;  // an electron on a nucleus is left out (:1638)
-      }
-    en_potential[w] = e;
-  }
-  return QMCKL_SUCCESS;
+  return get_potential(context, en_potential, 4, "qmckl_get_electron_en_potential");
 }
 
 /* ------------------------------------------------------------------ device extensions */

@@ -177,6 +177,22 @@ struct Context {
   DevBuf d_single_coord, d_single_out;
   uint64_t single_out_date = 0, single_out_ctx_date = 0;  // single_date and date of d_single_out
 
+  // Slater determinants, org/under_development/qmckl_determinant.org:115-556
+  char det_type = 0;
+  int64_t det_num_alpha = 0, det_num_beta = 0;
+  std::vector<int64_t> mo_index_alpha, mo_index_beta;  // [det][walk][n], 1-based MO indices
+  int64_t mo_index_walk[2] = {0, 0};                   // walk_num each index array was set for
+  uint32_t det_uninit = 0x1F;
+  uint64_t det_set_date = 1;  // bumped by the determinant setters
+  DevBuf d_det_idx;           // int32 0-based indices, alpha then beta
+  uint64_t det_idx_date = 0, det_idx_const = 0;  // det_set_date and const_date of d_det_idx
+  DevBuf d_det_value, d_det_inv, d_det_work, d_det_out;  // values, inverses, work, getter staging
+  DevBuf d_det_drift, d_det_kin, d_energy;               // [W][Ne][3], [2][W], [5][W]
+  uint64_t det_date = 0, det_seen = 0, energy_date = 0;
+  DevBuf d_pot_charge;
+  uint64_t pot_charge_date = 0;
+  DevBuf d_smw;  // Sherman-Morrison-Woodbury staging and scratch
+
   // kernel timing (profiling extension)
   bool profiling = false;
   std::vector<ProfEvent> prof_pending;
@@ -220,6 +236,9 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
 qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_ee_distance(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_en_distance(qmckl_context handle, Context* c, const char* fn);
+// e_kin, e_pot, e_local, V_ee, V_en per walker in c->d_energy ([5][W]); kinetic: with the
+// Slater determinants (api_det.cpp)
+qmckl_exit_code provide_energy(Context* c, qmckl_context handle, const char* fn, bool kinetic);
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------
 struct AoDevView {
@@ -387,6 +406,59 @@ struct DerivArgs {
 hipError_t launch_deriv(const DerivArgs& d, int what, double* out, hipStream_t s);
 int64_t deriv_size(const DerivArgs& d, int what);
 hipError_t launch_sum_walkers(const double* in, int64_t W, int64_t n, double scale, double* out, hipStream_t s);
+
+// ---- Slater determinants, local energy, Sherman-Morrison-Woodbury (kernels_det.hip) -----
+// One Slater matrix per (determinant, walker) of one spin: M[e][m] = mo_vgl value of MO
+// idx[d][w][m] at electron e0 + e of walker w.
+struct SlaterArgs {
+  const double* mo_vgl;  // [W*Ne][5][mo_num]
+  int64_t mo_num, elec_num, walk_num;
+  int n, e0;             // the spin's electrons e0 .. e0 + n - 1
+  const int32_t* idx;    // [det][W][n], 0-based MO indices (checked by the caller)
+  double* value;         // [det][W]: det M
+  double* inv;           // [det][W][n][n]: M^-1 (NaN when M is singular), or null
+  double* work;          // [det * W][n][n] working matrices, needed when n > 128
+  double* drift;         // [W][Ne][3]: 2 grad_e D / D of determinant 0, or null
+  double* kin;           // [W]: -1/2 sum_e lap_e D / D of determinant 0 (with drift), or null
+};
+hipError_t launch_slater(const SlaterArgs& a, int64_t n_mat, hipStream_t s);
+// out [det][W][5][n][n]: component c of MO idx[m] at electron e, [e][m] inner
+hipError_t launch_slater_vgl(const SlaterArgs& a, int64_t n_mat, double* out, hipStream_t s);
+// out[m] = value[m / nn] * inv[m] for m < count (adjugates of n x n matrices, nn = n * n)
+hipError_t launch_adjugate(const double* inv, const double* value, int64_t nn, int64_t count, double* out,
+                           hipStream_t s);
+// Per walker, from the coordinates [3][coord_ld]: V_ee, V_en, e_pot = V_ee + V_en + vnn and,
+// when kin_a is given, e_kin = kin_a + kin_b and e_local = e_kin + e_pot (null outputs skipped).
+struct PotArgs {
+  const double* coord;
+  int64_t coord_ld, elec_num, nucl_num;
+  const double *nucl_coord, *charge;  // [3][Nn], [Nn]
+  double vnn;
+  const double *kin_a, *kin_b;
+  double *vee, *ven, *e_pot, *e_kin, *e_local;
+};
+hipError_t launch_potential(const PotArgs& a, int64_t walk_num, hipStream_t s);
+// Sherman-Morrison-Woodbury on n_mat inverses [dim][lds] with n_upd updates [n_upd][lds] each
+// (1-based indices).  kind: 0 naive, 1 splitting, 2 splitting core (halves of the updates that
+// break down appended to later_* after later_num[m], stride later_cap per matrix), 3 Woodbury
+// 2x2, 4 Woodbury 3x3.  status[m]: 0 applied, 1 breakdown, 2 an index outside 1..dim; the
+// matrix and its determinant are written only when 0.
+struct SmwArgs {
+  int kind;
+  int64_t n_mat, lds, dim, n_upd;
+  const double* upd;
+  const uint64_t* upd_idx;
+  double breakdown;
+  double* s_inv;
+  double* det;          // [n_mat] or null
+  int32_t* status;      // [n_mat]
+  double* later_upd;    // kind 1: [n_mat][2][n_upd][lds] queue; kind 2: [n_mat][later_cap][lds]
+  uint64_t* later_idx;  // kind 1: [n_mat][2][n_upd]; kind 2: [n_mat][later_cap]
+  uint64_t* later_num;  // kind 2: [n_mat]
+  int64_t later_cap;
+  double* work;  // [n_mat][dim][dim] when dim > 128
+};
+hipError_t launch_smw(const SmwArgs& a, hipStream_t s);
 
 hipError_t launch_ee_distance(const double* coord, int64_t W, int64_t Ne, double* out, hipStream_t s);
 hipError_t launch_en_distance(const double* coord, int64_t P, const double* nucl_coord, int64_t Nn,
