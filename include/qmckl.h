@@ -421,6 +421,29 @@ qmckl_exit_code qmckl_get_jastrow_champ_single_accept(qmckl_context context);
  * QMCKL_FAILURE.  An invalid context returns QMCKL_INVALID_CONTEXT (the reference returns 0). */
 qmckl_exit_code qmckl_trexio_read(const qmckl_context context, const char* file_name, const int64_t size_max);
 
+/* ---- Orbital Hessians, MO cusp correction, forces on the orbitals ----------------------
+ * shell Hessian [point][3][4][shell] and AO Hessian [3][point][4][ao] ([i][j] = d_i d_j for
+ * j < 3, [i][3] = d_i of the Laplacian): org/qmckl_ao.org:5268-5305, 9397-9435.
+ * MO Hessian [3][point][3][mo]: org/qmckl_mo.org:2366-2420.
+ * qmckl_set_mo_basis_r_cusp (org/qmckl_mo.org:192-196, 479-700): per-nucleus radii inside which
+ * the nucleus' s AOs are replaced by a fitted cubic in r; every MO value/VGL/Hessian getter then
+ * returns the corrected MOs (fitted again by qmckl_mo_basis_select_mo).
+ * Forces (derivatives in the nuclear coordinates): AO values [nucl][point][3][ao], MO values
+ * [nucl][point][3][mo], MO gradients [nucl][3][point][3][mo], MO Laplacians
+ * [nucl][3][point][mo]: org/qmckl_forces.org:5127-5198, 5473-5535, 5917-5935, 6497-6535. */
+qmckl_exit_code qmckl_get_ao_basis_shell_hessian(qmckl_context context, double* const shell_hessian,
+                                                 const int64_t size_max);
+qmckl_exit_code qmckl_get_ao_basis_ao_hessian(qmckl_context context, double* const ao_hessian, const int64_t size_max);
+qmckl_exit_code qmckl_get_mo_basis_mo_hessian(qmckl_context context, double* const mo_hessian, const int64_t size_max);
+qmckl_exit_code qmckl_set_mo_basis_r_cusp(qmckl_context context, const double* r_cusp, const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_ao_value(qmckl_context context, double* const forces_ao_value, const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_mo_value(qmckl_context context, double* const forces_mo_value, const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_mo_value_inplace(qmckl_context context, double* const forces_mo_value,
+                                                  const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_mo_g(qmckl_context context, double* const forces_mo_g, const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_mo_g_inplace(qmckl_context context, double* const forces_mo_g, const int64_t size_max);
+qmckl_exit_code qmckl_get_forces_mo_l(qmckl_context context, double* const forces_mo_l, const int64_t size_max);
+
 /* ---- Slater determinants (org/under_development/qmckl_determinant.org:344-556, 569-640,
  * 1308-1495).  The reference does not build this module; names and argument lists are its own.
  * mo_index_alpha/beta: [det_num][walk_num][up_num / down_num], 1-based MO indices, set after

@@ -747,3 +747,143 @@ int oracle_jastrow(int64_t walk_num, int64_t elec_num, int64_t up_num, int64_t n
   free(cfull);
   return SUCCESS;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Hessians, org/qmckl_ao.org:5257-5585 (shells) and :9397-9718 (AOs).                  */
+/* ------------------------------------------------------------------------------------ */
+
+/* Derivatives of the radial part S = sum_p c_p exp(-alpha_p r^2) of shell s at r = (x,y,z):
+ * h[i][j] = d_i d_j S, t[i] = d_i lap S (org/qmckl_ao.org:5257-5265, 5540-5584), and its
+ * value, gradient and Laplacian in vgl[5]; same screening as the shell VGL. */
+static void shell_hessian_at(const double r[3], double r2, int64_t s, const int64_t *shell_prim_num,
+                             const int64_t *shell_prim_index, const double *exponent, const basis_extra *b,
+                             double cutoff, double vgl[5], double h[3][3], double t[3]) {
+  for (int i = 0; i < 5; ++i) vgl[i] = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    t[i] = 0.0;
+    for (int j = 0; j < 3; ++j) h[i][j] = 0.0;
+  }
+  for (int64_t p = shell_prim_index[s]; p < shell_prim_index[s] + shell_prim_num[s]; ++p) {
+    const double ar2 = exponent[p] * r2;
+    if (ar2 > cutoff) continue;
+    const double v = b->coef_normalized[p] * exp(-ar2);
+    const double ta = 2.0 * exponent[p];
+    vgl[0] += v;
+    for (int i = 0; i < 3; ++i) vgl[1 + i] += -ta * r[i] * v;
+    vgl[4] += (ta * ta * r2 - 3.0 * ta) * v;
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) h[i][j] += (ta * ta * r[i] * r[j] - (i == j ? ta : 0.0)) * v;
+      t[i] += (5.0 * ta * ta * r[i] - ta * ta * ta * r2 * r[i]) * v;
+    }
+  }
+}
+
+static double ipw(double y, int e) {
+  if (e < 0) return 0.0;
+  double r = 1.0;
+  for (int i = 0; i < e; ++i) r *= y;
+  return r;
+}
+
+/* Derivatives of the monomial M = y0^n0 y1^n1 y2^n2: value, gradient g, Hessian h, Laplacian
+ * and the gradient t of its Laplacian (the role of qmckl_ao_polynomial_hessian, :9585-9630). */
+static void monomial_derivs(const double Y[3], const int n[3], double *v, double g[3], double h[3][3],
+                            double *lap, double t[3]) {
+  double p[3][4], d1[3], d2[3], d3[3];
+  for (int d = 0; d < 3; ++d) {
+    const double e = n[d];
+    p[d][0] = ipw(Y[d], n[d]);
+    d1[d] = e * ipw(Y[d], n[d] - 1);
+    d2[d] = e * (e - 1.0) * ipw(Y[d], n[d] - 2);
+    d3[d] = e * (e - 1.0) * (e - 2.0) * ipw(Y[d], n[d] - 3);
+  }
+  *v = p[0][0] * p[1][0] * p[2][0];
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    g[i] = d1[i] * p[j][0] * p[k][0];
+    h[i][i] = d2[i] * p[j][0] * p[k][0];
+    h[i][j] = h[j][i] = d1[i] * d1[j] * p[k][0];
+    t[i] = d3[i] * p[j][0] * p[k][0] + d1[i] * (d2[j] * p[k][0] + p[j][0] * d2[k]);
+  }
+  *lap = h[0][0] + h[1][1] + h[2][2];
+}
+
+/* shell_hessian[point][3][4][shell_num]: [i][j] = d_i d_j S (j < 3), [i][3] = d_i lap S. */
+int oracle_shell_hessian(int64_t nucl_num, int64_t shell_num, int64_t prim_num, const double *nucl_coord,
+                         const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
+                         const int32_t *shell_ang_mom, const int64_t *shell_prim_num,
+                         const int64_t *shell_prim_index, const double *shell_factor, const double *exponent,
+                         const double *coefficient, const double *prim_factor, const int32_t *r_power,
+                         int32_t precision, int64_t point_num, const double *coord, double *out) {
+  basis_extra b;
+  finalize_basis(nucl_num, shell_num, prim_num, nucleus_index, nucleus_shell_num, shell_ang_mom,
+                 shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor, r_power,
+                 &b);
+  const double cutoff = oracle_cutoff(precision);
+  memset(out, 0, sizeof(double) * point_num * 12 * shell_num);
+  for (int64_t ip = 0; ip < point_num; ++ip)
+    for (int64_t a = 0; a < nucl_num; ++a) {
+      const double r[3] = {coord[ip] - nucl_coord[a], coord[ip + point_num] - nucl_coord[a + nucl_num],
+                           coord[ip + 2 * point_num] - nucl_coord[a + 2 * nucl_num]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      if (r2 > cutoff * b.nucleus_range[a]) continue;
+      for (int64_t s = nucleus_index[a]; s < nucleus_index[a] + nucleus_shell_num[a]; ++s) {
+        double vgl[5], h[3][3], t[3];
+        shell_hessian_at(r, r2, s, shell_prim_num, shell_prim_index, exponent, &b, cutoff, vgl, h, t);
+        for (int i = 0; i < 3; ++i) {
+          double *o = out + (ip * 3 + i) * 4 * shell_num + s;
+          for (int j = 0; j < 3; ++j) o[j * shell_num] = h[i][j];
+          o[3 * shell_num] = t[i];
+        }
+      }
+    }
+  free_basis(&b);
+  return SUCCESS;
+}
+
+/* ao_hessian[3][point][4][ao_num]: [i][p][j][k] = d_i d_j AO_k (j < 3), [i][p][3][k] =
+ * d_i lap AO_k, AO_k = f_k M_k S_s by the product rule of org/qmckl_ao.org:9686-9709. */
+int oracle_ao_hessian(int64_t nucl_num, int64_t shell_num, int64_t prim_num, int64_t ao_num,
+                      const double *nucl_coord, const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
+                      const int32_t *shell_ang_mom, const int64_t *shell_prim_num, const int64_t *shell_prim_index,
+                      const double *shell_factor, const double *exponent, const double *coefficient,
+                      const double *prim_factor, const int32_t *r_power, const double *ao_factor,
+                      int32_t precision, int64_t point_num, const double *coord, double *out) {
+  basis_extra b;
+  finalize_basis(nucl_num, shell_num, prim_num, nucleus_index, nucleus_shell_num, shell_ang_mom,
+                 shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor, r_power,
+                 &b);
+  const double cutoff = oracle_cutoff(precision);
+  memset(out, 0, sizeof(double) * 3 * point_num * 4 * ao_num);
+#pragma omp parallel for schedule(static)
+  for (int64_t ip = 0; ip < point_num; ++ip)
+    for (int64_t a = 0; a < nucl_num; ++a) {
+      const double r[3] = {coord[ip] - nucl_coord[a], coord[ip + point_num] - nucl_coord[a + nucl_num],
+                           coord[ip + 2 * point_num] - nucl_coord[a + 2 * nucl_num]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      if (r2 > cutoff * b.nucleus_range[a]) continue;
+      for (int64_t s = nucleus_index[a]; s < nucleus_index[a] + nucleus_shell_num[a]; ++s) {
+        double S[5], Sh[3][3], St[3];
+        shell_hessian_at(r, r2, s, shell_prim_num, shell_prim_index, exponent, &b, cutoff, S, Sh, St);
+        const int l = shell_ang_mom[s];
+        int64_t k = b.ao_index[s];
+        for (int na = l; na >= 0; --na)
+          for (int nb = l - na; nb >= 0; --nb, ++k) {
+            const int n[3] = {na, nb, l - na - nb};
+            double M, Mg[3], Mh[3][3], Ml, Mt[3];
+            monomial_derivs(r, n, &M, Mg, Mh, &Ml, Mt);
+            const double f = ao_factor[k];
+            for (int i = 0; i < 3; ++i) {
+              double *o = out + ((int64_t)i * point_num + ip) * 4 * ao_num + k;
+              for (int j = 0; j < 3; ++j)
+                o[j * ao_num] = (Mh[i][j] * S[0] + M * Sh[i][j] + Mg[j] * S[1 + i] + Mg[i] * S[1 + j]) * f;
+              double cross = 0.0;
+              for (int j = 0; j < 3; ++j) cross += Mh[j][i] * S[1 + j] + Mg[j] * Sh[j][i];
+              o[3 * ao_num] = (Mt[i] * S[0] + M * St[i] + Ml * S[1 + i] + Mg[i] * S[4] + 2.0 * cross) * f;
+            }
+          }
+      }
+    }
+  free_basis(&b);
+  return SUCCESS;
+}

@@ -335,7 +335,7 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
   return QMCKL_SUCCESS;
 }
 
-static AoDevView ao_view(Context* c) {
+AoDevView ao_view(Context* c) {
   const DevConst& d = c->dc;
   AoDevView v;
   v.nucl_num = c->nucl_num;
@@ -358,7 +358,7 @@ static AoDevView ao_view(Context* c) {
   return v;
 }
 
-static qmckl_exit_code ao_ready(Context* c, qmckl_context handle, const char* fn) {
+qmckl_exit_code ao_ready(Context* c, qmckl_context handle, const char* fn) {
   if (!c->ao_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
   if (c->point_num <= 0) return fail(handle, QMCKL_NOT_PROVIDED, fn, "no points set");
   qmckl_exit_code rc = upload_ao(c, handle, fn);
@@ -435,6 +435,10 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     e = launch_ao_mo(a, mode, c->stream);
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
+  if (!c->r_cusp.empty()) {
+    rc = apply_mo_cusp(c, handle, out, 0, fn);
+    if (rc != QMCKL_SUCCESS) return rc;
+  }
   if (d_out == nullptr) c->mo_vgl_date = c->date;
   return QMCKL_SUCCESS;
 }
@@ -907,6 +911,11 @@ qmckl_exit_code qmckl_mo_basis_select_mo(const qmckl_context context, const int3
   c->mo_coef.swap(coef);
   c->mo_num = k;
   touch_const(c);
+  if (!c->r_cusp.empty()) {
+    // the cusp fit is per MO: fitted again for the kept MOs (org/qmckl_mo.org:959-967)
+    const std::vector<double> r(c->r_cusp);
+    return qmckl_set_mo_basis_r_cusp(context, r.data(), (int64_t)r.size());
+  }
   return QMCKL_SUCCESS;
 }
 

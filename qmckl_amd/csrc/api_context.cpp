@@ -205,7 +205,8 @@ std::vector<DevBuf*> device_buffers(Context* c) {
                     &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c, &d.jas_charge,
                                 &c->d_single_coord, &c->d_single_out, &c->d_det_idx, &c->d_det_value,
                                 &c->d_det_inv, &c->d_det_work, &c->d_det_out, &c->d_det_drift,
-                                &c->d_det_kin, &c->d_energy, &c->d_pot_charge, &c->d_smw};
+                                &c->d_det_kin, &c->d_energy, &c->d_pot_charge, &c->d_smw, &c->d_cusp,
+                                &c->d_ao_hess, &c->d_orb_out, &c->d_nucl_ao};
   for (DevBuf& b : c->d_deriv) bufs.push_back(&b);
   return bufs;
 }
@@ -287,7 +288,7 @@ qmckl_context qmckl_context_copy(const qmckl_context context) {
   CP(jas_uninit); CP(jas_provided); CP(asymp_jasa);
   CP(single_num); CP(single_coord); CP(single_date);
   CP(det_type); CP(det_num_alpha); CP(det_num_beta); CP(mo_index_alpha); CP(mo_index_beta);
-  CP(det_uninit); CP(det_set_date);
+  CP(det_uninit); CP(det_set_date); CP(r_cusp); CP(cusp_param);
 #undef CP
   n->mo_index_walk[0] = c->mo_index_walk[0];
   n->mo_index_walk[1] = c->mo_index_walk[1];

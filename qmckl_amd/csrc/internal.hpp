@@ -193,6 +193,17 @@ struct Context {
   uint64_t pot_charge_date = 0;
   DevBuf d_smw;  // Sherman-Morrison-Woodbury staging and scratch
 
+  // MO cusp correction, org/qmckl_mo.org:479-700: r_cusp [Nn] (empty: none), cusp_param
+  // [Nn][4][mo]; device: r_cusp, params, the s-AO table of each nucleus
+  std::vector<double> r_cusp, cusp_param;
+  DevBuf d_cusp;
+  int64_t cusp_s_total = 0;
+  bool cusp_uploaded = false;
+  // orbital Hessians and forces: AO Hessian [3][P][4][ao] (valid at d_ao_hess_date), outputs,
+  // first AO of each nucleus [Nn + 1]
+  DevBuf d_ao_hess, d_orb_out, d_nucl_ao;
+  uint64_t ao_hess_date = 0, nucl_ao_date = 0;
+
   // kernel timing (profiling extension)
   bool profiling = false;
   std::vector<ProfEvent> prof_pending;
@@ -231,6 +242,9 @@ qmckl_exit_code hip_fail(qmckl_context handle, hipError_t e, const char* fn);
 hipError_t copy_to_host(Context* c, void* dst, const void* d_src, size_t bytes);
 
 // providers
+struct AoDevView;
+AoDevView ao_view(Context* c);  // the AO tables of c on its device
+qmckl_exit_code ao_ready(Context* c, qmckl_context handle, const char* fn);  // tables and coordinates uploaded
 qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, const char* fn);
 qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn);
@@ -239,6 +253,9 @@ qmckl_exit_code provide_en_distance(qmckl_context handle, Context* c, const char
 // e_kin, e_pot, e_local, V_ee, V_en per walker in c->d_energy ([5][W]); kinetic: with the
 // Slater determinants (api_det.cpp)
 qmckl_exit_code provide_energy(Context* c, qmckl_context handle, const char* fn, bool kinetic);
+// With r_cusp set, the cusp correction of a device MO VGL [P][5][mo] (hessian = 0) or MO
+// Hessian [3][P][3][mo] (hessian = 1) of c's points, in place (api_orb.cpp)
+qmckl_exit_code apply_mo_cusp(Context* c, qmckl_context handle, double* out, int hessian, const char* fn);
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------
 struct AoDevView {
@@ -459,6 +476,45 @@ struct SmwArgs {
   double* work;  // [n_mat][dim][dim] when dim > 128
 };
 hipError_t launch_smw(const SmwArgs& a, hipStream_t s);
+
+// ---- orbital Hessians, forces on the orbitals, MO cusp correction (kernels_orb.hip) ----
+// [3][P][4][ao]: d_i d_j AO (j < 3), d_i lap AO (j = 3)
+hipError_t launch_ao_hessian(const AoDevView& v, const double* coord, int64_t P, double* out, hipStream_t s);
+// [P][3][4][shell]: the same for the radial parts of the shells
+hipError_t launch_shell_hessian(const AoDevView& v, const double* coord, int64_t P, double* out, hipStream_t s);
+// out[a][u][v][i] (strides oa, ou, ov) = sign * sum_{k_start[a] <= k < k_start[a+1]}
+// src[base + u su + v sv + k] * ct[k][i] (ct row stride ldct), for a < na, u < U, v < V, i < mo
+struct ContractArgs {
+  const double* src;
+  int64_t base, su, sv, U, V;
+  const double* ct;
+  int64_t ldct, mo;
+  const int64_t* k_start;  // [na + 1], device
+  int64_t na;
+  double* out;
+  int64_t oa, ou, ov;
+  double sign;
+};
+hipError_t launch_orb_contract(const ContractArgs& c, hipStream_t s);
+// [Nn][P][3][ao]: -grad AO for the AOs on the nucleus, 0 for the others
+hipError_t launch_forces_ao_value(const AoDevView& v, const double* ao_vgl, int64_t P, double* out, hipStream_t s);
+// Cusp correction in place: at every point closer than r_cusp[A] to nucleus A, the s AOs of A
+// (s_ao[s_start[A] .. s_start[A+1]), at most cusp_max_s_ao()) taken out of the MOs and the
+// polynomial param[A][0..3][i] r^n added; out is the MO VGL [P][5][mo] (hessian = 0) or the MO
+// Hessian [3][P][3][mo] (hessian = 1).
+struct CuspArgs {
+  const double* coord;  // [3][P]
+  int64_t P, mo, nucl_num;
+  const double *r_cusp, *param;  // [Nn], [Nn][4][mo]
+  const int32_t *s_start, *s_ao;
+  const double* ct;  // [ao][ldct] (FP64 C^T)
+  int64_t ldct;
+  AoDevView v;
+  double* out;
+  int hessian;
+};
+int cusp_max_s_ao();
+hipError_t launch_mo_cusp(const CuspArgs& c, hipStream_t s);
 
 hipError_t launch_ee_distance(const double* coord, int64_t W, int64_t Ne, double* out, hipStream_t s);
 hipError_t launch_en_distance(const double* coord, int64_t P, const double* nucl_coord, int64_t Nn,
