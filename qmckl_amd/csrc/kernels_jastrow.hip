@@ -1526,30 +1526,63 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   constexpr int NPR = een_npair(C), NSG = een_nsingle(C);
   {
     // Once per workgroup (ready at the first item's barrier): the constant inputs, and the
-    // owner wave of every group of units of a tile.  The groups are dealt longest first to
-    // the least loaded wave (lowest index on ties): the pairs of every full nucleus tile,
-    // then its single units, then, when the last tile holds at most 4 nuclei, its pairs
-    // and single units on the 4x4x4 products (about a third of the work of a full-tile
-    // unit; a quarter for a pair when that tile holds at most 2 nuclei).  A second table serves the electron tiles of at most 8 rows, whose full
-    // nucleus tiles run on the 4x4x4 products by rows (about two thirds of the work).
+    // owner wave of every group of units of a tile.  Waves w and w + 4 of the workgroup
+    // share a SIMD (measured on gfx950), on which the FP64 matrix instructions and all other
+    // vector instructions issue one at a time; a SIMD is best used while both its waves have
+    // work.  So every wave gets the same number of groups (one more for some when they do
+    // not divide) and the SIMDs the same cost: the groups, in decreasing cost (the pairs of
+    // every full nucleus tile, its single units, then, when the last tile holds at most 4
+    // nuclei, its pairs and single units on the 4x4x4 products), go to the least loaded SIMD
+    // that has room, and there to the wave with fewer groups.  A group's cost is its steps
+    // times the matrix cycles of a step (64 per 16x16x4, 16 per 4x4x4 product) plus about
+    // 4.5 cycles per vector instruction, plus its fold.  A second table serves the electron
+    // tiles of at most 8 rows, whose full nucleus tiles run on the 4x4x4 products by rows.
     const int Nn = (int)v.nucl_num, NA = gn_na(Nn), NA16 = 16 * NA;
     const EenResLds L(gn_jp((int)v.elec_num), NA16, C, DIM);
     lds_copy4(lds + L.nuc, v.nuc_pad, 4 * NA16, wave, threadIdx.x & 63);
     lds_copy4(lds + L.cfl, v.cfull_pad, DIM * NA16, wave, threadIdx.x & 63);
     if (threadIdx.x == 0) {
+      static_assert(EQ_WAVES == 8, "two waves per SIMD: waves w and w + 4");
       const bool mini = Nn - 16 * (NA - 1) <= 4, mini2 = Nn - 16 * (NA - 1) <= 2;
       const int NAF = mini ? NA - 1 : NA;
       const int n0 = NPR * NAF, n1 = n0 + NSG * NAF, n3 = n1 + (mini ? NPR + NSG : 0);
+      const int nsteps = gn_jp((int)v.elec_num) >> 2;
+      const int cap = (n3 + EQ_WAVES - 1) / EQ_WAVES;  // groups per wave, at most
       int* own = reinterpret_cast<int*>(lds + L.own);
       for (int rows = 0; rows < 2; ++rows) {
-        int load[EQ_WAVES];
-        for (int wv = 0; wv < EQ_WAVES; ++wv) load[wv] = 0;
+        int simd_load[4] = {0, 0, 0, 0}, wave_load[EQ_WAVES], wave_cnt[EQ_WAVES];
+        for (int wv = 0; wv < EQ_WAVES; ++wv) wave_load[wv] = wave_cnt[wv] = 0;
         for (int it = 0; it < n3; ++it) {
-          const int weight = it < n0 ? (rows ? 8 : 12) : it < n1 ? (rows ? 4 : 6) : it < n1 + NPR ? (mini2 ? 3 : 4) : 2;
-          int owner = 0;
-          for (int wv = 1; wv < EQ_WAVES; ++wv)
-            if (load[wv] < load[owner]) owner = wv;
-          load[owner] += weight;
+          // matrix cycles and vector instructions of one step, and the fold
+          int mc, vi, fold;
+          if (it < n0) {
+            mc = rows ? 320 : 640, vi = rows ? 25 : 14, fold = 1200;
+          } else if (it < n1) {
+            mc = rows ? 160 : 320, vi = rows ? 22 : 13, fold = 700;
+          } else if (it < n1 + NPR) {
+            mc = mini2 ? 80 : 160, vi = 14, fold = 1200;
+          } else {
+            mc = 80, vi = 13, fold = 700;
+          }
+          const int cost = nsteps * (mc + (9 * vi) / 2) + fold;
+          int sb = -1;
+          for (int sd = 0; sd < 4; ++sd) {
+            const bool room = wave_cnt[sd] < cap || wave_cnt[sd + 4] < cap;
+            if (room && (sb < 0 || simd_load[sd] < simd_load[sb])) sb = sd;
+          }
+          const int wa = sb, wb = sb + 4;
+          int owner;
+          if (wave_cnt[wa] >= cap)
+            owner = wb;
+          else if (wave_cnt[wb] >= cap)
+            owner = wa;
+          else if (wave_cnt[wa] != wave_cnt[wb])
+            owner = wave_cnt[wa] < wave_cnt[wb] ? wa : wb;
+          else
+            owner = wave_load[wa] <= wave_load[wb] ? wa : wb;
+          simd_load[sb] += cost;
+          wave_load[owner] += cost;
+          ++wave_cnt[owner];
           own[rows * n3 + it] = owner;
         }
       }
