@@ -65,6 +65,8 @@ def main():
                     help="instead of one molecule, evaluate 64 molecules of 10..400 electrons (256 walkers "
                          "each) from one recorded HIP graph, against the same steps launched eagerly")
     ap.add_argument("--molecules", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched workload: keep the default product shapes")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="copy this many MiB device-to-device once before timing (a copy kernel of "
                          "known size, the scale for HBM traffic counters)")
@@ -385,9 +387,20 @@ def bench_batched(args, q, S, L):
     ne = [s.elec_num for s, _, _ in mols]
     nao = [s.ao_num for s, _, _ in mols]
 
+    ctx_arr = np.asarray(ctxs, np.int64)
+    tuned = {}
+
     def set_mode(m):
         for ctx in ctxs:
             q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, m))
+        if not args.no_autotune:  # product shapes per size bucket, timed once per process
+            shapes = np.zeros((len(ctxs), 3), np.int32)
+            t0 = time.perf_counter()
+            q.check(ctxs[0], L.qmckl_amd_autotune(ctx_arr.ctypes.data, len(ctxs), shapes.ctypes.data))
+            q.device_synchronize()
+            tuned[m] = {"seconds": time.perf_counter() - t0,
+                        "shapes": sorted({(int(-(-s.mo_num_kept // 16)), *map(int, sh))
+                                          for (s, _, _), sh in zip(mols, shapes)})}
 
     def eager_step():
         for (s, _, _), ctx, cb, m, v, g in zip(mols, ctxs, coord, mo, val, gl):
@@ -447,6 +460,11 @@ def bench_batched(args, q, S, L):
         "mo_err_vs_fp64_max_normalised": dict(zip(["value", "dx", "dy", "dz", "lap"], err.tolist())),
         "eager_path": {"value": n_walk / t_eager, "ms_per_step": t_eager * 1e3},
         "graph_speedup_vs_eager": t_eager / t_graph,
+        "autotune": None if args.no_autotune else {
+            {q.MO_FP64: "fp64", q.MO_FP32: "fp32", q.MO_BF16X3: "bf16x3"}[m]: {
+                "seconds": round(v["seconds"], 3),
+                "shapes (column tiles: per pass, waves, passes)": [list(x) for x in v["shapes"]]}
+            for m, v in tuned.items()},
     }
     print(json.dumps(out))
 

@@ -538,6 +538,23 @@ qmckl_exit_code qmckl_woodbury_3x3_doc(const qmckl_context context, const uint64
 #define QMCKL_AMD_MO_BF16X3 ((int32_t)2)
 qmckl_exit_code qmckl_amd_set_mo_precision(qmckl_context context, const int32_t mode);
 qmckl_exit_code qmckl_amd_get_mo_precision(qmckl_context context, int32_t* const mode);
+/* Shape of the fused AO->MO product: col_tiles column tiles of 16 MOs per pass over the AOs
+ * (1..16; 0 restores the default) on `waves` waves per workgroup (4 for 1..8 tiles, 8 for
+ * 5..16).  The default takes every tile in one pass up to 8 tiles on 4 waves, else passes of
+ * at most 16 tiles on 8 waves.  A change of shape rebuilds the product's operand layout (a
+ * constant-data change).  qmckl_amd_get_mo_tile returns the shape in use and the number of
+ * passes. */
+qmckl_exit_code qmckl_amd_set_mo_tile(qmckl_context context, const int32_t col_tiles, const int32_t waves);
+qmckl_exit_code qmckl_amd_get_mo_tile(qmckl_context context, int32_t* const col_tiles, int32_t* const waves,
+                                      int32_t* const passes);
+/* Autotuner of the product shape: groups the contexts into size buckets (device, MO
+ * precision, column tiles, 32-AO chunks and 16-point tiles rounded up to powers of two),
+ * times every built shape (1 to 4 passes on 4 or 8 waves) on the first context of each bucket
+ * with its current points (HIP events on its stream; the context's MO VGL buffer is
+ * overwritten), and sets the fastest shape on every context of the bucket.  The choice is
+ * kept per process: a bucket is timed once.  shapes (NULL or [n][3]): the column tiles,
+ * waves and passes chosen for each context.  Needs the MO basis and the points set. */
+qmckl_exit_code qmckl_amd_autotune(const qmckl_context* contexts, const int64_t n, int32_t* const shapes);
 
 /* HIP stream (a hipStream_t, or NULL for the context's own stream) used by every kernel
  * of this context. */
@@ -567,15 +584,17 @@ qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, dou
  *   qmckl_amd_get_jastrow_champ_value_device(..., d_value[i], ...)  (skipped if d_value is NULL)
  *   qmckl_amd_get_jastrow_champ_gl_device(..., d_gl[i], ...)        (skipped if d_gl is NULL)
  * once eagerly (walk_num_i is the context's current walker count; its coordinates must have
- * been set), then records the same steps into a graph that forks from and joins back to
- * `stream` (a hipStream_t; NULL: a stream of the batch's own); qmckl_amd_batch_replay
- * launches that graph on it.  Each replay reads the
- * current contents of the d_coord[i] buffers.  The contexts' streams are set to the batch's
- * branch streams until qmckl_amd_batch_destroy; the contexts must outlive the batch, and kernel
- * profiling must be off.  qmckl_amd_batch_replay returns QMCKL_FAILURE without launching
- * anything when a context has changed since the recording: its walker, electron or MO count,
- * its MO precision, any constant data (a setter other than the coordinates), or the address or
- * size of any of its device buffers (a getter that allocated or grew one). */
+ * been set), then records the same work for all the contexts as grouped launches (one per
+ * kernel and shape, each covering every context) into a graph that forks from and joins
+ * back to `stream` (a hipStream_t; NULL: a stream of the batch's own);
+ * qmckl_amd_batch_replay launches that graph on it, with the same results as the eager
+ * steps, bit for bit.  Each replay reads the current contents of the d_coord[i] buffers.
+ * The contexts' streams are set to the batch's stream until qmckl_amd_batch_destroy; the
+ * contexts must be distinct and outlive the batch, and kernel profiling must be off.
+ * qmckl_amd_batch_replay returns QMCKL_FAILURE without launching anything when a context has
+ * changed since the recording: its walker, electron or MO count, its MO precision, any
+ * constant data (a setter other than the coordinates, a new product shape), or the address
+ * or size of any of its device buffers (a getter that allocated or grew one). */
 qmckl_exit_code qmckl_amd_batch_record(const qmckl_context* contexts, const int64_t n, void* stream,
                                        const double* const* d_coord, double* const* d_mo_vgl,
                                        double* const* d_value, double* const* d_gl, int64_t* const batch);

@@ -194,6 +194,16 @@ static std::vector<int64_t> fused_ao_order(const Context* c) {
   return perm;
 }
 
+// Shape of c's fused product for nct column tiles: the requested one (capped at nct tiles,
+// the waves adjusted when the cap leaves the requested pair unbuilt), else the default.
+static void mo_tile_shape(const Context* c, int64_t nct, int& G, int& nw) {
+  ao_mo_default_shape(nct, G, nw);
+  if (c->mo_tile_g > 0) {
+    G = (int)std::min<int64_t>(c->mo_tile_g, std::max<int64_t>(nct, 1));
+    nw = ao_mo_variant_ok(0, G, c->mo_tile_nw) ? c->mo_tile_nw : ao_mo_waves(G);
+  }
+}
+
 // Uploads the AO tables and nuclear coordinates (once per context date change).
 static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* fn) {
   qmckl_exit_code rc = device_ready(c, handle, fn);
@@ -302,9 +312,11 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
     // row k scaled by ao_factor[k] (the fused kernel's AO values leave the factor out:
     // sum_k (f_k P_k) C_kj = sum_k P_k (f_k C_kj))
     const int64_t nct = pad / 16, nchunk = kpad / AO_MO_KC;
-    const int G = (int)std::min<int64_t>(AO_MO_MAX_G, nct);
+    int G = 0, nw = 0;
+    mo_tile_shape(c, nct, G, nw);
     const int64_t ngroup = (nct + G - 1) / G, NG16 = 16 * G;
     d.ao_mo_g = G;
+    d.ao_mo_nw = nw;
     d.ao_mo_ngroup = (int)ngroup;
     std::vector<double> b64((size_t)(ngroup * nchunk) * AO_MO_KC * NG16, 0.0);
     std::vector<float> b32(b64.size(), 0.0f);
@@ -382,6 +394,53 @@ qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn)
   return QMCKL_SUCCESS;
 }
 
+void ao_mo_default_shape(int64_t nct, int& G, int& nw) {
+  if (nct <= 8) {
+    G = (int)std::max<int64_t>(nct, 1);
+    nw = 4;
+    return;
+  }
+  const int64_t ngroup = (nct + 15) / 16;
+  G = (int)((nct + ngroup - 1) / ngroup);  // >= 5
+  nw = 8;
+}
+
+qmckl_exit_code mo_product_args(qmckl_context handle, Context* c, double* d_out, const char* fn, AoMoArgs& a,
+                                int& mode) {
+  if (!c->mo_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
+  qmckl_exit_code rc = ao_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const int64_t P = c->point_num, Nmo = c->mo_num;
+  const DevConst& d = c->dc;
+  mode = c->mo_precision;
+  a.coord = c->d_coord.as<double>();
+  a.coord_ld = P;
+  a.p0 = 0;
+  a.np = P;
+  // tiles of the same electron of consecutive walkers (points close to one another) when
+  // the points are the walkers' electrons; else consecutive points
+  const bool walkers = c->walk_num > 1 && c->walk_num * c->elec_num == P;
+  a.tw = walkers ? c->walk_num : 1;
+  a.te = walkers ? c->elec_num : P;
+  a.desc = d.ao_desc.as<double>();
+  a.desc_off = d.ao_desc_off.as<int64_t>();
+  a.desc_max = d.ao_desc_max;
+  a.ns_max = d.ao_ns_max;
+  a.nchunk = d.ao_nchunk;
+  a.G = d.ao_mo_g;
+  a.nw = d.ao_mo_nw;
+  a.ngroup = d.ao_mo_ngroup;
+  a.B = mode == QMCKL_AMD_MO_FP64 ? d.coef_ch64.ptr : mode == QMCKL_AMD_MO_FP32 ? d.coef_ch32.ptr : d.coef_chbf.ptr;
+  a.b_chunk_bytes = mode == QMCKL_AMD_MO_FP64   ? AO_MO_KC * 16 * a.G * 8
+                    : mode == QMCKL_AMD_MO_FP32 ? AO_MO_KC * 16 * a.G * 4
+                                                : 3 * 16 * a.G * AO_MO_BF_LD * 2;
+  a.C = d_out;
+  a.ldc = Nmo;
+  a.N = Nmo;
+  a.cutoff = ao_view(c).cutoff;
+  return QMCKL_SUCCESS;
+}
+
 // MO VGL into d_out (or into the context buffer when d_out is null); org/qmckl_mo.org:1749-1878.
 // The AO VGL is evaluated inside the product kernel (it is never stored), unless the FP64
 // AO VGL of these points is already resident, in which case it is multiplied directly.
@@ -406,33 +465,11 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     e = launch_gemm_f64(c->d_ao_vgl.as<double>(), Nao, c->dc.coef_t.as<double>(), pad, out, Nmo, 5 * P, Nmo,
                         Nao, c->stream);
   } else {
-    const DevConst& d = c->dc;
     AoMoArgs a;
-    a.coord = c->d_coord.as<double>();
-    a.coord_ld = P;
-    a.p0 = 0;
-    a.np = P;
-    // tiles of the same electron of consecutive walkers (points close to one another) when
-    // the points are the walkers' electrons; else consecutive points
-    const bool walkers = c->walk_num > 1 && c->walk_num * c->elec_num == P;
-    a.tw = walkers ? c->walk_num : 1;
-    a.te = walkers ? c->elec_num : P;
-    a.desc = d.ao_desc.as<double>();
-    a.desc_off = d.ao_desc_off.as<int64_t>();
-    a.desc_max = d.ao_desc_max;
-    a.ns_max = d.ao_ns_max;
-    a.nchunk = d.ao_nchunk;
-    a.G = d.ao_mo_g;
-    a.ngroup = d.ao_mo_ngroup;
-    a.B = mode == QMCKL_AMD_MO_FP64 ? d.coef_ch64.ptr : mode == QMCKL_AMD_MO_FP32 ? d.coef_ch32.ptr : d.coef_chbf.ptr;
-    a.b_chunk_bytes = mode == QMCKL_AMD_MO_FP64   ? AO_MO_KC * 16 * a.G * 8
-                      : mode == QMCKL_AMD_MO_FP32 ? AO_MO_KC * 16 * a.G * 4
-                                                  : 3 * 16 * a.G * AO_MO_BF_LD * 2;
-    a.C = out;
-    a.ldc = Nmo;
-    a.N = Nmo;
-    a.cutoff = ao_view(c).cutoff;
-    e = launch_ao_mo(a, mode, c->stream);
+    int m = 0;
+    rc = mo_product_args(handle, c, out, fn, a, m);
+    if (rc != QMCKL_SUCCESS) return rc;
+    e = launch_ao_mo(a, m, c->stream);
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   if (!c->r_cusp.empty()) {
@@ -980,6 +1017,39 @@ qmckl_exit_code qmckl_amd_set_mo_precision(qmckl_context context, const int32_t 
   std::lock_guard<std::recursive_mutex> g(c->mutex);
   c->mo_precision = mode;
   c->mo_vgl_date = 0;
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_amd_set_mo_tile(qmckl_context context, const int32_t col_tiles, const int32_t waves) {
+  const char* fn = "qmckl_amd_set_mo_tile";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (col_tiles < 0 || col_tiles > AO_MO_MAX_G) return fail(context, QMCKL_INVALID_ARG_2, fn, "column tiles: 0 (default) or 1..16");
+  if (col_tiles > 0 && !ao_mo_variant_ok(0, col_tiles, waves))
+    return fail(context, QMCKL_INVALID_ARG_3, fn, "waves: 4 for 1..8 column tiles, 8 for 5..16");
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (c->mo_tile_g == col_tiles && c->mo_tile_nw == (col_tiles > 0 ? waves : 0)) return QMCKL_SUCCESS;
+  c->mo_tile_g = col_tiles;
+  c->mo_tile_nw = col_tiles > 0 ? waves : 0;
+  touch_const(c);  // the operand layout of C_t depends on the shape
+  return QMCKL_SUCCESS;
+}
+
+qmckl_exit_code qmckl_amd_get_mo_tile(qmckl_context context, int32_t* const col_tiles, int32_t* const waves,
+                                      int32_t* const passes) {
+  const char* fn = "qmckl_amd_get_mo_tile";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  if (col_tiles == nullptr || waves == nullptr || passes == nullptr)
+    return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  const int64_t nct = ((c->mo_num + 15) / 16);
+  int G = 0, nw = 0;
+  mo_tile_shape(c, nct, G, nw);
+  *col_tiles = G;
+  *waves = nw;
+  *passes = (int32_t)((nct + G - 1) / G);
   return QMCKL_SUCCESS;
 }
 

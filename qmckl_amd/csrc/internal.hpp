@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/qmckl.h"
@@ -55,6 +56,7 @@ struct DevConst {
   DevBuf coef_ch64, coef_ch32, coef_chbf;
   int64_t mo_pad = 0, k_pad = 0;
   int ao_mo_g = 1, ao_mo_ngroup = 1;  // column tiles per pass of the fused kernel, passes
+  int ao_mo_nw = 4;                   // its waves per workgroup
   // 32-AO chunk descriptors of the fused kernel (AoMoArgs), offsets in doubles
   DevBuf ao_desc, ao_desc_off;
   int ao_desc_max = 0, ao_nchunk = 0, ao_ns_max = 1;
@@ -122,6 +124,9 @@ struct Context {
   uint32_t mo_uninit = 0x3;
   bool mo_provided = false;
   int32_t mo_precision = QMCKL_AMD_MO_FP64;
+  // requested shape of the fused AO->MO product: column tiles per pass and waves per
+  // workgroup (0: the default for the MO count)
+  int32_t mo_tile_g = 0, mo_tile_nw = 0;
 
   // CHAMP Jastrow, org/qmckl_jastrow_champ.org:407-488
   int64_t aord = -1, bord = -1, cord = -1, dim_c = -1, type_nucl_num = -1;
@@ -247,6 +252,11 @@ AoDevView ao_view(Context* c);  // the AO tables of c on its device
 qmckl_exit_code ao_ready(Context* c, qmckl_context handle, const char* fn);  // tables and coordinates uploaded
 qmckl_exit_code provide_ao_vgl(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, const char* fn);
+struct AoMoArgs;
+// The fused product's arguments for c's current points into d_out (tables and coordinates
+// uploaded first); mode is c's MO precision.
+qmckl_exit_code mo_product_args(qmckl_context handle, Context* c, double* d_out, const char* fn, AoMoArgs& a,
+                                int& mode);
 qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_ee_distance(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_en_distance(qmckl_context handle, Context* c, const char* fn);
@@ -295,7 +305,7 @@ hipError_t launch_split_bf16x3(const double* src, int64_t n, uint16_t* dst, int6
 // row k scaled by ao_factor[k] (zero past ao_num), b_chunk_bytes per (group, chunk); the MO
 // columns are taken G tiles of 16 at a time, ngroup groups.  C: [points][5][ldc], columns < N.
 constexpr int AO_MO_KC = 32;
-constexpr int AO_MO_MAX_G = 8;  // column tiles per pass
+constexpr int AO_MO_MAX_G = 16;  // column tiles per pass (8 per 4-wave, 16 per 8-wave workgroup)
 constexpr int AO_MO_BF_LD = 40;  // bf16 row of a BF16X3 C_t chunk: 32 AOs + 8 padding
 struct AoMoArgs {
   const double* coord;
@@ -309,12 +319,39 @@ struct AoMoArgs {
   int desc_max, ns_max, nchunk;
   const void* B;
   int b_chunk_bytes, G, ngroup;
+  int nw;  // waves per workgroup: 4 (G <= 8) or 8 (5 <= G <= 16)
   double* C;
   int64_t ldc, N;
   double cutoff;
 };
 size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode);
 hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s);
+// The default shape of the product for nct column tiles of 16 MOs: every tile in one pass of
+// 4 waves up to 8 tiles, else passes of at most 16 tiles on 8 waves.
+void ao_mo_default_shape(int64_t nct, int& G, int& nw);
+bool ao_mo_variant_ok(int mode, int G, int nw);
+int ao_mo_waves(int G);  // the default waves per workgroup for G column tiles
+// One launch over several contexts' products (AoMoArgs device array), one workgroup per item;
+// items packed as context | point tile << 16 | column group << 48, all of one (mode, G, nw).
+hipError_t launch_ao_mo_batch(int mode, int G, int nw, const AoMoArgs* d_args, const uint64_t* d_items, int nitems,
+                              size_t lds, hipStream_t s);
+// The products of a batch of contexts as one grouped launch per shape (mode, G, nw): built
+// once (device argument and item tables allocated), launched per step, freed with the batch.
+struct MoBatchLaunch {
+  int mode = 0, G = 0, nw = 0;
+  AoMoArgs* d_args = nullptr;
+  uint64_t* d_items = nullptr;
+  int nitems = 0;
+  size_t lds = 0;
+  double cost = 0.0;  // items x chunks x column tiles, for dealing the launches to streams
+};
+struct MoBatchPlan {
+  std::vector<MoBatchLaunch> launches;
+};
+hipError_t mo_batch_build(const std::vector<std::pair<AoMoArgs, int>>& in, MoBatchPlan& plan);
+// the launches dealt to the streams (n of them), the costliest first to the least loaded
+hipError_t mo_batch_launch(const MoBatchPlan& plan, const hipStream_t* s, int n);
+void mo_batch_free(MoBatchPlan& plan);
 hipError_t launch_to_f32(const double* src, int64_t n, float* dst, hipStream_t s);
 
 struct JasDevView {
@@ -340,6 +377,15 @@ struct JasOut {
   double *value, *gl;                 // [W], [W][4][Ne]
   double* scratch;                    // partial sums, size jastrow_scratch_doubles()
 };
+// One context's Jastrow evaluation as the grouped (batch) kernels take it: view, outputs,
+// scratch regions and the e-e-n tile shape.
+struct JasJob {
+  JasDevView v;
+  JasOut o;
+  double *R0, *gN, *Sg, *Jt, *JtP;
+  int ntiles, TI, JC, have_een, fused;
+};
+
 int64_t jastrow_scratch_doubles(const JasDevView& v);
 // The Jastrow view of c for walk_num walkers at the device coordinates coord
 // [3][walk_num * elec_num], on the device tables of c; false (and nothing uploaded or
@@ -353,6 +399,23 @@ JasOut jastrow_out(Context* c);
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_t s);
+// Grouped Jastrow launches over a batch of contexts (one launch per kernel and cord): built
+// once (device job tables allocated), launched per step, freed with the batch.
+struct JasBatchLaunch {
+  int kind = 0, C = 0;
+  JasJob* d_jobs = nullptr;
+  int* d_starts = nullptr;
+  int njobs = 0, nblocks = 0, threads = 0;
+  size_t lds = 0;
+};
+struct JasBatchPlan {
+  std::vector<JasBatchLaunch> launches;
+};
+hipError_t jastrow_batch_build(const std::vector<std::pair<JasDevView, JasOut>>& in, JasBatchPlan& plan);
+// the resident e-e-n kernel on s_res, everything else on s; s waits for s_res (res_done)
+// before the totals
+hipError_t jastrow_batch_launch(const JasBatchPlan& plan, hipStream_t s, hipStream_t s_res, hipEvent_t res_done);
+void jastrow_batch_free(JasBatchPlan& plan);
 // Gradient of L = sum_w [wJ J_w + wL E_w] (E_w = -1/2 sum_i (lap_i J + |grad_i J|^2) + V_ee
 // + V_en; V_nn is added by the caller) in the Jastrow parameters and nuclear coordinates,
 // per walker into out [W][nout]: [0] the walker's value, then d/da [type][aord+1], d/db

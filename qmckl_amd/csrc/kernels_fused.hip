@@ -32,6 +32,11 @@
 // every precision on the MI355X.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <tuple>
+#include <vector>
+
 #include "ao_device.hpp"
 #include "internal.hpp"
 
@@ -46,15 +51,15 @@ typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr int KC = AO_MO_KC;  // AOs per chunk
-constexpr int NWAVE = 4;      // waves per workgroup
 
 // Copies n bytes (a multiple of 16, both addresses 16-byte aligned) from global memory to
 // LDS, 16 bytes per lane, the waves of the block taking 1 KiB slices in turn.  Asynchronous:
 // the caller waits (s_waitcnt) and synchronises before reading dst.
+template <int NW>
 __device__ __forceinline__ void lds_copy(void* dst, const void* src, int nbytes, int wave, int lane) {
   const char* s = static_cast<const char*>(src);
   char* d = static_cast<char*>(dst);
-  for (int base = wave * 1024; base < nbytes; base += NWAVE * 1024) {
+  for (int base = wave * 1024; base < nbytes; base += NW * 1024) {
     const int t = base + 16 * lane;
     if (t < nbytes) __builtin_amdgcn_global_load_lds(s + t, (lds_void*)(d + base), 16, 0, 0);
   }
@@ -133,8 +138,12 @@ constexpr int TP = 16;
 constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row = 5 p + c)
 constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
 constexpr int SH = 6;          // doubles per staged (point, shell): 5 radial VGL + padding
-constexpr int QS = 64 * NWAVE / KC;  // points between the two of a thread in the AO step
-static_assert((64 * NWAVE) % KC == 0 && TP % QS == 0, "AO step: whole points per pass");
+// points between those of one thread in the AO step, with NW waves per workgroup
+template <int NW>
+constexpr int qstep() {
+  static_assert((64 * NW) % KC == 0 && TP % (64 * NW / KC) == 0, "AO step: whole points per pass");
+  return 64 * NW / KC;
+}
 constexpr int KP16 = AO_MO_BF_LD;
 // Row strides of the FP64 / FP32 operand tiles, so that the operand reads A[16 r + li][4 s
 // + lg] of a wave cover the 64 four-byte LDS banks evenly.  Doubles: lane (li, lg) starts at
@@ -210,10 +219,22 @@ __device__ __forceinline__ void tile_product(d4& acc, const void* at, const void
   }
 }
 
-template <int MODE, int G>
-__global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k_ao_mo_tile(AoMoArgs A) {
-  extern __shared__ __attribute__((aligned(16))) double lds[];
-  constexpr int NMAIN = G < NWAVE ? G : NWAVE;  // waves with a first column tile
+// Column tiles of a wave: NW = 4 waves take column tiles w and w + 4 when they exist (G <= 8),
+// except that with G = 5 the tiles (r, 4) are spread over the waves by r mod 4 (at most 7
+// accumulator tiles per wave); NW = 8 waves take column tiles w and w + 8 (G <= 16), so that
+// up to 256 MOs need one pass over the AOs.
+template <int G, int NW>
+constexpr bool second_tile() {
+  static_assert((NW == 4 && G >= 1 && G <= 8) || (NW == 8 && G >= 1 && G <= 16), "column tiles per pass");
+  return NW == 4 ? G > 5 : G > 8;
+}
+
+// Evaluates point tile `tile` x column group `group` of the product described by A.
+template <int MODE, int G, int NW>
+__device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int tile, const int group, double* lds) {
+  constexpr int NMAIN = G < NW ? G : NW;  // waves with a first column tile
+  constexpr bool TWO = second_tile<G, NW>();
+  constexpr int QS = qstep<NW>();
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // LDS: descriptor buffers [2][desc_max] (addresses computed from `lds`, so that every
@@ -222,39 +243,38 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
   int64_t* pidx = reinterpret_cast<int64_t*>(xt + 3 * TP);  // [TP] point of each tile row
   double* sh = xt + 4 * TP;                          // [TP][ns_max][SH]
   void* at = sh + TP * A.ns_max * SH;                // operand tile (TileA<MODE>)
-  const int group = blockIdx.y;
-  const int64_t tp0 = A.p0 + (int64_t)blockIdx.x * TP;  // first position p' of the tile
+  const int64_t tp0 = A.p0 + (int64_t)tile * TP;  // first position p' of the tile
   const int ntp = (int)min((int64_t)TP, A.p0 + A.np - tp0);
   const char* bsrc = static_cast<const char*>(A.B) + (int64_t)group * A.nchunk * A.b_chunk_bytes;
-  const bool has1 = wave < NMAIN, has2 = G > 5 && wave + 4 < G;
+  const bool has1 = wave < NMAIN, has2 = TWO && wave + NW < G;
   if (tid < TP) {
     // position p' = i tw + w holds point p = w te + i
     const int64_t pp = tp0 + tid, i = pp / A.tw;
     pidx[tid] = tid < ntp ? (pp - i * A.tw) * A.te + i : 0;
   }
   __syncthreads();
-  for (int t = tid; t < TP * 3; t += 64 * NWAVE) {
+  for (int t = tid; t < TP * 3; t += 64 * NW) {
     const int q = t / 3, d = t - 3 * q;
     xt[t] = q < ntp ? A.coord[pidx[q] + d * A.coord_ld] : 0.0;
   }
-  // column tile `wave` (all row tiles); then column tile wave + 4 (G > 5) or the tiles
-  // (wave, 4), (wave + 4, 4) (G = 5)
-  d4 acc1[RT], acc2[G > 5 ? RT : 2];
+  // column tile `wave` (all row tiles); then column tile wave + NW (TWO) or the tiles
+  // (wave, 4), (wave + 4, 4) (NW = 4, G = 5)
+  d4 acc1[RT], acc2[TWO ? RT : 2];
 #pragma unroll
   for (int r = 0; r < RT; ++r) acc1[r] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int r = 0; r < (G > 5 ? RT : 2); ++r) acc2[r] = d4{0.0, 0.0, 0.0, 0.0};
-  lds_copy(lds, A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
+  for (int r = 0; r < (TWO ? RT : 2); ++r) acc2[r] = d4{0.0, 0.0, 0.0, 0.0};
+  lds_copy<NW>(lds, A.desc + A.desc_off[0], 8 * (A.desc_off[1] - A.desc_off[0]), wave, lane);
   for (int ch = 0; ch < A.nchunk; ++ch) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();  // descriptor ch in LDS; every wave done with chunk ch - 1
     if (ch + 1 < A.nchunk)
-      lds_copy(lds + ((ch + 1) & 1) * A.desc_max, A.desc + A.desc_off[ch + 1],
+      lds_copy<NW>(lds + ((ch + 1) & 1) * A.desc_max, A.desc + A.desc_off[ch + 1],
                8 * (A.desc_off[ch + 2] - A.desc_off[ch + 1]), wave, lane);
     const Chunk c = chunk_view(lds + (ch & 1) * A.desc_max);
     // 1. radial VGL of the chunk's shells at the 16 points
     int live = 0;
-    for (int t = tid; t < TP * c.ns; t += 64 * NWAVE) {
+    for (int t = tid; t < TP * c.ns; t += 64 * NW) {
       const int q = t / c.ns, s = t - q * c.ns;
       const int nl = (int)c.shl[2 * s] & 255;
       const double x = xt[3 * q] - c.nuc[4 * nl], y = xt[3 * q + 1] - c.nuc[4 * nl + 1],
@@ -305,11 +325,11 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
     if (has1)
 #pragma unroll
       for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc1[r], at, bch, r, wave, li, lg);
-    if constexpr (G > 5) {
+    if constexpr (TWO) {
       if (has2)
 #pragma unroll
-        for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc2[r], at, bch, r, wave + 4, li, lg);
-    } else if constexpr (G == 5) {
+        for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc2[r], at, bch, r, wave + NW, li, lg);
+    } else if constexpr (NW == 4 && G == 5) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
         if (wave + 4 * x < RT) tile_product<MODE, G>(acc2[x], at, bch, wave + 4 * x, 4, li, lg);
@@ -329,30 +349,52 @@ __global__ void __launch_bounds__(64 * NWAVE, G > 5 ? 2 : (MODE == 0 ? 3 : 4)) k
   if (has1)
 #pragma unroll
     for (int r = 0; r < RT; ++r) store(acc1[r], r, wave);
-  if constexpr (G > 5) {
+  if constexpr (TWO) {
     if (has2)
 #pragma unroll
-      for (int r = 0; r < RT; ++r) store(acc2[r], r, wave + 4);
-  } else if constexpr (G == 5) {
+      for (int r = 0; r < RT; ++r) store(acc2[r], r, wave + NW);
+  } else if constexpr (NW == 4 && G == 5) {
 #pragma unroll
     for (int x = 0; x < 2; ++x)
       if (wave + 4 * x < RT) store(acc2[x], wave + 4 * x, 4);
   }
 }
 
-template <class K>
-hipError_t launch_kernel(K kernel, const AoMoArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
-  return hipGetLastError();
+// Occupancy bound of a variant: two waves per SIMD when a wave keeps two column tiles (or
+// has eight waves), else three (FP64) or four.
+template <int MODE, int G, int NW>
+constexpr int ao_mo_min_blocks() {
+  return NW == 8 ? 1 : (G > 5 ? 2 : (MODE == 0 ? 3 : 4));
 }
 
-template <int MODE, int G>
-hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
-  // dynamic LDS up to the CU's 160 KiB less the kernel's static LDS (the block-wide OR of
-  // the chunk screening uses a little)
-  static size_t max_dyn = 0;
+template <int MODE, int G, int NW>
+__global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>())) k_ao_mo_tile(AoMoArgs A) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  ao_mo_tile_body<MODE, G, NW>(A, blockIdx.x, blockIdx.y, lds);
+}
+
+// One launch over the products of several contexts (a batch of molecules): workgroup b
+// evaluates item b of `items`, (context, point tile, column group) packed as context |
+// tile << 16 | group << 48 (context and group < 2^16, tile < 2^32).  The host orders the
+// items by decreasing cost, so that the hardware dispatcher, which hands out workgroups in
+// order as CUs free up, leaves the cheapest items for the end.
+template <int MODE, int G, int NW>
+__global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>()))
+    k_ao_mo_batch(const AoMoArgs* __restrict__ args, const uint64_t* __restrict__ items, int nitems) {
+  extern __shared__ __attribute__((aligned(16))) double lds[];
+  const uint64_t it = items[blockIdx.x];
+  const int m = __builtin_amdgcn_readfirstlane((int)(it & 0xffffu));
+  const int tile = __builtin_amdgcn_readfirstlane((int)((it >> 16) & 0xffffffffu));
+  const int group = __builtin_amdgcn_readfirstlane((int)(it >> 48));
+  ao_mo_tile_body<MODE, G, NW>(args[m], tile, group, lds);
+}
+
+// Dynamic LDS limit of a kernel: the CU's 160 KiB less its static LDS (the block-wide OR of
+// the chunk screening uses a little), set once per kernel.
+template <class K>
+hipError_t prepare_lds(K kernel, size_t lds, size_t& max_dyn) {
   if (max_dyn == 0) {
-    const void* k = reinterpret_cast<const void*>(&k_ao_mo_tile<MODE, G>);
+    const void* k = reinterpret_cast<const void*>(kernel);
     hipFuncAttributes fa;
     hipError_t e = hipFuncGetAttributes(&fa, k);
     if (e != hipSuccess) return e;
@@ -361,25 +403,65 @@ hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
     if (e != hipSuccess) return e;
     max_dyn = dyn;
   }
-  if (lds > max_dyn) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
-  return launch_kernel(k_ao_mo_tile<MODE, G>, a, grid, dim3(64 * NWAVE), lds, s);
+  return lds > max_dyn ? hipErrorInvalidValue : hipSuccess;
 }
 
-template <int MODE>
-hipError_t launch_mode(const AoMoArgs& a, size_t lds, hipStream_t s) {
-  switch (a.G) {
-    case 1: return launch_g<MODE, 1>(a, lds, s);
-    case 2: return launch_g<MODE, 2>(a, lds, s);
-    case 3: return launch_g<MODE, 3>(a, lds, s);
-    case 4: return launch_g<MODE, 4>(a, lds, s);
-    case 5: return launch_g<MODE, 5>(a, lds, s);
-    case 6: return launch_g<MODE, 6>(a, lds, s);
-    case 7: return launch_g<MODE, 7>(a, lds, s);
-    case 8: return launch_g<MODE, 8>(a, lds, s);
-    default: return hipErrorInvalidValue;
-  }
+template <int MODE, int G, int NW>
+hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
+  static size_t max_dyn = 0;
+  hipError_t e = prepare_lds(&k_ao_mo_tile<MODE, G, NW>, lds, max_dyn);
+  if (e != hipSuccess) return e;
+  const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
+  hipLaunchKernelGGL((k_ao_mo_tile<MODE, G, NW>), grid, dim3(64 * NW), lds, s, a);
+  return hipGetLastError();
 }
+
+template <int MODE, int G, int NW>
+hipError_t launch_batch_g(const AoMoArgs* d_args, const uint64_t* d_items, int nitems, size_t lds, hipStream_t s) {
+  static size_t max_dyn = 0;
+  hipError_t e = prepare_lds(&k_ao_mo_batch<MODE, G, NW>, lds, max_dyn);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_ao_mo_batch<MODE, G, NW>), dim3((unsigned)nitems), dim3(64 * NW), lds, s, d_args, d_items,
+                     nitems);
+  return hipGetLastError();
+}
+
+// Calls F::template run<MODE, G, NW>() for a runtime (mode, G, NW) among the built variants:
+// NW = 4 for G = 1..8, NW = 8 for G = 5..16.
+template <class F>
+hipError_t dispatch_variant(int mode, int G, int NW, F&& f) {
+#define AO_MO_V(M, GG, W) \
+  if (mode == M && G == GG && NW == W) return f.template run<M, GG, W>();
+#define AO_MO_MODE(M)                                                                                    \
+  AO_MO_V(M, 1, 4) AO_MO_V(M, 2, 4) AO_MO_V(M, 3, 4) AO_MO_V(M, 4, 4) AO_MO_V(M, 5, 4) AO_MO_V(M, 6, 4)  \
+  AO_MO_V(M, 7, 4) AO_MO_V(M, 8, 4) AO_MO_V(M, 5, 8) AO_MO_V(M, 6, 8) AO_MO_V(M, 7, 8) AO_MO_V(M, 8, 8)  \
+  AO_MO_V(M, 9, 8) AO_MO_V(M, 10, 8) AO_MO_V(M, 11, 8) AO_MO_V(M, 12, 8) AO_MO_V(M, 13, 8)             \
+  AO_MO_V(M, 14, 8) AO_MO_V(M, 15, 8) AO_MO_V(M, 16, 8)
+  AO_MO_MODE(0)
+  AO_MO_MODE(1)
+  AO_MO_MODE(2)
+#undef AO_MO_MODE
+#undef AO_MO_V
+  return hipErrorInvalidValue;
+}
+
+struct RunOne {
+  const AoMoArgs& a;
+  size_t lds;
+  hipStream_t s;
+  template <int M, int G, int NW>
+  hipError_t run() { return launch_g<M, G, NW>(a, lds, s); }
+};
+
+struct RunBatch {
+  const AoMoArgs* args;
+  const uint64_t* items;
+  int n;
+  size_t lds;
+  hipStream_t s;
+  template <int M, int G, int NW>
+  hipError_t run() { return launch_batch_g<M, G, NW>(args, items, n, lds, s); }
+};
 
 }  // namespace
 
@@ -388,16 +470,105 @@ size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
          (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
 }
 
+bool ao_mo_variant_ok(int mode, int G, int NW) {
+  if (mode < 0 || mode > 2) return false;
+  return (NW == 4 && G >= 1 && G <= 8) || (NW == 8 && G >= 5 && G <= 16);
+}
+
 hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s) {
   if (a.np <= 0) return hipSuccess;
-  if (a.G < 1 || a.G > AO_MO_MAX_G || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
+  if (!ao_mo_variant_ok(mode, a.G, a.nw) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
       a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te)
     return hipErrorInvalidValue;
   const size_t lds = ao_mo_lds_bytes(a, mode);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  if (mode == 0) return launch_mode<0>(a, lds, s);
-  if (mode == 1) return launch_mode<1>(a, lds, s);
-  return launch_mode<2>(a, lds, s);
+  RunOne r{a, lds, s};
+  return dispatch_variant(mode, a.G, a.nw, r);
+}
+
+hipError_t launch_ao_mo_batch(int mode, int G, int NW, const AoMoArgs* d_args, const uint64_t* d_items, int nitems,
+                              size_t lds, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (!ao_mo_variant_ok(mode, G, NW) || lds > 160 * 1024) return hipErrorInvalidValue;
+  RunBatch r{d_args, d_items, nitems, lds, s};
+  return dispatch_variant(mode, G, NW, r);
+}
+
+int ao_mo_waves(int G) { return G > 8 ? 8 : 4; }
+
+hipError_t mo_batch_build(const std::vector<std::pair<AoMoArgs, int>>& in, MoBatchPlan& plan) {
+  mo_batch_free(plan);
+  // contexts grouped by shape; within a shape, the costliest contexts' items first (an item
+  // costs about its context's number of 32-AO chunks)
+  std::map<std::tuple<int, int, int>, std::vector<int>> by_shape;
+  for (int i = 0; i < (int)in.size(); ++i) {
+    const AoMoArgs& a = in[i].first;
+    const int mode = in[i].second;
+    if (a.np <= 0) continue;
+    if (!ao_mo_variant_ok(mode, a.G, a.nw) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
+        a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te ||
+        ao_mo_lds_bytes(a, mode) > 160 * 1024)
+      return hipErrorInvalidValue;
+    by_shape[std::make_tuple(mode, a.G, a.nw)].push_back(i);
+  }
+  hipError_t e = hipSuccess;
+  for (auto& kv : by_shape) {
+    std::vector<int>& ids = kv.second;
+    if (ids.size() > 65535) return hipErrorInvalidValue;
+    std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return in[x].first.nchunk > in[y].first.nchunk; });
+    std::vector<AoMoArgs> args;
+    std::vector<uint64_t> items;
+    MoBatchLaunch l;
+    l.mode = std::get<0>(kv.first);
+    l.G = std::get<1>(kv.first);
+    l.nw = std::get<2>(kv.first);
+    for (size_t m = 0; m < ids.size(); ++m) {
+      const AoMoArgs& a = in[ids[m]].first;
+      args.push_back(a);
+      l.lds = std::max(l.lds, ao_mo_lds_bytes(a, l.mode));
+      const int64_t ntile = (a.np + TP - 1) / TP;
+      if (ntile > 0xffffffffLL || a.ngroup > 65535) return hipErrorInvalidValue;
+      for (int64_t t = 0; t < ntile; ++t)
+        for (int g = 0; g < a.ngroup; ++g) items.push_back((uint64_t)m | ((uint64_t)t << 16) | ((uint64_t)g << 48));
+      l.cost += (double)ntile * a.ngroup * a.nchunk * a.G;
+    }
+    if (items.size() > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    l.nitems = (int)items.size();
+    e = hipMalloc(reinterpret_cast<void**>(&l.d_args), sizeof(AoMoArgs) * args.size());
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&l.d_items), sizeof(uint64_t) * items.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(l.d_args, args.data(), sizeof(AoMoArgs) * args.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(l.d_items, items.data(), sizeof(uint64_t) * items.size(), hipMemcpyHostToDevice);
+    plan.launches.push_back(l);
+    if (e != hipSuccess) break;
+  }
+  if (e != hipSuccess) mo_batch_free(plan);
+  return e;
+}
+
+hipError_t mo_batch_launch(const MoBatchPlan& plan, const hipStream_t* s, int n) {
+  std::vector<int> order(plan.launches.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return plan.launches[x].cost > plan.launches[y].cost; });
+  std::vector<double> load(n, 0.0);
+  for (int i : order) {
+    const MoBatchLaunch& l = plan.launches[i];
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[k] += l.cost;
+    const hipError_t e = launch_ao_mo_batch(l.mode, l.G, l.nw, l.d_args, l.d_items, l.nitems, l.lds, s[k]);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+void mo_batch_free(MoBatchPlan& plan) {
+  for (MoBatchLaunch& l : plan.launches) {
+    if (l.d_args) (void)hipFree(l.d_args);
+    if (l.d_items) (void)hipFree(l.d_items);
+  }
+  plan.launches.clear();
 }
 
 }  // namespace qmckl_amd

@@ -25,6 +25,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
+#include <map>
+#include <vector>
 #include <type_traits>
 
 #include "exp_neg.hpp"
@@ -120,6 +123,13 @@ __device__ __forceinline__ double wave_sum64(double x) {
   x = row_sum16(x);
   x = swap_sum_f64<true>(x, x);
   return swap_sum_f64<false>(x, x);
+}
+
+// A wave-uniform double in scalar registers (its two halves through readfirstlane).
+__device__ __forceinline__ double uniform_f64(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u), hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // Deterministic sum over the lanes of a block of any multiple of 64 threads: a fixed
@@ -303,13 +313,12 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
 // are held in registers.  The block has 64 ceil(Ne / 64) threads (at most pair_block(C)), one
 // electron i per thread.
 template <int C>
-__global__ void __launch_bounds__(pair_block(C)) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
-                                                     double* __restrict__ gN, double* __restrict__ Sg) {
+__device__ __forceinline__ void jas_pair_body(const JasDevView& v, const JasOut& o, double* __restrict__ R0,
+                                              double* __restrict__ gN, double* __restrict__ Sg, const int64_t w) {
   constexpr int K1 = C >= 2 ? C - 1 : 0;
   extern __shared__ double lds[];
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
   const int NAg = gn_na(Nn), JPg = gn_jp(Ne);
-  const int64_t w = blockIdx.x;
   double* xe = lds;            // [Ne][3]
   double* red = lds + 3 * Ne;  // [waves]
   const int nth = (int)blockDim.x;
@@ -401,20 +410,26 @@ __global__ void __launch_bounds__(pair_block(C)) k_jas_pair(JasDevView v, JasOut
   }
 }
 
+template <int C>
+__global__ void __launch_bounds__(pair_block(C)) k_jas_pair(JasDevView v, JasOut o, double* __restrict__ R0,
+                                                     double* __restrict__ gN, double* __restrict__ Sg) {
+  jas_pair_body<C>(v, o, R0, gN, Sg, blockIdx.x);
+}
+
 // ------------------------------------------------------------------------------------
 // e-e-n term.  Tile: TI electrons x all nuclei; j walked in chunks of JC.
 // Jt scratch: [W][ntiles] partial J_eeN.
 // ------------------------------------------------------------------------------------
 template <int C>
-__global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, const double* __restrict__ R0,
-                                                    const double* __restrict__ gN, double* __restrict__ Jt, int TI,
-                                                    int JC, int ntiles) {
+__device__ __forceinline__ void jas_een_body(const JasDevView& v, const JasOut& o, const double* __restrict__ R0,
+                                             const double* __restrict__ gN, double* __restrict__ Jt, int TI, int JC,
+                                             int ntiles, const int64_t b) {
   constexpr int NQ = C;  // q = 1..C
   constexpr Lkm<C> T{};
   extern __shared__ double lds[];
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
-  const int64_t w = blockIdx.x / ntiles;
-  const int tile = blockIdx.x - (int)(w * ntiles);
+  const int64_t w = b / ntiles;
+  const int tile = (int)(b - w * ntiles);
   const int i0 = tile * TI;
   const int ti = min(TI, Ne - i0);
   // LDS layout
@@ -598,6 +613,13 @@ __global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, cons
     for (int p = 0; p < npairs; ++p) s += red[5 * p];
     Jt[w * ntiles + tile] = s;
   }
+}
+
+template <int C>
+__global__ void __launch_bounds__(kBlock) k_jas_een(JasDevView v, JasOut o, const double* __restrict__ R0,
+                                                    const double* __restrict__ gN, double* __restrict__ Jt, int TI,
+                                                    int JC, int ntiles) {
+  jas_een_body<C>(v, o, R0, gN, Jt, TI, JC, ntiles, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -798,11 +820,9 @@ struct EenMfmaLds {
 };
 
 template <int C>
-__global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView v, JasOut o,
-                                                                     const double* __restrict__ R0,
-                                                                     const double* __restrict__ gN,
-                                                                     const double* __restrict__ Sg,
-                                                                     double* __restrict__ Jt, int ntiles, int JS) {
+__device__ __forceinline__ void jas_een_mfma_body(const JasDevView& v, const JasOut& o, const double* __restrict__ R0,
+                                                  const double* __restrict__ gN, const double* __restrict__ Sg,
+                                                  double* __restrict__ Jt, int ntiles, int JS, const int64_t b) {
   constexpr int K1 = C - 1;
   constexpr int NW = EenWaves<C>::n;
   constexpr int NT = 64 * NW;
@@ -824,7 +844,7 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
   int64_t w;
   int tile;
-  een_block_work(blockIdx.x, v.walk_num, ntiles, w, tile);
+  een_block_work(b, v.walk_num, ntiles, w, tile);
   const int i0 = tile * EM;
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (k, q) role
@@ -986,6 +1006,15 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView
     for (int il = 0; il < EM; ++il) sj += jrow[il];
     Jt[w * ntiles + tile] = sj;
   }
+}
+
+template <int C>
+__global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma(JasDevView v, JasOut o,
+                                                                     const double* __restrict__ R0,
+                                                                     const double* __restrict__ gN,
+                                                                     const double* __restrict__ Sg,
+                                                                     double* __restrict__ Jt, int ntiles, int JS) {
+  jas_een_mfma_body<C>(v, o, R0, gN, Sg, Jt, ntiles, JS, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1513,16 +1542,15 @@ __device__ __forceinline__ void een_butterfly(double (&vec)[N], int l32) {
 // per walker instead of once per tile, the constant ones (nucleus table, c-coefficients)
 // once per workgroup, and no workgroup launch separates two tiles.
 template <int C>
-__global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, double* __restrict__ Jt,
-                                                              double* __restrict__ JtP, int ntiles) {
+__device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasOut& o, double* __restrict__ Jt,
+                                                 double* __restrict__ JtP, int ntiles, const int bx, const int gx) {
   constexpr int K1 = C - 1;
   constexpr int NT = 64 * EQ_WAVES;
   constexpr int DIM = Lkm<C>::dim;
   extern __shared__ double lds[];
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int nitem = (int)v.walk_num * ntiles;  // < 2^31 (checked by the launcher)
-  const int it0 = (int)((int64_t)nitem * blockIdx.x / gridDim.x),
-            it1 = (int)((int64_t)nitem * (blockIdx.x + 1) / gridDim.x);
+  const int it0 = (int)((int64_t)nitem * bx / gx), it1 = (int)((int64_t)nitem * (bx + 1) / gx);
   constexpr int NPR = een_npair(C), NSG = een_nsingle(C);
   {
     // Once per workgroup (ready at the first item's barrier): the constant inputs, and the
@@ -1593,7 +1621,9 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
     // The sizes and the thread index pass through an empty asm statement, so that what
     // derives from them (LDS offsets, bounds, lane offsets) is recomputed per item instead
     // of being held in registers across the loop.
-    int Ne = (int)v.elec_num, Nn = (int)v.nucl_num, tidl = threadIdx.x;
+    // (readfirstlane: the view may come from a job table in memory)
+    int Ne = __builtin_amdgcn_readfirstlane((int)v.elec_num), Nn = __builtin_amdgcn_readfirstlane((int)v.nucl_num),
+        tidl = threadIdx.x;
     asm volatile("" : "+s"(Ne), "+s"(Nn), "+v"(tidl));
     const int tid = tidl, lane = tid & 63, li = lane & 15, lg = lane >> 4;
     const int JP = gn_jp(Ne), NA = gn_na(Nn), NA16 = 16 * NA;
@@ -1672,7 +1702,7 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
       // lane index and kappa through an empty asm statement: what derives from them lives
       // in this phase only
       int tp = threadIdx.x;
-      double kee = v.rescale_ee;
+      double kee = uniform_f64(v.rescale_ee);
       asm volatile("" : "+v"(tp), "+s"(kee));
       const int il = 2 * wave + ((tp & 63) >> 5), l32 = tp & 31, i = i0 + il;
       const double kinv = 1.0 / kee;
@@ -1837,12 +1867,17 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, Jas
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, const double* __restrict__ Jt,
-                                                      const double* __restrict__ JtP, int ntiles, int have_een,
-                                                      int fused) {
+template <int C>
+__global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, double* __restrict__ Jt,
+                                                              double* __restrict__ JtP, int ntiles) {
+  jas_een_res_body<C>(v, o, Jt, JtP, ntiles, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void jas_total_body(const JasDevView& v, const JasOut& o, const double* __restrict__ Jt,
+                                               const double* __restrict__ JtP, int ntiles, int have_een, int fused,
+                                               const int64_t w) {
   __shared__ double sval;
   const int Ne = (int)v.elec_num;
-  const int64_t w = blockIdx.x;
   if (threadIdx.x == 0) {
     double een = 0.0;
     if (have_een)
@@ -1879,6 +1914,74 @@ __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, co
 #pragma unroll
     for (int c = 0; c < 4; ++c) g[c * Ne + i] = s[c] * val;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, const double* __restrict__ Jt,
+                                                      const double* __restrict__ JtP, int ntiles, int have_een,
+                                                      int fused) {
+  jas_total_body(v, o, Jt, JtP, ntiles, have_een, fused, blockIdx.x);
+}
+
+// ------------------------------------------------------------------------------------
+// Grouped launches over the Jastrow evaluations of several contexts (a batch of molecules):
+// one launch per kernel (and cord), in which workgroup b serves job m, the one whose block
+// range [starts[m], starts[m + 1]) holds b, as that job's workgroup b - starts[m] of
+// starts[m + 1] - starts[m].  The job's view and outputs are read from the job table in
+// place (wave-uniform loads; a local copy of the view would live in scratch memory).
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int job_of(const int* __restrict__ starts, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <int C>
+__global__ void __launch_bounds__(pair_block(C)) k_jas_pair_batch(const JasJob* __restrict__ jobs,
+                                                           const int* __restrict__ starts, int njobs) {
+  const int m = job_of(starts, njobs, blockIdx.x);
+  const JasJob& J = jobs[m];
+  jas_pair_body<C>(J.v, J.o, J.R0, J.gN, J.Sg, (int64_t)blockIdx.x - starts[m]);
+}
+
+template <int C>
+__global__ void __launch_bounds__(kBlock) k_jas_een_batch(const JasJob* __restrict__ jobs,
+                                                          const int* __restrict__ starts, int njobs) {
+  const int m = job_of(starts, njobs, blockIdx.x);
+  const JasJob& J = jobs[m];
+  jas_een_body<C>(J.v, J.o, J.R0, J.gN, J.Jt, J.TI, J.JC, J.ntiles, (int64_t)blockIdx.x - starts[m]);
+}
+
+template <int C>
+__global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma_batch(const JasJob* __restrict__ jobs,
+                                                                           const int* __restrict__ starts,
+                                                                           int njobs) {
+  const int m = job_of(starts, njobs, blockIdx.x);
+  const JasJob& J = jobs[m];
+  jas_een_mfma_body<C>(J.v, J.o, J.R0, J.gN, J.Sg, J.Jt, J.ntiles, J.JC, (int64_t)blockIdx.x - starts[m]);
+}
+
+// The persistent resident kernel over several contexts: each context gets its own share of
+// the workgroups (one per CU), in proportion to its work, and runs its (walker, tile) items
+// on them as a launch of its own would.
+template <int C>
+__global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res_batch(const JasJob* __restrict__ jobs,
+                                                                    const int* __restrict__ starts, int njobs) {
+  const int m = job_of(starts, njobs, blockIdx.x);
+  const JasJob& J = jobs[m];
+  jas_een_res_body<C>(J.v, J.o, J.Jt, J.JtP, J.ntiles, (int)blockIdx.x - starts[m], starts[m + 1] - starts[m]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_jas_total_batch(const JasJob* __restrict__ jobs,
+                                                            const int* __restrict__ starts, int njobs) {
+  const int m = job_of(starts, njobs, blockIdx.x);
+  const JasJob& J = jobs[m];
+  jas_total_body(J.v, J.o, J.Jt, J.JtP, J.ntiles, J.have_een, J.fused, (int64_t)blockIdx.x - starts[m]);
 }
 
 struct EenShape {
@@ -2269,6 +2372,221 @@ hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_
   hipLaunchKernelGGL(k_jas_total, dim3((unsigned)v.walk_num), dim3(kBlock), 0, s, v, o, sv.Jt, sv.JtP, sh.ntiles,
                      have_een, een_fused(v) ? 1 : 0);
   return hipGetLastError();
+}
+
+// ---- grouped launches of a batch of contexts -------------------------------------------
+namespace {
+
+enum { JB_PAIR = 0, JB_EEN_VALU = 1, JB_EEN_MFMA = 2, JB_EEN_RES = 3, JB_TOTAL = 4 };
+
+template <class K>
+hipError_t allow_lds(K kernel) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             160 * 1024);
+}
+
+template <int C>
+hipError_t launch_batch_c(const JasBatchLaunch& l, hipStream_t s) {
+  const dim3 g((unsigned)l.nblocks), b((unsigned)l.threads);
+  hipError_t e = hipSuccess;
+  switch (l.kind) {
+    case JB_PAIR:
+      if ((e = allow_lds(&k_jas_pair_batch<C>)) != hipSuccess) return e;
+      hipLaunchKernelGGL(k_jas_pair_batch<C>, g, b, l.lds, s, l.d_jobs, l.d_starts, l.njobs);
+      break;
+    case JB_EEN_VALU:
+      if constexpr (C >= 2) {
+        if ((e = allow_lds(&k_jas_een_batch<C>)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_jas_een_batch<C>, g, b, l.lds, s, l.d_jobs, l.d_starts, l.njobs);
+        break;
+      }
+      return hipErrorInvalidValue;
+    case JB_EEN_MFMA:
+      if constexpr (C >= 2 && C <= EMFMA_MAX_C) {
+        if ((e = allow_lds(&k_jas_een_mfma_batch<C>)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_jas_een_mfma_batch<C>, g, b, l.lds, s, l.d_jobs, l.d_starts, l.njobs);
+        break;
+      }
+      return hipErrorInvalidValue;
+    case JB_EEN_RES:
+      if constexpr (C >= 2 && C <= EMFMA_MAX_C) {
+        if ((e = allow_lds(&k_jas_een_res_batch<C>)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_jas_een_res_batch<C>, g, b, l.lds, s, l.d_jobs, l.d_starts, l.njobs);
+        break;
+      }
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Workgroups of the resident kernel per job: about kResBlocksPerCu per CU in all, shared in
+// proportion to the jobs' work, each job's count a divisor of its walker count (at least
+// one) so that every workgroup takes whole walkers (the walker's tables are built once per
+// workgroup).  Many workgroups, not one per CU: the batch's other kernels run beside this
+// one, and the dispatcher balances workgroups as CUs free up where a static share would not.
+constexpr int kResBlocksPerCu = 4;
+std::vector<int> res_shares(const std::vector<double>& cost, const std::vector<int64_t>& walkers, int ncu) {
+  const int n = (int)cost.size();
+  double sum = 0.0;
+  for (double c : cost) sum += c;
+  std::vector<int> sh(n, 1);
+  for (int m = 0; m < n; ++m) {
+    const double x = sum > 0 ? (double)kResBlocksPerCu * ncu * cost[m] / sum : 1.0;
+    const int64_t W = walkers[m];
+    int64_t best = 1;
+    for (int64_t d = 1; d <= W && d <= (int64_t)std::ceil(x); ++d)
+      if (W % d == 0) best = d;
+    sh[m] = (int)best;
+  }
+  return sh;
+}
+
+}  // namespace
+
+hipError_t jastrow_batch_build(const std::vector<std::pair<JasDevView, JasOut>>& in, JasBatchPlan& plan) {
+  jastrow_batch_free(plan);
+  // family key (kind, C) -> jobs, block counts, threads, LDS
+  struct Fam {
+    std::vector<JasJob> jobs;
+    std::vector<int64_t> blocks, walkers;
+    std::vector<double> cost;
+    int threads = 0;
+    size_t lds = 0;
+  };
+  std::map<std::pair<int, int>, Fam> fam;
+  for (const auto& vo : in) {
+    const JasDevView& v = vo.first;
+    const JasOut& o = vo.second;
+    const ScratchView sv = scratch_view(v, o);
+    const EenShape sh = een_shape(v);
+    const int C = (int)v.cord;
+    const bool have_een = C >= 2 && v.dim_c > 0, fused = een_fused(v);
+    JasJob J;
+    J.v = v;
+    J.o = o;
+    J.R0 = sv.R0;
+    J.gN = sv.gN;
+    J.Sg = sv.Sg;
+    J.Jt = sv.Jt;
+    J.JtP = sv.JtP;
+    J.ntiles = sh.ntiles;
+    J.TI = sh.TI;
+    J.JC = sh.JC;
+    J.have_een = have_een ? 1 : 0;
+    J.fused = fused ? 1 : 0;
+    const int64_t W = v.walk_num, Ne = v.elec_num;
+    auto add = [&](int kind, int64_t blocks, int threads, size_t lds, double cost) {
+      Fam& f = fam[{kind, kind == JB_TOTAL ? 0 : C}];
+      f.jobs.push_back(J);
+      f.blocks.push_back(blocks);
+      f.walkers.push_back(W);
+      f.cost.push_back(cost);
+      f.threads = std::max(f.threads, threads);
+      f.lds = std::max(f.lds, lds);
+    };
+    if (!fused) {
+      const size_t lds_pair = sizeof(double) * (3 * Ne + kPairBlock / 64);
+      if (lds_pair > 160 * 1024) return hipErrorInvalidValue;
+      add(JB_PAIR, W, (int)std::min<int64_t>(pair_block(C), 64 * ((Ne + 63) / 64)), lds_pair, 0.0);
+    }
+    if (have_een) {
+      if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
+      if (fused)
+        add(JB_EEN_RES, 0, 64 * EQ_WAVES, sh.lds,
+            (double)W * sh.ntiles * gn_jp(Ne) * 16.0 * gn_na(v.nucl_num));
+      else if (sh.mfma)
+        add(JB_EEN_MFMA, W * sh.ntiles, 64 * (C * (C - 1) / 2), sh.lds, 0.0);
+      else
+        add(JB_EEN_VALU, W * sh.ntiles, kBlock, sh.lds, 0.0);
+    }
+    add(JB_TOTAL, W, kBlock, 0, 0.0);
+  }
+  int ncu = 256, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  hipError_t e = hipSuccess;
+  for (auto& kv : fam) {  // in (kind, C) order: pair, e-e-n kernels, total
+    Fam& f = kv.second;
+    const int n = (int)f.jobs.size();
+    if (kv.first.first == JB_EEN_RES) {
+      const std::vector<int> sh = res_shares(f.cost, f.walkers, ncu);
+      for (int m = 0; m < n; ++m) f.blocks[m] = sh[m];
+    }
+    std::vector<int> starts(n + 1, 0);
+    int64_t acc = 0;
+    for (int m = 0; m < n; ++m) {
+      starts[m] = (int)acc;
+      acc += f.blocks[m];
+    }
+    if (acc > INT32_MAX) return hipErrorInvalidValue;
+    starts[n] = (int)acc;
+    JasBatchLaunch l;
+    l.kind = kv.first.first;
+    l.C = kv.first.second;
+    l.njobs = n;
+    l.nblocks = (int)acc;
+    l.threads = f.threads;
+    l.lds = f.lds;
+    e = hipMalloc(reinterpret_cast<void**>(&l.d_jobs), sizeof(JasJob) * n);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&l.d_starts), sizeof(int) * (n + 1));
+    if (e == hipSuccess) e = hipMemcpy(l.d_jobs, f.jobs.data(), sizeof(JasJob) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(l.d_starts, starts.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+    plan.launches.push_back(l);
+    if (e != hipSuccess) break;
+  }
+  if (e != hipSuccess) jastrow_batch_free(plan);
+  return e;
+}
+
+hipError_t jastrow_batch_launch(const JasBatchPlan& plan, hipStream_t s, hipStream_t s_res, hipEvent_t res_done) {
+  // the resident e-e-n kernel on s_res (it needs nothing from the other kernels); the pair
+  // and other e-e-n kernels on s; the totals on s once s_res is done
+  hipError_t e = hipSuccess;
+  bool res = false;
+  for (const JasBatchLaunch& l : plan.launches) {
+    if (l.nblocks <= 0) continue;
+    hipStream_t st = l.kind == JB_EEN_RES ? s_res : s;
+    if (l.kind == JB_EEN_RES) res = true;
+    if (l.kind == JB_TOTAL) {
+      if (res) {
+        e = hipEventRecord(res_done, s_res);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, res_done, 0);
+        if (e != hipSuccess) return e;
+        res = false;
+      }
+      hipLaunchKernelGGL(k_jas_total_batch, dim3((unsigned)l.nblocks), dim3(kBlock), 0, st, l.d_jobs, l.d_starts,
+                         l.njobs);
+      e = hipGetLastError();
+    } else {
+      switch (l.C) {
+        case 0: e = launch_batch_c<0>(l, st); break;
+        case 1: e = launch_batch_c<1>(l, st); break;
+        case 2: e = launch_batch_c<2>(l, st); break;
+        case 3: e = launch_batch_c<3>(l, st); break;
+        case 4: e = launch_batch_c<4>(l, st); break;
+        case 5: e = launch_batch_c<5>(l, st); break;
+        case 6: e = launch_batch_c<6>(l, st); break;
+        case 7: e = launch_batch_c<7>(l, st); break;
+        case 8: e = launch_batch_c<8>(l, st); break;
+        default: e = hipErrorInvalidValue;
+      }
+    }
+    if (e != hipSuccess) return e;
+  }
+  if (res) {  // no totals launched (cannot happen: every job has one); join anyway
+    e = hipEventRecord(res_done, s_res);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, res_done, 0);
+  }
+  return e;
+}
+
+void jastrow_batch_free(JasBatchPlan& plan) {
+  for (JasBatchLaunch& l : plan.launches) {
+    if (l.d_jobs) (void)hipFree(l.d_jobs);
+    if (l.d_starts) (void)hipFree(l.d_starts);
+  }
+  plan.launches.clear();
 }
 
 int64_t jastrow_inter_size(const JasDevView& v, int kind) {

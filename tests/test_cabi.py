@@ -145,3 +145,30 @@ def test_c99_program_against_the_header(tmp_path):
                    check=True, capture_output=True, text=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
+def _mo_tile(ctx):
+    import ctypes
+
+    g, w, p = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    q.check(ctx, q.lib().qmckl_amd_get_mo_tile(ctx, ctypes.byref(g), ctypes.byref(w), ctypes.byref(p)))
+    return g.value, w.value, p.value
+
+
+def test_tile_setter_refuses_unbuilt_shapes():
+    """Host-side rules of qmckl_amd_set_mo_tile / qmckl_amd_get_mo_tile (no GPU call)."""
+    from qmckl_amd import system as S
+
+    s, x, W = S.workload("small", walk_num=1)
+    ctx = q.context_create()
+    try:
+        S.load(ctx, s)
+        L = q.lib()
+        assert L.qmckl_amd_set_mo_tile(ctx, 17, 8) == 2
+        assert L.qmckl_amd_set_mo_tile(ctx, 9, 4) == 3
+        assert L.qmckl_amd_set_mo_tile(ctx, 4, 8) == 3
+        assert L.qmckl_amd_set_mo_tile(ctx, -1, 4) == 2
+        assert L.qmckl_amd_set_mo_tile(ctx, 6, 8) == 0
+        assert _mo_tile(ctx) == (1, 4, 1)  # 7 MOs: one column tile, waves adjusted to the cap
+    finally:
+        q.context_destroy(ctx)
