@@ -968,7 +968,7 @@ qmckl_exit_code provide_en_distance(qmckl_context handle, Context* c, const char
     e = hipMemcpyAsync(c->dc.nucl_coord.ptr, c->nucl_coord.data(), sizeof(double) * 3 * Nn,
                        hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
-    e = launch_en_distance(c->d_coord.as<double>(), P, c->dc.nucl_coord.as<double>(), Nn,
+    e = launch_en_distance(c->d_coord.as<double>(), P, P, c->dc.nucl_coord.as<double>(), Nn,
                            c->d_en_dist.as<double>(), c->stream);
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   c->en_dist_date = c->date;

@@ -1,6 +1,7 @@
 // CHAMP Jastrow factor: parameter setters/getters, finalisation (asymptotic shifts),
 // the provider of value and gradient/Laplacian, and the getters of its intermediates.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -282,8 +283,8 @@ hipError_t need_force_inter(Context* c, const JasDevView& v, DerivArgs& d, int s
     n = v.walk_num * v.elec_num * v.nucl_num;
     e = c->d_deriv[slot].reserve(sizeof(double) * std::max<int64_t>(n, 1));
     if (e == hipSuccess)
-      e = launch_en_distance(v.coord, v.walk_num * v.elec_num, v.nucl_coord, v.nucl_num, c->d_deriv[slot].as<double>(),
-                             c->stream);
+      e = launch_en_distance(v.coord, v.walk_num * v.elec_num, v.coord_ld, v.nucl_coord, v.nucl_num,
+                             c->d_deriv[slot].as<double>(), c->stream);
   } else {
     const int what = slot == DB_FTMP_C ? DV_FORCES_TMP_C : slot == DB_FDTMP_C ? DV_FORCES_DTMP_C : DV_FORCES_EEN_N;
     n = deriv_size(d, what);
@@ -294,11 +295,12 @@ hipError_t need_force_inter(Context* c, const JasDevView& v, DerivArgs& d, int s
   return e;
 }
 
-// Fills `d` and provides the intermediates `what` reads.
-hipError_t prepare_deriv(Context* c, int what, DerivArgs& d) {
-  const JasDevView v = jas_view(c);
+// Fills `d` and provides the intermediates `what` reads, for the walkers of the view v (all
+// of c's walkers, or a range of them).  With `bytes` non-null nothing is computed: it
+// receives the size of those intermediates.
+hipError_t prepare_deriv(Context* c, const JasDevView& v, int what, DerivArgs& d, int64_t* bytes = nullptr) {
   d = DerivArgs{};
-  d.W = c->walk_num;
+  d.W = v.walk_num;
   d.Ne = c->elec_num;
   d.Nn = c->nucl_num;
   d.up = c->up_num;
@@ -315,18 +317,26 @@ hipError_t prepare_deriv(Context* c, int what, DerivArgs& d) {
   d.cfull = v.cfull;
   hipError_t e = hipSuccess;
   auto inter = [&](int kind) -> const double* {
-    if (e == hipSuccess) e = need_inter(c, v, kind);
+    if (bytes) *bytes += 8 * jastrow_inter_size(v, kind);
+    else if (e == hipSuccess) e = need_inter(c, v, kind);
     return c->d_deriv[kind].as<double>();
   };
   auto force = [&](int slot) -> const double* {
-    if (e == hipSuccess) e = need_force_inter(c, v, d, slot);
+    if (bytes) {
+      *bytes += 8 * (slot == DB_EDIST ? v.walk_num * v.elec_num * v.nucl_num
+                                      : deriv_size(d, slot == DB_FTMP_C   ? DV_FORCES_TMP_C
+                                                      : slot == DB_FDTMP_C ? DV_FORCES_DTMP_C
+                                                                           : DV_FORCES_EEN_N));
+    } else if (e == hipSuccess) {
+      e = need_force_inter(c, v, d, slot);
+    }
     return c->d_deriv[slot].as<double>();
   };
   if (what == DV_EE_PDERIV || what == DV_EE_GL_PDERIV) {
     d.ee = inter(J_EE_RESCALED);
     if (what == DV_EE_GL_PDERIV) d.eeg = inter(J_EE_RESCALED_GL);
     const std::vector<double> ajb = asymp_jasb_pderiv_of(c);
-    if (e == hipSuccess) e = upload_table(c, DB_AJB, ajb.data(), sizeof(double) * ajb.size());
+    if (e == hipSuccess && !bytes) e = upload_table(c, DB_AJB, ajb.data(), sizeof(double) * ajb.size());
     d.ajb = c->d_deriv[DB_AJB].as<double>();
     return e;
   }
@@ -337,14 +347,14 @@ hipError_t prepare_deriv(Context* c, int what, DerivArgs& d) {
     if (what == DV_FORCES_EN_G || what == DV_FORCES_EN_L) d.edist = force(DB_EDIST);
     if (what == DV_EN_PDERIV) {
       const std::vector<double> aja = asymp_jasa_pderiv_of(c);
-      if (e == hipSuccess) e = upload_table(c, DB_AJA, aja.data(), sizeof(double) * aja.size());
+      if (e == hipSuccess && !bytes) e = upload_table(c, DB_AJA, aja.data(), sizeof(double) * aja.size());
       d.aja = c->d_deriv[DB_AJA].as<double>();
     }
     return e;
   }
   // e-e-n quantities
   const std::vector<int64_t> lk = lkpm_table(c->cord);
-  if (e == hipSuccess) e = upload_table(c, DB_LKPM, lk.data(), sizeof(int64_t) * lk.size());
+  if (e == hipSuccess && !bytes) e = upload_table(c, DB_LKPM, lk.data(), sizeof(int64_t) * lk.size());
   d.lkpm = c->d_deriv[DB_LKPM].as<int64_t>();
   d.tmp_c = inter(J_TMP_C);
   d.een_n = inter(J_EEN_N);
@@ -363,6 +373,14 @@ hipError_t prepare_deriv(Context* c, int what, DerivArgs& d) {
   d.fdtmp_c = force(DB_FDTMP_C);
   d.feen_n = force(DB_FEEN_N);
   return e;
+}
+
+// Device memory for the materialised intermediates of one derivative getter: 2 GiB, or
+// QMCKL_AMD_DERIV_BUDGET_MB megabytes.
+int64_t deriv_budget_bytes() {
+  const char* s = std::getenv("QMCKL_AMD_DERIV_BUDGET_MB");
+  const long long mb = s ? std::atoll(s) : 0;
+  return (mb > 0 ? (int64_t)mb : 2048) << 20;
 }
 
 // Host getter of a derivative: `what` evaluated on the device, walker-summed quantities
@@ -393,18 +411,38 @@ qmckl_exit_code get_deriv(qmckl_context context, int what, double* out, int64_t 
     std::fill(out, out + n, 0.0);
     return QMCKL_SUCCESS;
   }
+  // The intermediates (reference layouts, several MB per walker at a hundred electrons) are
+  // materialised for ranges of walkers that fit the budget, each range's per-walker results
+  // written in place; the walker sums run once over all walkers, in walker order, so the
+  // results do not depend on the ranges.
+  const JasDevView all = jas_view(c);
+  const int64_t W = c->walk_num, per_walker = n_all / std::max<int64_t>(W, 1);
+  int64_t wbytes = 0;
   DerivArgs d;
-  hipError_t e = prepare_deriv(c, what, d);
-  if (e == hipSuccess) e = c->d_deriv[DB_OUT].reserve(sizeof(double) * std::max<int64_t>(n, 1));
+  {
+    JasDevView one = all;
+    one.walk_num = 1;
+    (void)prepare_deriv(c, one, what, d, &wbytes);
+  }
+  const int64_t chunk = std::max<int64_t>(1, deriv_budget_bytes() / std::max<int64_t>(wbytes, 1));
+  hipError_t e = c->d_deriv[DB_OUT].reserve(sizeof(double) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess && summed) e = c->d_deriv[DB_PART].reserve(sizeof(double) * std::max<int64_t>(n_all, 1));
+  double* dst = summed ? c->d_deriv[DB_PART].as<double>() : c->d_deriv[DB_OUT].as<double>();
+  for (int64_t w0 = 0; w0 < W && e == hipSuccess; w0 += chunk) {
+    JasDevView v = all;
+    if (chunk < W) {  // a range of walkers; nothing cached across ranges
+      v.walk_num = std::min(chunk, W - w0);
+      v.coord = all.coord + w0 * c->elec_num;
+      for (uint64_t& dd : c->deriv_date) dd = 0;
+    }
+    e = prepare_deriv(c, v, what, d);
+    if (e == hipSuccess) e = launch_deriv(d, what, dst + w0 * per_walker, c->stream);
+  }
+  if (chunk < W)
+    for (uint64_t& dd : c->deriv_date) dd = 0;
   if (e == hipSuccess && summed) {
-    e = c->d_deriv[DB_PART].reserve(sizeof(double) * std::max<int64_t>(n_all, 1));
-    if (e == hipSuccess) e = launch_deriv(d, what, c->d_deriv[DB_PART].as<double>(), c->stream);
-    const double scale = (what == DV_EEN_PDERIV || what == DV_EEN_GL_PDERIV) ? 1.0 : 1.0 / (double)c->walk_num;
-    if (e == hipSuccess)
-      e = launch_sum_walkers(c->d_deriv[DB_PART].as<double>(), c->walk_num, n, scale, c->d_deriv[DB_OUT].as<double>(),
-                             c->stream);
-  } else if (e == hipSuccess) {
-    e = launch_deriv(d, what, c->d_deriv[DB_OUT].as<double>(), c->stream);
+    const double scale = (what == DV_EEN_PDERIV || what == DV_EEN_GL_PDERIV) ? 1.0 : 1.0 / (double)W;
+    e = launch_sum_walkers(c->d_deriv[DB_PART].as<double>(), W, n, scale, c->d_deriv[DB_OUT].as<double>(), c->stream);
   }
   if (e != hipSuccess) return hip_fail(context, e, fn);
   return copy_out(c, context, c->d_deriv[DB_OUT].ptr, out, n, fn);

@@ -580,7 +580,8 @@ int cusp_max_s_ao();
 hipError_t launch_mo_cusp(const CuspArgs& c, hipStream_t s);
 
 hipError_t launch_ee_distance(const double* coord, int64_t W, int64_t Ne, double* out, hipStream_t s);
-hipError_t launch_en_distance(const double* coord, int64_t P, const double* nucl_coord, int64_t Nn,
+// e-n distances [P][Nn] of the P points coord[p + d ld] (d = 0, 1, 2)
+hipError_t launch_en_distance(const double* coord, int64_t P, int64_t ld, const double* nucl_coord, int64_t Nn,
                               double* out, hipStream_t s);
 
 }  // namespace qmckl_amd

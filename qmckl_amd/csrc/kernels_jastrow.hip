@@ -2200,12 +2200,12 @@ __global__ void k_ee_distance(const double* __restrict__ coord, int64_t W, int64
   out[t] = sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-__global__ void k_en_distance(const double* __restrict__ coord, int64_t P, const double* __restrict__ nc, int64_t Nn,
-                              double* __restrict__ out) {
+__global__ void k_en_distance(const double* __restrict__ coord, int64_t P, int64_t ld, const double* __restrict__ nc,
+                              int64_t Nn, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P * Nn) return;
   const int64_t a = t % Nn, p = t / Nn;
-  const double dx = coord[p] - nc[a], dy = coord[p + P] - nc[a + Nn], dz = coord[p + 2 * P] - nc[a + 2 * Nn];
+  const double dx = coord[p] - nc[a], dy = coord[p + ld] - nc[a + Nn], dz = coord[p + 2 * ld] - nc[a + 2 * Nn];
   out[t] = sqrt(dx * dx + dy * dy + dz * dz);
 }
 
@@ -2619,10 +2619,10 @@ hipError_t launch_ee_distance(const double* coord, int64_t W, int64_t Ne, double
   return hipGetLastError();
 }
 
-hipError_t launch_en_distance(const double* coord, int64_t P, const double* nucl_coord, int64_t Nn, double* out,
-                              hipStream_t s) {
+hipError_t launch_en_distance(const double* coord, int64_t P, int64_t ld, const double* nucl_coord, int64_t Nn,
+                              double* out, hipStream_t s) {
   const int64_t n = P * Nn;
-  hipLaunchKernelGGL(k_en_distance, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coord, P, nucl_coord, Nn, out);
+  hipLaunchKernelGGL(k_en_distance, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, coord, P, ld, nucl_coord, Nn, out);
   return hipGetLastError();
 }
 

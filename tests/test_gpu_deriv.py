@@ -76,3 +76,28 @@ def test_derivatives_follow_coordinate_updates():
     ref = oracle.jastrow_derivatives(s, x2, W)["forces_jastrow_een"].ravel()
     assert np.abs(f2 - ref).max() <= 1e-11 * np.abs(ref).max()
     q.context_destroy(ctx)
+
+
+def test_derivatives_in_walker_ranges(monkeypatch):
+    """With a 1 MB budget (QMCKL_AMD_DERIV_BUDGET_MB) every getter materialises the
+    intermediates one walker range at a time (here one walker: about 1.5 MB each); the
+    results equal the whole-batch evaluation bit for bit (the walker sums run once, in walker
+    order) and the oracle."""
+    s, x, W = S.jastrow_workload(5, 20, 20, 5, 5, 5, walk_num=24)
+    ctx = q.context_create()
+    try:
+        S.load(ctx, s, mo=False)
+        S.set_coords(ctx, s, x, W)
+        ref = oracle.jastrow_derivatives(s, x, W)
+        monkeypatch.delenv("QMCKL_AMD_DERIV_BUDGET_MB", raising=False)
+        whole = {k: q.get_array(ctx, fn, ref[k].size) for k, fn in GETTERS.items()}
+        monkeypatch.setenv("QMCKL_AMD_DERIV_BUDGET_MB", "1")
+        S.set_coords(ctx, s, x, W)  # a new context date: nothing cached
+        for k, fn in GETTERS.items():
+            got = q.get_array(ctx, fn, ref[k].size)
+            assert np.array_equal(got, whole[k]), f"{k}: walker ranges change the result"
+            r = ref[k].ravel()
+            assert np.abs(got - r).max() <= 1e-11 * max(np.abs(r).max(), 1e-300), k
+    finally:
+        monkeypatch.delenv("QMCKL_AMD_DERIV_BUDGET_MB", raising=False)
+        q.context_destroy(ctx)
