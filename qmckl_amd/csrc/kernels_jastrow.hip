@@ -1996,10 +1996,21 @@ EenShape een_shape(const JasDevView& v) {
   const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
   s.mfma = v.cord >= 2 && v.cord <= EMFMA_MAX_C;
   if (s.mfma) {
-    // electrons j per LDS pass: all of them when they fit, else passes of 256
+    // electrons j per LDS pass: all of them when they fit (up to about 310 electrons at
+    // cord 5), else the fewest passes that fit, of equal length (the pair records of the
+    // tile are rebuilt for every pass and nucleus tile)
     const int C = (int)v.cord;
     s.TI = EM;
-    s.JC = std::min((Ne + 3) & ~3, 256);
+    const int ne4 = (Ne + 3) & ~3;
+    auto fits = [&](int js) {
+      return sizeof(double) * (size_t)EenMfmaLds(js, C, (int)v.dim_c, C * (C - 1) / 2).total <= 160 * 1024;
+    };
+    int npass = 1, js = ne4;
+    while (!fits(js) && js > 4) {
+      ++npass;
+      js = (((ne4 + npass - 1) / npass) + 3) & ~3;
+    }
+    s.JC = js;
     s.ntiles = (Ne + EM - 1) / EM;
     const size_t res_lds = sizeof(double) * (size_t)EenResLds(gn_jp(Ne), 16 * gn_na(Nn), C, (int)v.dim_c).total;
     s.res = res_lds <= 160 * 1024;
