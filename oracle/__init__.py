@@ -53,8 +53,9 @@ def lib():
         L.oracle_cutoff.restype = _D
         L.oracle_cutoff.argtypes = [_I32]
         L.oracle_prim_vgl.argtypes = [_I, _I, _P, _P, _P, _I32, _I, _P, _P]
-        L.oracle_shell_vgl.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I, _P, _P]
-        L.oracle_ao_vgl.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I, _P, _P]
+        L.oracle_shell_vgl.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I, _P, _P]
+        L.oracle_ao_vgl.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I, _P,
+                                    _P]
         L.oracle_mo_vgl.argtypes = [_I, _I, _I, _P, _P, _P]
         L.oracle_ee_distance.argtypes = [_I, _I, _P, _P]
         L.oracle_en_distance.argtypes = [_I, _I, _I, _P, _P, _P]
@@ -85,6 +86,10 @@ def _basis_args(s):
     ]
 
 
+def _slater(s):
+    return 1 if getattr(s, "basis_type", "G") == "S" else 0
+
+
 def ao_vgl(s, coord_T, precision=None):
     """AO VGL [P][5][ao] of the points coord_T [3][P]."""
     x = _c(coord_T)
@@ -94,7 +99,7 @@ def ao_vgl(s, coord_T, precision=None):
     af = _c(s.ao_factor)
     prec = s.precision if precision is None else precision
     lib().oracle_ao_vgl(s.nucl_num, s.shell_num, s.prim_num, s.ao_num, *[_p(a) for a in args], _p(af),
-                        prec, P, _p(x), _p(out))
+                        prec, _slater(s), P, _p(x), _p(out))
     return out
 
 
@@ -104,7 +109,8 @@ def shell_vgl(s, coord_T, precision=None):
     out = np.empty((P, 5, s.shell_num))
     args = _basis_args(s)
     prec = s.precision if precision is None else precision
-    lib().oracle_shell_vgl(s.nucl_num, s.shell_num, s.prim_num, *[_p(a) for a in args], prec, P, _p(x), _p(out))
+    lib().oracle_shell_vgl(s.nucl_num, s.shell_num, s.prim_num, *[_p(a) for a in args], prec, _slater(s), P, _p(x),
+                           _p(out))
     return out
 
 

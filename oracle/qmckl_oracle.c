@@ -109,19 +109,69 @@ int oracle_prim_vgl(int64_t nucl_num, int64_t prim_num, const int64_t *nucleus_p
   return SUCCESS;
 }
 
+/* Slater shells of nucleus a at displacement (x, y, z), org/qmckl_ao.org:4930-5040
+ * (qmckl_compute_ao_basis_shell_slater_vgl): the nucleus is skipped when r > cutoff*range
+ * (r, not r^2); at r < 1e-15 a shell of r_power 0 takes the sum of its coefficients as value
+ * (gradient and Laplacian 0); else each primitive with alpha r <= cutoff adds
+ * v = c r^n e^(-alpha r) (n = r_power, the power by repeated multiplication), the gradient
+ * (n/r - alpha) v (x, y, z)/r and the Laplacian (n^2 - 2 n alpha r + alpha r (alpha r - 2) + n)
+ * v / r^2. */
+static void slater_shells(int64_t a, double x, double y, double z, int64_t shell_num,
+                          const int64_t *nucleus_index, const int64_t *nucleus_shell_num,
+                          const int64_t *shell_prim_num, const int64_t *shell_prim_index,
+                          const double *exponent, const int32_t *r_power, const basis_extra *b,
+                          double cutoff, double *shell) {
+  const double r = sqrt(x * x + y * y + z * z);
+  if (r > cutoff * b->nucleus_range[a]) return;
+  for (int64_t s = nucleus_index[a]; s < nucleus_index[a] + nucleus_shell_num[a]; ++s) {
+    double v0 = 0., v1 = 0., v2 = 0., v3 = 0., v4 = 0.;
+    if (r < 1.e-15) {
+      for (int64_t p = shell_prim_index[s]; p < shell_prim_index[s] + shell_prim_num[s]; ++p)
+        if (r_power[s] == 0) v0 += b->coef_normalized[p];
+    } else {
+      const double r_inv = 1.0 / r, n = (double)r_power[s];
+      for (int64_t p = shell_prim_index[s]; p < shell_prim_index[s] + shell_prim_num[s]; ++p) {
+        const double ar = exponent[p] * r;
+        if (ar > cutoff) continue;
+        double rn = 1.0;
+        for (int32_t e = 0; e < r_power[s]; ++e) rn *= r;
+        const double v = b->coef_normalized[p] * rn * exp(-ar);
+        v0 += v;
+        const double g = (n * r_inv - exponent[p]) * v;
+        v1 += g * x * r_inv;
+        v2 += g * y * r_inv;
+        v3 += g * z * r_inv;
+        v4 += r_inv * r_inv * (n * n - 2.0 * n * ar + ar * (ar - 2.0) + n) * v;
+      }
+    }
+    shell[s] = v0;
+    shell[s + shell_num] = v1;
+    shell[s + 2 * shell_num] = v2;
+    shell[s + 3 * shell_num] = v3;
+    shell[s + 4 * shell_num] = v4;
+  }
+}
+
 /* Shell VGL of one point, org/qmckl_ao.org:4769-4868
  * (qmckl_compute_ao_basis_shell_gaussian_vgl): nucleus skipped when r^2 > cutoff*range,
- * primitive skipped when alpha r^2 > cutoff.  shell[5][shell_num]. */
+ * primitive skipped when alpha r^2 > cutoff; Slater shells (slater != 0) as slater_shells.
+ * shell[5][shell_num]. */
 static void shell_vgl_point(double px, double py, double pz, int64_t nucl_num, int64_t shell_num,
                             const double *nucl_coord, const int64_t *nucleus_index,
                             const int64_t *nucleus_shell_num, const int64_t *shell_prim_num,
                             const int64_t *shell_prim_index, const double *exponent,
-                            const basis_extra *b, double cutoff, double *shell) {
+                            const int32_t *r_power, int32_t slater, const basis_extra *b, double cutoff,
+                            double *shell) {
   memset(shell, 0, sizeof(double) * 5 * shell_num);
   for (int64_t a = 0; a < nucl_num; ++a) {
     const double x = px - nucl_coord[a];
     const double y = py - nucl_coord[a + nucl_num];
     const double z = pz - nucl_coord[a + 2 * nucl_num];
+    if (slater) {
+      slater_shells(a, x, y, z, shell_num, nucleus_index, nucleus_shell_num, shell_prim_num, shell_prim_index,
+                    exponent, r_power, b, cutoff, shell);
+      continue;
+    }
     const double r2 = x * x + y * y + z * z;
     if (r2 > cutoff * b->nucleus_range[a]) continue;
     for (int64_t s = nucleus_index[a]; s < nucleus_index[a] + nucleus_shell_num[a]; ++s) {
@@ -199,8 +249,8 @@ int oracle_shell_vgl(int64_t nucl_num, int64_t shell_num, int64_t prim_num, cons
                      const int32_t *shell_ang_mom, const int64_t *shell_prim_num,
                      const int64_t *shell_prim_index, const double *shell_factor,
                      const double *exponent, const double *coefficient, const double *prim_factor,
-                     const int32_t *r_power, int32_t precision, int64_t point_num, const double *coord,
-                     double *shell_vgl) {
+                     const int32_t *r_power, int32_t precision, int32_t slater, int64_t point_num,
+                     const double *coord, double *shell_vgl) {
   basis_extra b;
   finalize_basis(nucl_num, shell_num, prim_num, nucleus_index, nucleus_shell_num, shell_ang_mom,
                  shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor,
@@ -209,20 +259,22 @@ int oracle_shell_vgl(int64_t nucl_num, int64_t shell_num, int64_t prim_num, cons
   for (int64_t ip = 0; ip < point_num; ++ip)
     shell_vgl_point(coord[ip], coord[ip + point_num], coord[ip + 2 * point_num], nucl_num, shell_num,
                     nucl_coord, nucleus_index, nucleus_shell_num, shell_prim_num, shell_prim_index,
-                    exponent, &b, cutoff, shell_vgl + ip * 5 * shell_num);
+                    exponent, r_power, slater, &b, cutoff, shell_vgl + ip * 5 * shell_num);
   free_basis(&b);
   return SUCCESS;
 }
 
 /* AO VGL, org/qmckl_ao.org:8438-8572 (qmckl_compute_ao_vgl_doc), with the shells from
- * :4769-4868 and the polynomials from :6233-6331.  ao_vgl[point][5][ao_num]. */
+ * :4769-4868 (Gaussian) or :4930-5040 (Slater) and the polynomials from :6233-6331; the
+ * nucleus screening of this routine (r^2 > cutoff*range) applies to both shell types.
+ * ao_vgl[point][5][ao_num]. */
 int oracle_ao_vgl(int64_t nucl_num, int64_t shell_num, int64_t prim_num, int64_t ao_num,
                   const double *nucl_coord, const int64_t *nucleus_index,
                   const int64_t *nucleus_shell_num, const int32_t *shell_ang_mom,
                   const int64_t *shell_prim_num, const int64_t *shell_prim_index,
                   const double *shell_factor, const double *exponent, const double *coefficient,
                   const double *prim_factor, const int32_t *r_power, const double *ao_factor,
-                  int32_t precision, int64_t point_num, const double *coord, double *ao_vgl) {
+                  int32_t precision, int32_t slater, int64_t point_num, const double *coord, double *ao_vgl) {
   basis_extra b;
   finalize_basis(nucl_num, shell_num, prim_num, nucleus_index, nucleus_shell_num, shell_ang_mom,
                  shell_prim_num, shell_prim_index, shell_factor, exponent, coefficient, prim_factor,
@@ -241,7 +293,8 @@ int oracle_ao_vgl(int64_t nucl_num, int64_t shell_num, int64_t prim_num, int64_t
     for (int64_t ip = 0; ip < point_num; ++ip) {
       const double X[3] = {coord[ip], coord[ip + point_num], coord[ip + 2 * point_num]};
       shell_vgl_point(X[0], X[1], X[2], nucl_num, shell_num, nucl_coord, nucleus_index,
-                      nucleus_shell_num, shell_prim_num, shell_prim_index, exponent, &b, cutoff, shell);
+                      nucleus_shell_num, shell_prim_num, shell_prim_index, exponent, r_power, slater, &b, cutoff,
+                      shell);
       double *out = ao_vgl + ip * 5 * ao_num;
       memset(out, 0, sizeof(double) * 5 * ao_num);
       for (int64_t a = 0; a < nucl_num; ++a) {

@@ -82,14 +82,56 @@ __device__ __forceinline__ void monomial_vgl(double yx, double yy, double yz, in
   p[4] = (da - 1.0) * xa2 * yzv + (db - 1.0) * yb2 * xz + (dc - 1.0) * zc2 * xy;
 }
 
+// r^n for an integer n by repeated multiplication (n < 0: 1 / r^|n|), as Fortran's r**n.
+__device__ __forceinline__ double ipow_r(double r, int n) {
+  double p = 1.0;
+  for (int i = 0; i < (n < 0 ? -n : n); ++i) p *= r;
+  return n < 0 ? 1.0 / p : p;
+}
+
+// One Slater primitive c r^n e^(-alpha r) at distance r (1/r = r_inv), org/qmckl_ao.org:5004-5034:
+// skipped when alpha r > cutoff; adds its value to v0, (n/r - alpha) v to g (the gradient is
+// g (x, y, z) / r) and (n^2 - 2 n alpha r + alpha r (alpha r - 2) + n) v / r^2 to lap.
+template <int DEG = 13>
+__device__ __forceinline__ void slater_prim(double al, double coef, int n, double r, double r_inv, double cutoff,
+                                            double& v0, double& g, double& lap) {
+  const double ar = al * r;
+  if (ar > cutoff) return;
+  const double v = coef * ipow_r(r, n) * exp_neg_deg<DEG>(-ar);
+  const double dn = (double)n;
+  v0 += v;
+  g += (dn * r_inv - al) * v;
+  lap += r_inv * r_inv * (dn * dn - 2.0 * dn * ar + ar * (ar - 2.0) + dn) * v;
+}
+
 // Radial value, gradient and Laplacian of shell s (on nucleus a) at displacement
 // (x, y, z) = r - R_a, r2 = |r - R_a|^2; org/qmckl_ao.org:4769-4868: the nucleus is skipped
 // when r2 > cutoff * range, a primitive when alpha r2 > cutoff.
+// Slater shells (v.slater), org/qmckl_ao.org:4930-5040: the nucleus is skipped when
+// r > cutoff * range; at r < 1e-15 a shell of r_power 0 has the sum of its coefficients as
+// value, others 0; else the primitives of slater_prim.
 __device__ __forceinline__ void shell_radial(const AoDevView& v, int s, int a, double x, double y, double z,
                                              double r2, double r[5]) {
   const double cutoff = v.cutoff;
   double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
-  if (!(r2 > cutoff * v.nucl_range[a])) {
+  if (v.slater) {
+    const double rr = sqrt(r2);
+    const int p0 = v.shell_prim_start[s], np = v.shell_prim_count[s], n = v.shell_rpow[s];
+    if (!(rr > cutoff * v.nucl_range[a])) {
+      if (rr < 1.e-15) {
+        if (n == 0)
+          for (int i = 0; i < np; ++i) v0 += v.prim_coef[p0 + i];
+      } else {
+        const double r_inv = 1.0 / rr;
+        double g = 0.0;
+        for (int i = 0; i < np; ++i)
+          slater_prim(v.prim_expo[p0 + i], v.prim_coef[p0 + i], n, rr, r_inv, cutoff, v0, g, v4);
+        v1 = g * x * r_inv;
+        v2 = g * y * r_inv;
+        v3 = g * z * r_inv;
+      }
+    }
+  } else if (!(r2 > cutoff * v.nucl_range[a])) {
     const int p0 = v.shell_prim_start[s], np = v.shell_prim_count[s];
     for (int i = 0; i < np; ++i) {
       const double al = v.prim_expo[p0 + i];

@@ -24,9 +24,10 @@ qmckl_exit_code finalize_basis(Context* c, qmckl_context ctx) {
   t = AoTables();
   const int64_t Nn = c->nucl_num;
   if (Nn <= 0) return fail(ctx, QMCKL_NOT_PROVIDED, "qmckl_finalize_basis", "nucleus data is not provided");
-  if (c->ao_type != 'G')
-    return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "only Gaussian ('G') shells are implemented");
+  if (c->ao_type != 'G' && c->ao_type != 'S')
+    return fail(ctx, QMCKL_FAILURE, "qmckl_finalize_basis", "basis type must be 'G' or 'S'");
   if (c->r_power.empty()) c->r_power.assign(c->shell_num, 0);
+  t.slater = c->ao_type == 'S';
   t.nucl_num = Nn;
   t.shell_num = c->shell_num;
   t.prim_num = c->prim_num;
@@ -49,6 +50,7 @@ qmckl_exit_code finalize_basis(Context* c, qmckl_context ctx) {
       t.shell_prim_start.push_back((int32_t)t.prim_expo.size());
       t.shell_prim_count.push_back((int32_t)np);
       t.shell_ao_start.push_back((int32_t)ao);
+      t.shell_rpow.push_back(c->r_power[s]);
       for (int64_t p = p0; p < p0 + np; ++p) {
         t.prim_expo.push_back(c->exponent[p]);
         t.prim_coef.push_back(c->coefficient[p] * c->prim_factor[p] * c->shell_factor[s]);
@@ -195,12 +197,17 @@ static std::vector<int64_t> fused_ao_order(const Context* c) {
 }
 
 // Shape of c's fused product for nct column tiles: the requested one (capped at nct tiles,
-// the waves adjusted when the cap leaves the requested pair unbuilt), else the default.
+// the waves adjusted when the cap leaves the requested pair unbuilt), else the default; with
+// Slater shells at most 8 tiles on 4 waves (the shapes built for them).
 static void mo_tile_shape(const Context* c, int64_t nct, int& G, int& nw) {
   ao_mo_default_shape(nct, G, nw);
   if (c->mo_tile_g > 0) {
     G = (int)std::min<int64_t>(c->mo_tile_g, std::max<int64_t>(nct, 1));
     nw = ao_mo_variant_ok(0, G, c->mo_tile_nw) ? c->mo_tile_nw : ao_mo_waves(G);
+  }
+  if (c->tables.slater && nw != 4) {  // Slater shells: the 4-wave shapes only
+    G = std::min(G, 8);
+    nw = 4;
   }
 }
 
@@ -222,6 +229,7 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
   up(d.shell_prim_start, t.shell_prim_start.data(), 4 * t.shell_prim_start.size());
   up(d.shell_prim_count, t.shell_prim_count.data(), 4 * t.shell_prim_count.size());
   up(d.shell_ao_start, t.shell_ao_start.data(), 4 * t.shell_ao_start.size());
+  up(d.shell_rpow, t.shell_rpow.data(), 4 * t.shell_rpow.size());
   up(d.prim_expo, t.prim_expo.data(), 8 * t.prim_expo.size());
   up(d.prim_coef, t.prim_coef.data(), 8 * t.prim_coef.size());
   up(d.ao_shell, t.ao_shell.data(), 4 * t.ao_shell.size());
@@ -260,7 +268,10 @@ static qmckl_exit_code upload_ao(Context* c, qmckl_context handle, const char* f
       int p_first = 0;
       for (int sh : shells) {
         const int nl = (int)(std::find(nuclei.begin(), nuclei.end(), t.shell_nucl[sh]) - nuclei.begin());
-        desc.insert(desc.end(), {(double)(nl + 256 * t.shell_prim_count[sh]), (double)p_first});
+        // shell record: chunk-local nucleus + 256 x primitives, first primitive + 65536 x
+        // (r_power + 128)
+        desc.insert(desc.end(), {(double)(nl + 256 * t.shell_prim_count[sh]),
+                                 (double)p_first + 65536.0 * (double)(t.shell_rpow[sh] + 128)});
         p_first += t.shell_prim_count[sh];
       }
       for (int sh : shells)
@@ -367,6 +378,8 @@ AoDevView ao_view(Context* c) {
   v.nucl_coord = d.nucl_coord.as<double>();
   // cutoff = -log(epsilon), epsilon = 2^(1-p): org/qmckl_ao.org:118-120, :7811-7815
   v.cutoff = -log(ldexp(1.0, 1 - c->precision));
+  v.slater = c->tables.slater ? 1 : 0;
+  v.shell_rpow = d.shell_rpow.as<int32_t>();
   return v;
 }
 
@@ -438,6 +451,7 @@ qmckl_exit_code mo_product_args(qmckl_context handle, Context* c, double* d_out,
   a.ldc = Nmo;
   a.N = Nmo;
   a.cutoff = ao_view(c).cutoff;
+  a.slater = c->tables.slater ? 1 : 0;
   return QMCKL_SUCCESS;
 }
 
@@ -780,6 +794,8 @@ qmckl_exit_code qmckl_get_ao_basis_primitive_vgl(qmckl_context context, double* 
   const int64_t n = c->point_num * 5 * c->prim_num;
   if (out == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
   if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  // Gaussian primitives only, as the reference (org/qmckl_ao.org:4596-4625)
+  if (c->tables.slater) return fail(context, QMCKL_FAILURE, "compute_ao_basis_primitive_vgl", "Not yet implemented");
   hipError_t e = c->d_jint.reserve(sizeof(double) * n);
   if (e == hipSuccess)
     e = launch_prim_vgl(ao_view(c), c->prim_num, c->dc.prim_nucl_c.as<int32_t>(), c->dc.prim_expo_c.as<double>(),

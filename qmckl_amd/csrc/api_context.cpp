@@ -195,7 +195,7 @@ hipError_t copy_to_host(Context* c, void* dst, const void* d_src, size_t bytes) 
 std::vector<DevBuf*> device_buffers(Context* c) {
   DevConst& d = c->dc;
   std::vector<DevBuf*> bufs = {&d.shell_nucl, &d.shell_l, &d.shell_prim_start, &d.shell_prim_count,
-                    &d.shell_ao_start, &d.prim_expo, &d.prim_coef, &d.ao_shell, &d.ao_pow,
+                    &d.shell_ao_start, &d.shell_rpow, &d.prim_expo, &d.prim_coef, &d.ao_shell, &d.ao_pow,
                     &d.ao_factor, &d.nucl_range, &d.nucl_coord, &d.coef_t, &d.coef_ch64,
                     &d.coef_ch32, &d.coef_chbf, &d.ao_desc, &d.ao_desc_off, &d.jas_a, &d.jas_b,
                     &d.jas_cfull, &d.jas_cfull_pad, &d.jas_nuc_pad, &d.jas_type, &d.jas_kappa_en,

@@ -51,6 +51,7 @@ qmckl_exit_code nucl_ao_ready(Context* c, qmckl_context h, const char* fn) {
 qmckl_exit_code provide_ao_hessian(Context* c, qmckl_context h, const char* fn) {
   qmckl_exit_code rc = ao_ready(c, h, fn);
   if (rc != QMCKL_SUCCESS) return rc;
+  if (c->tables.slater) return fail(h, QMCKL_FAILURE, fn, "Hessians of Slater shells are not implemented");
   if (c->ao_hess_date == c->date) return QMCKL_SUCCESS;
   const int64_t P = c->point_num;
   hipError_t e = c->d_ao_hess.reserve(sizeof(double) * 12 * P * c->ao_num);
@@ -197,6 +198,7 @@ qmckl_exit_code qmckl_get_ao_basis_shell_hessian(qmckl_context context, double* 
   if (!getter_ctx(context, shell_hessian, size_max, n, fn, rc)) return rc;
   rc = ao_ready(c, context, fn);
   if (rc != QMCKL_SUCCESS) return rc;
+  if (c->tables.slater) return fail(context, QMCKL_FAILURE, fn, "Hessians of Slater shells are not implemented");
   hipError_t e = c->d_orb_out.reserve(sizeof(double) * std::max<int64_t>(n, 1));
   if (e == hipSuccess)
     e = launch_shell_hessian(ao_view(c), c->d_coord.as<double>(), c->point_num, c->d_orb_out.as<double>(), c->stream);

@@ -23,8 +23,8 @@ using namespace qmckl_amd;
 namespace {
 
 // bucket: device, MO precision, column tiles, 32-AO chunks and point tiles rounded up to
-// powers of two
-typedef std::tuple<int, int, int64_t, int64_t, int64_t> Bucket;
+// powers of two, shell type
+typedef std::tuple<int, int, int64_t, int64_t, int64_t, int> Bucket;
 
 std::mutex g_tune_mutex;
 std::map<Bucket, std::pair<int, int>> g_tuned;
@@ -35,15 +35,21 @@ int64_t pow2_ceil(int64_t x) {
   return p;
 }
 
-// the built shapes worth timing for nct column tiles: 1 to 4 passes on 4 or 8 waves
-std::vector<std::pair<int, int>> candidates(int64_t nct) {
+// the built shapes worth timing for nct column tiles: 1 to 4 passes on 4 or 8 waves (4 only
+// for Slater shells), and the widest passes of each
+std::vector<std::pair<int, int>> candidates(int64_t nct, int slater) {
   std::vector<std::pair<int, int>> out;
   for (int nw : {4, 8})
     for (int64_t passes = 1; passes <= 4; ++passes) {
       const int G = (int)((nct + passes - 1) / passes);
-      if (!ao_mo_variant_ok(0, G, nw)) continue;
+      if (!ao_mo_variant_ok(0, G, nw, slater)) continue;
       if (std::find(out.begin(), out.end(), std::make_pair(G, nw)) == out.end()) out.push_back({G, nw});
     }
+  // the widest passes, for any number of column tiles
+  const std::pair<int, int> wide[2] = {{(int)std::min<int64_t>(nct, 8), 4}, {(int)std::min<int64_t>(nct, 16), 8}};
+  for (const auto& w : wide)
+    if (ao_mo_variant_ok(0, w.first, w.second, slater) && std::find(out.begin(), out.end(), w) == out.end())
+      out.push_back(w);
   return out;
 }
 
@@ -98,7 +104,8 @@ qmckl_exit_code qmckl_amd_autotune(const qmckl_context* contexts, const int64_t 
     qmckl_exit_code rc = device_ready(c, contexts[i], fn);
     if (rc != QMCKL_SUCCESS) return rc;
     const int64_t nct = (c->mo_num + 15) / 16, nchunk = (c->ao_num + AO_MO_KC - 1) / AO_MO_KC;
-    key[i] = Bucket(c->device, c->mo_precision, nct, pow2_ceil(nchunk), pow2_ceil((c->point_num + 15) / 16));
+    key[i] = Bucket(c->device, c->mo_precision, nct, pow2_ceil(nchunk), pow2_ceil((c->point_num + 15) / 16),
+                    c->tables.slater ? 1 : 0);
   }
   for (int64_t i = 0; i < n; ++i) {
     Context* c = ctx_of(contexts[i]);
@@ -120,7 +127,7 @@ qmckl_exit_code qmckl_amd_autotune(const qmckl_context* contexts, const int64_t 
       if (e != hipSuccess) return hip_fail(contexts[i], e, fn);
       c->mo_vgl_date = 0;  // the timing runs overwrite the context's MO VGL
       float best_ms = 1e30f;
-      for (const auto& cand : candidates(nct)) {
+      for (const auto& cand : candidates(nct, c->tables.slater ? 1 : 0)) {
         float ms = 0.f;
         qmckl_exit_code rc = time_shape(contexts[i], c, cand.first, cand.second, c->d_mo_vgl.as<double>(), fn, ms);
         if (rc != QMCKL_SUCCESS) return rc;

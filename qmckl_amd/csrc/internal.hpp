@@ -39,14 +39,16 @@ struct AoTables {
   std::vector<int32_t> ao_pow;  // a | b<<8 | c<<16
   std::vector<double> ao_factor;
   // per nucleus
-  std::vector<double> nucl_range;  // max 1/alpha over its primitives
+  std::vector<double> nucl_range;  // max (r_power + 1)/alpha over its primitives
   std::vector<int32_t> nucl_max_l;
   int32_t chunk_shells_max = 1;  // most shells met by the AOs of one 32-AO chunk
+  bool slater = false;            // 'S' basis: Slater shells
+  std::vector<int32_t> shell_rpow;  // r_power per shell
 };
 
 // Device copies of the constant data (uploaded lazily before the first kernel).
 struct DevConst {
-  DevBuf shell_nucl, shell_l, shell_prim_start, shell_prim_count, shell_ao_start;
+  DevBuf shell_nucl, shell_l, shell_prim_start, shell_prim_count, shell_ao_start, shell_rpow;
   DevBuf prim_expo, prim_coef, ao_shell, ao_pow, ao_factor, nucl_range;
   DevBuf nucl_coord;  // [3][nucl]
   DevBuf coef_t;      // MO coefficients transposed [k_pad][mo_pad] (FP64, zero-padded)
@@ -275,6 +277,8 @@ struct AoDevView {
   const int32_t *ao_shell, *ao_pow;
   const double *ao_factor, *nucl_range, *nucl_coord;
   double cutoff;
+  int slater;                // Slater ('S') shells: r^n e^(-alpha r) radial parts
+  const int32_t* shell_rpow;  // r_power per shell (the n of Slater shells)
 };
 // out_kind: 0 = FP64 [P][5][ao] (ld = ao_num), 1 = FP32, 2 = bf16x3 planes
 hipError_t launch_ao_vgl(const AoDevView& v, const double* coord, int64_t coord_ld, int64_t p0,
@@ -319,7 +323,8 @@ struct AoMoArgs {
   int desc_max, ns_max, nchunk;
   const void* B;
   int b_chunk_bytes, G, ngroup;
-  int nw;  // waves per workgroup: 4 (G <= 8) or 8 (5 <= G <= 16)
+  int nw;      // waves per workgroup: 4 (G <= 8) or 8 (5 <= G <= 16)
+  int slater;  // Slater shells (the shell records carry r_power)
   double* C;
   int64_t ldc, N;
   double cutoff;
@@ -329,16 +334,16 @@ hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s);
 // The default shape of the product for nct column tiles of 16 MOs: every tile in one pass of
 // 4 waves up to 8 tiles, else passes of at most 16 tiles on 8 waves.
 void ao_mo_default_shape(int64_t nct, int& G, int& nw);
-bool ao_mo_variant_ok(int mode, int G, int nw);
+bool ao_mo_variant_ok(int mode, int G, int nw, int slater = 0);  // a built (G, nw) shape
 int ao_mo_waves(int G);  // the default waves per workgroup for G column tiles
 // One launch over several contexts' products (AoMoArgs device array), one workgroup per item;
 // items packed as context | point tile << 16 | column group << 48, all of one (mode, G, nw).
-hipError_t launch_ao_mo_batch(int mode, int G, int nw, const AoMoArgs* d_args, const uint64_t* d_items, int nitems,
-                              size_t lds, hipStream_t s);
+hipError_t launch_ao_mo_batch(int mode, int G, int nw, int slater, const AoMoArgs* d_args, const uint64_t* d_items,
+                              int nitems, size_t lds, hipStream_t s);
 // The products of a batch of contexts as one grouped launch per shape (mode, G, nw): built
 // once (device argument and item tables allocated), launched per step, freed with the batch.
 struct MoBatchLaunch {
-  int mode = 0, G = 0, nw = 0;
+  int mode = 0, G = 0, nw = 0, slater = 0;
   AoMoArgs* d_args = nullptr;
   uint64_t* d_items = nullptr;
   int nitems = 0;

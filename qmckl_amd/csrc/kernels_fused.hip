@@ -84,12 +84,41 @@ __device__ __forceinline__ Chunk chunk_view(const double* d) {
 // Radial VGL of shell s of the chunk at displacement (x, y, z) from its nucleus, screened as
 // shell_radial (org/qmckl_ao.org:4769-4868); e^x by the degree-DEG series of exp_neg_deg.
 // Returns whether any primitive passed the screening (else r is exactly zero).
-template <int DEG>
+template <int DEG, bool SL>
 __device__ __forceinline__ bool chunk_radial(const Chunk& c, int s, double x, double y, double z, double cutoff,
                                              double r[5]) {
   const double code = c.shl[2 * s];
-  const int nl = (int)code & 255, np = (int)code >> 8, p0 = (int)c.shl[2 * s + 1];
+  const int pcode = (int)c.shl[2 * s + 1];
+  const int nl = (int)code & 255, np = (int)code >> 8, p0 = pcode & 0xffff;
   const double r2 = x * x + y * y + z * z;
+  if constexpr (SL) {
+    // Slater shells, org/qmckl_ao.org:4930-5040, with the nucleus screening of the AO
+    // assembly (:8515, r^2 > cutoff * range) as well as the shell's own (r > cutoff * range)
+    const int n = (pcode >> 16) - 128;
+    const double range = c.nuc[4 * nl + 3], rr = sqrt(r2);
+    double v0 = 0.0, g = 0.0, lap = 0.0, rx = 0.0;
+    bool live = false;
+    if (!(r2 > cutoff * range) && !(rr > cutoff * range)) {
+      if (rr < 1.e-15) {
+        if (n == 0)
+          for (int i = 0; i < np; ++i) v0 += c.prm[2 * (p0 + i) + 1];
+        live = n == 0 && np > 0;
+      } else {
+        rx = 1.0 / rr;
+        for (int i = 0; i < np; ++i) {
+          const double al = c.prm[2 * (p0 + i)];
+          live = live || !(al * rr > cutoff);
+          slater_prim<DEG>(al, c.prm[2 * (p0 + i) + 1], n, rr, rx, cutoff, v0, g, lap);
+        }
+      }
+    }
+    r[0] = v0;
+    r[1] = g * x * rx;
+    r[2] = g * y * rx;
+    r[3] = g * z * rx;
+    r[4] = lap;
+    return live;
+  }
   // per primitive: e = c e^(-alpha r^2) and t = -2 alpha e; the gradient is (sum t) (x, y, z)
   // and the Laplacian 3 sum t - 2 sum t alpha r^2, applied once per shell
   double v0 = 0, st = 0, st2 = 0;
@@ -229,8 +258,9 @@ constexpr bool second_tile() {
   return NW == 4 ? G > 5 : G > 8;
 }
 
-// Evaluates point tile `tile` x column group `group` of the product described by A.
-template <int MODE, int G, int NW>
+// Evaluates point tile `tile` x column group `group` of the product described by A (SL:
+// Slater shells, a compile-time branch so that the Gaussian kernels keep their registers).
+template <int MODE, int G, int NW, bool SL>
 __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int tile, const int group, double* lds) {
   constexpr int NMAIN = G < NW ? G : NW;  // waves with a first column tile
   constexpr bool TWO = second_tile<G, NW>();
@@ -280,7 +310,7 @@ __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int til
       const double x = xt[3 * q] - c.nuc[4 * nl], y = xt[3 * q + 1] - c.nuc[4 * nl + 1],
                    z = xt[3 * q + 2] - c.nuc[4 * nl + 2];
       double r[5];
-      const bool lv = chunk_radial<MODE == 0 ? 13 : 9>(c, s, x, y, z, A.cutoff, r);
+      const bool lv = chunk_radial<MODE == 0 ? 13 : 9, SL>(c, s, x, y, z, A.cutoff, r);
       if (q < ntp) live |= (int)lv;
       double* o = sh + (q * A.ns_max + s) * SH;
 #pragma unroll
@@ -367,10 +397,10 @@ constexpr int ao_mo_min_blocks() {
   return NW == 8 ? 1 : (G > 5 ? 2 : (MODE == 0 ? 3 : 4));
 }
 
-template <int MODE, int G, int NW>
+template <int MODE, int G, int NW, bool SL>
 __global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>())) k_ao_mo_tile(AoMoArgs A) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
-  ao_mo_tile_body<MODE, G, NW>(A, blockIdx.x, blockIdx.y, lds);
+  ao_mo_tile_body<MODE, G, NW, SL>(A, blockIdx.x, blockIdx.y, lds);
 }
 
 // One launch over the products of several contexts (a batch of molecules): workgroup b
@@ -378,7 +408,7 @@ __global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>())) k_
 // tile << 16 | group << 48 (context and group < 2^16, tile < 2^32).  The host orders the
 // items by decreasing cost, so that the hardware dispatcher, which hands out workgroups in
 // order as CUs free up, leaves the cheapest items for the end.
-template <int MODE, int G, int NW>
+template <int MODE, int G, int NW, bool SL>
 __global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>()))
     k_ao_mo_batch(const AoMoArgs* __restrict__ args, const uint64_t* __restrict__ items, int nitems) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -386,7 +416,7 @@ __global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>()))
   const int m = __builtin_amdgcn_readfirstlane((int)(it & 0xffffu));
   const int tile = __builtin_amdgcn_readfirstlane((int)((it >> 16) & 0xffffffffu));
   const int group = __builtin_amdgcn_readfirstlane((int)(it >> 48));
-  ao_mo_tile_body<MODE, G, NW>(args[m], tile, group, lds);
+  ao_mo_tile_body<MODE, G, NW, SL>(args[m], tile, group, lds);
 }
 
 // Dynamic LDS limit of a kernel: the CU's 160 KiB less its static LDS (the block-wide OR of
@@ -406,32 +436,45 @@ hipError_t prepare_lds(K kernel, size_t lds, size_t& max_dyn) {
   return lds > max_dyn ? hipErrorInvalidValue : hipSuccess;
 }
 
-template <int MODE, int G, int NW>
+template <int MODE, int G, int NW, bool SL>
 hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
   static size_t max_dyn = 0;
-  hipError_t e = prepare_lds(&k_ao_mo_tile<MODE, G, NW>, lds, max_dyn);
+  hipError_t e = prepare_lds(&k_ao_mo_tile<MODE, G, NW, SL>, lds, max_dyn);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
-  hipLaunchKernelGGL((k_ao_mo_tile<MODE, G, NW>), grid, dim3(64 * NW), lds, s, a);
+  hipLaunchKernelGGL((k_ao_mo_tile<MODE, G, NW, SL>), grid, dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
-template <int MODE, int G, int NW>
+template <int MODE, int G, int NW, bool SL>
 hipError_t launch_batch_g(const AoMoArgs* d_args, const uint64_t* d_items, int nitems, size_t lds, hipStream_t s) {
   static size_t max_dyn = 0;
-  hipError_t e = prepare_lds(&k_ao_mo_batch<MODE, G, NW>, lds, max_dyn);
+  hipError_t e = prepare_lds(&k_ao_mo_batch<MODE, G, NW, SL>, lds, max_dyn);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_ao_mo_batch<MODE, G, NW>), dim3((unsigned)nitems), dim3(64 * NW), lds, s, d_args, d_items,
-                     nitems);
+  hipLaunchKernelGGL((k_ao_mo_batch<MODE, G, NW, SL>), dim3((unsigned)nitems), dim3(64 * NW), lds, s, d_args,
+                     d_items, nitems);
   return hipGetLastError();
 }
 
-// Calls F::template run<MODE, G, NW>() for a runtime (mode, G, NW) among the built variants:
-// NW = 4 for G = 1..8, NW = 8 for G = 5..16.
+// Calls F::template run<MODE, G, NW, SL>() for a runtime (mode, G, NW, slater) among the
+// built variants: Gaussian shells NW = 4 for G = 1..8 and NW = 8 for G = 5..16; Slater shells
+// NW = 4 for G = 1..8.
 template <class F>
-hipError_t dispatch_variant(int mode, int G, int NW, F&& f) {
+hipError_t dispatch_variant(int mode, int G, int NW, int slater, F&& f) {
+  if (slater) {
+#define AO_MO_S(M, GG) \
+  if (mode == M && G == GG && NW == 4) return f.template run<M, GG, 4, true>();
+#define AO_MO_SMODE(M) AO_MO_S(M, 1) AO_MO_S(M, 2) AO_MO_S(M, 3) AO_MO_S(M, 4) AO_MO_S(M, 5) AO_MO_S(M, 6) \
+  AO_MO_S(M, 7) AO_MO_S(M, 8)
+    AO_MO_SMODE(0)
+    AO_MO_SMODE(1)
+    AO_MO_SMODE(2)
+#undef AO_MO_SMODE
+#undef AO_MO_S
+    return hipErrorInvalidValue;
+  }
 #define AO_MO_V(M, GG, W) \
-  if (mode == M && G == GG && NW == W) return f.template run<M, GG, W>();
+  if (mode == M && G == GG && NW == W) return f.template run<M, GG, W, false>();
 #define AO_MO_MODE(M)                                                                                    \
   AO_MO_V(M, 1, 4) AO_MO_V(M, 2, 4) AO_MO_V(M, 3, 4) AO_MO_V(M, 4, 4) AO_MO_V(M, 5, 4) AO_MO_V(M, 6, 4)  \
   AO_MO_V(M, 7, 4) AO_MO_V(M, 8, 4) AO_MO_V(M, 5, 8) AO_MO_V(M, 6, 8) AO_MO_V(M, 7, 8) AO_MO_V(M, 8, 8)  \
@@ -449,8 +492,8 @@ struct RunOne {
   const AoMoArgs& a;
   size_t lds;
   hipStream_t s;
-  template <int M, int G, int NW>
-  hipError_t run() { return launch_g<M, G, NW>(a, lds, s); }
+  template <int M, int G, int NW, bool SL>
+  hipError_t run() { return launch_g<M, G, NW, SL>(a, lds, s); }
 };
 
 struct RunBatch {
@@ -459,8 +502,8 @@ struct RunBatch {
   int n;
   size_t lds;
   hipStream_t s;
-  template <int M, int G, int NW>
-  hipError_t run() { return launch_batch_g<M, G, NW>(args, items, n, lds, s); }
+  template <int M, int G, int NW, bool SL>
+  hipError_t run() { return launch_batch_g<M, G, NW, SL>(args, items, n, lds, s); }
 };
 
 }  // namespace
@@ -470,28 +513,28 @@ size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
          (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
 }
 
-bool ao_mo_variant_ok(int mode, int G, int NW) {
+bool ao_mo_variant_ok(int mode, int G, int NW, int slater) {
   if (mode < 0 || mode > 2) return false;
-  return (NW == 4 && G >= 1 && G <= 8) || (NW == 8 && G >= 5 && G <= 16);
+  return (NW == 4 && G >= 1 && G <= 8) || (!slater && NW == 8 && G >= 5 && G <= 16);
 }
 
 hipError_t launch_ao_mo(const AoMoArgs& a, int mode, hipStream_t s) {
   if (a.np <= 0) return hipSuccess;
-  if (!ao_mo_variant_ok(mode, a.G, a.nw) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
+  if (!ao_mo_variant_ok(mode, a.G, a.nw, a.slater) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
       a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te)
     return hipErrorInvalidValue;
   const size_t lds = ao_mo_lds_bytes(a, mode);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   RunOne r{a, lds, s};
-  return dispatch_variant(mode, a.G, a.nw, r);
+  return dispatch_variant(mode, a.G, a.nw, a.slater, r);
 }
 
-hipError_t launch_ao_mo_batch(int mode, int G, int NW, const AoMoArgs* d_args, const uint64_t* d_items, int nitems,
-                              size_t lds, hipStream_t s) {
+hipError_t launch_ao_mo_batch(int mode, int G, int NW, int slater, const AoMoArgs* d_args, const uint64_t* d_items,
+                              int nitems, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  if (!ao_mo_variant_ok(mode, G, NW) || lds > 160 * 1024) return hipErrorInvalidValue;
+  if (!ao_mo_variant_ok(mode, G, NW, slater) || lds > 160 * 1024) return hipErrorInvalidValue;
   RunBatch r{d_args, d_items, nitems, lds, s};
-  return dispatch_variant(mode, G, NW, r);
+  return dispatch_variant(mode, G, NW, slater, r);
 }
 
 int ao_mo_waves(int G) { return G > 8 ? 8 : 4; }
@@ -500,16 +543,16 @@ hipError_t mo_batch_build(const std::vector<std::pair<AoMoArgs, int>>& in, MoBat
   mo_batch_free(plan);
   // contexts grouped by shape; within a shape, the costliest contexts' items first (an item
   // costs about its context's number of 32-AO chunks)
-  std::map<std::tuple<int, int, int>, std::vector<int>> by_shape;
+  std::map<std::tuple<int, int, int, int>, std::vector<int>> by_shape;
   for (int i = 0; i < (int)in.size(); ++i) {
     const AoMoArgs& a = in[i].first;
     const int mode = in[i].second;
     if (a.np <= 0) continue;
-    if (!ao_mo_variant_ok(mode, a.G, a.nw) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
+    if (!ao_mo_variant_ok(mode, a.G, a.nw, a.slater) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
         a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te ||
         ao_mo_lds_bytes(a, mode) > 160 * 1024)
       return hipErrorInvalidValue;
-    by_shape[std::make_tuple(mode, a.G, a.nw)].push_back(i);
+    by_shape[std::make_tuple(mode, a.G, a.nw, a.slater)].push_back(i);
   }
   hipError_t e = hipSuccess;
   for (auto& kv : by_shape) {
@@ -522,6 +565,7 @@ hipError_t mo_batch_build(const std::vector<std::pair<AoMoArgs, int>>& in, MoBat
     l.mode = std::get<0>(kv.first);
     l.G = std::get<1>(kv.first);
     l.nw = std::get<2>(kv.first);
+    l.slater = std::get<3>(kv.first);
     for (size_t m = 0; m < ids.size(); ++m) {
       const AoMoArgs& a = in[ids[m]].first;
       args.push_back(a);
@@ -557,7 +601,7 @@ hipError_t mo_batch_launch(const MoBatchPlan& plan, const hipStream_t* s, int n)
     const MoBatchLaunch& l = plan.launches[i];
     const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[k] += l.cost;
-    const hipError_t e = launch_ao_mo_batch(l.mode, l.G, l.nw, l.d_args, l.d_items, l.nitems, l.lds, s[k]);
+    const hipError_t e = launch_ao_mo_batch(l.mode, l.G, l.nw, l.slater, l.d_args, l.d_items, l.nitems, l.lds, s[k]);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

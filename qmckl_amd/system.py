@@ -59,6 +59,7 @@ class System:
     precision: int = 53
     keep_mo: np.ndarray = None  # optional select_mo mask
     r_power: np.ndarray = None  # optional int32 [shell] (qmckl_set_ao_basis_r_power); default 0
+    basis_type: str = "G"  # 'G' Gaussian or 'S' Slater shells (qmckl_set_ao_basis_type)
     name: str = ""
     meta: dict = field(default_factory=dict)
 
@@ -131,7 +132,7 @@ def load(ctx, s: System, jastrow=True, mo=True):
     a = f64(s.nucl_charge); keep.append(a)
     c(L.qmckl_set_nucleus_charge(ctx, ptr(a), a.size), "set_nucleus_charge")
     c(L.qmckl_set_electron_num(ctx, s.up, s.dn), "set_electron_num")
-    c(L.qmckl_set_ao_basis_type(ctx, b"G"), "set_ao_basis_type")
+    c(L.qmckl_set_ao_basis_type(ctx, s.basis_type.encode()), "set_ao_basis_type")
     c(L.qmckl_set_ao_basis_shell_num(ctx, s.shell_num), "set_ao_basis_shell_num")
     c(L.qmckl_set_ao_basis_prim_num(ctx, s.prim_num), "set_ao_basis_prim_num")
     for name, arr, conv in [
@@ -180,6 +181,8 @@ def from_context(ctx, name=""):
     """The System held by a context whose nucleus, electron, AO-basis and MO-basis groups are set
     (for instance by qmckl_trexio_read), read back through the getters.  The Jastrow fields are
     zero parameters of orders 5/5/5 over one nucleus type."""
+    import ctypes
+
     from . import get_array, get_scalar
 
     i64 = lambda fn, n: get_array(ctx, "qmckl_get_" + fn, n, np.int64)
@@ -201,6 +204,7 @@ def from_context(ctx, name=""):
         coefficient=f64("ao_basis_coefficient", prim), prim_factor=f64("ao_basis_prim_factor", prim),
         ao_factor=f64("ao_basis_ao_factor", ao), mo_coef=f64("mo_basis_coefficient", mo * ao).reshape(mo, ao),
         r_power=get_array(ctx, "qmckl_get_ao_basis_r_power", shell, np.int32),
+        basis_type=get_scalar(ctx, "qmckl_get_ao_basis_type", ctypes.c_char).decode(),
         type_nucl_vector=np.zeros(Nn, np.int64), a_vector=np.zeros((1, 6)), b_vector=np.zeros(6),
         c_vector=np.zeros((1, dim_c_vector(5))), rescale_en=np.ones(1), name=name)
 
@@ -395,6 +399,46 @@ def molecule_workload(elec_num, walk_num=256, seed=11):
     mo = _orthonormal_rows(rng, max(up, dn), nao)
     s = System(nucl_charge=np.full(nn, 7.0), nucl_coord=coord, up=up, dn=dn, mo_coef=mo, precision=41,
                name=f"molecule-{elec_num}e", **basis, **_jastrow(rng, [0] * nn, 1))
+    x = electron_coords(rng, s, walk_num)
+    return s, x, walk_num
+
+
+def slater_workload(walk_num=4, seed=17):
+    """A system of Slater ('S') shells: 3 nuclei (Z = 6, 1, 8) in a 6-bohr box; per nucleus an
+    s shell with r_power 0, an s shell with r_power 1, a p shell with r_power 0 and a d shell
+    with r_power 1 (cartesian, 16 AOs per nucleus), 2 primitives each with exponents in
+    [0.5, 3] and the normalisation (2 alpha)^(n + 3/2) / sqrt((2n + 2)!) of r^n e^(-alpha r) as
+    primitive factors; 12 electrons (6 up), all 48 MOs as random orthonormal rows; Jastrow
+    orders 5/5/5; precision 53.  Returns (System, coord_T, walk_num)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nn = 3
+    coord = _place_nuclei(rng, nn, 6.0, 1.5)
+    shells = [(0, 0), (0, 1), (1, 0), (2, 1)]  # (l, r_power)
+    ang, rpow, pnum, pidx, nidx, nsn, expo, coef, pf = [], [], [], [], [], [], [], [], []
+    for a in range(nn):
+        nidx.append(len(ang))
+        nsn.append(len(shells))
+        for l, n in shells:
+            ang.append(l)
+            rpow.append(n)
+            pidx.append(len(expo))
+            pnum.append(2)
+            for al in sorted(rng.uniform(0.5, 3.0, 2), reverse=True):
+                expo.append(al)
+                coef.append(rng.uniform(0.2, 1.0))
+                pf.append((2.0 * al) ** (n + 1.5) / math.sqrt(math.factorial(2 * n + 2)))
+    nao = sum((l + 1) * (l + 2) // 2 for l in ang)
+    basis = dict(shell_ang_mom=np.array(ang, np.int32), shell_prim_num=np.array(pnum, np.int64),
+                 shell_prim_index=np.array(pidx, np.int64), nucleus_index=np.array(nidx, np.int64),
+                 nucleus_shell_num=np.array(nsn, np.int64), shell_factor=np.ones(len(ang)),
+                 exponent=np.array(expo), coefficient=np.array(coef), prim_factor=np.array(pf),
+                 ao_factor=rng.uniform(0.8, 1.2, nao))
+    up, dn = 6, 6
+    mo = _orthonormal_rows(rng, nao, nao)
+    types = [0, 1, 2]
+    s = System(nucl_charge=np.array([6.0, 1.0, 8.0]), nucl_coord=coord, up=up, dn=dn, mo_coef=mo, precision=53,
+               r_power=np.array(rpow, np.int32), basis_type="S", name="slater-3n-12e", **basis,
+               **_jastrow(rng, types, 3))
     x = electron_coords(rng, s, walk_num)
     return s, x, walk_num
 

@@ -218,3 +218,36 @@ def test_context_copy_after_device_coordinates():
     np.testing.assert_array_equal(v, q.get_array(ctx, "qmckl_get_jastrow_champ_value", W))
     for c in (ctx, c2):
         q.context_destroy(c)
+
+
+# ---------------------------------------------------------------- Slater ('S') shells
+def test_slater_basis_vs_oracle():
+    """A basis of Slater shells (r^n e^(-alpha r), n = r_power 0 and 1; s, p and d shells)
+    through the AO and shell VGL getters and the fused AO->MO product in FP64 (1e-12 of scale)
+    and 3xBF16 (the split-precision contract), against the oracle; one electron of one walker
+    sits exactly on a nucleus (the r = 0 branch).  The primitive VGL and the Hessians are
+    Gaussian-only here, as the primitive VGL is in the reference: both answer QMCKL_FAILURE."""
+    s, x, W = S.slater_workload(walk_num=8)
+    x = x.copy()
+    x[:, 0] = s.nucl_coord[:, 0]  # walker 0, electron 0 on nucleus 0
+    ctx = make_ctx(s, x, W)
+    try:
+        P = W * s.elec_num
+        ao = q.get_array(ctx, "qmckl_get_ao_basis_ao_vgl", P * 5 * s.ao_num).reshape(P, 5, s.ao_num)
+        vgl_close(ao, oracle.ao_vgl(s, x), 1e-12, "slater ao_vgl")
+        sh = q.get_array(ctx, "qmckl_get_ao_basis_shell_vgl", P * 5 * s.shell_num).reshape(P, 5, s.shell_num)
+        vgl_close(sh, oracle.shell_vgl(s, x), 1e-12, "slater shell_vgl")
+        ref = oracle.mo_vgl(oracle.kept_mo_coef(s), oracle.ao_vgl(s, x))
+        S.set_coords(ctx, s, x, W)  # a new date: the MO product evaluates the AOs itself
+        vgl_close(_mo(ctx, s, W, q.MO_FP64), ref, 1e-12, "slater mo_vgl fp64")
+        # the point on nucleus 0 holds the s AOs at their cusp, the largest terms of the
+        # product: the FP32 sums of the leading bf16 product inside the matrix instructions
+        # reach 1.3e-7 of the largest MO value there (the exception of the batched molecules)
+        assert_split_contract(_mo(ctx, s, W, q.MO_BF16X3), ref, "slater bf16x3",
+                              bound=(1.5e-7,) * 4 + SPLIT_BOUND[4:])
+        out = np.zeros(P * 5 * s.prim_num)
+        assert q.lib().qmckl_get_ao_basis_primitive_vgl(ctx, out.ctypes.data, out.size) == q.QMCKL_FAILURE
+        out = np.zeros(P * 12 * s.ao_num)
+        assert q.lib().qmckl_get_ao_basis_ao_hessian(ctx, out.ctypes.data, out.size) == q.QMCKL_FAILURE
+    finally:
+        q.context_destroy(ctx)

@@ -322,3 +322,113 @@ def test_rescaled_distance_gl_match_finite_differences():
         off = np.arange(Ne) != i
         np.testing.assert_allclose((lap_ee / h2 ** 2)[:, off], r["ee_rescaled_gl"][:, off, i, 3], atol=2e-5)
         np.testing.assert_allclose(lap_en / h2 ** 2, r["en_rescaled_gl"][:, :, i, 3], atol=2e-5)
+
+
+# ------------------------------------------------------------------------ Slater shells
+def _slater_closed_form(s, x):
+    """AO values [P][ao] of a Slater basis from the definition: ao_factor * x^a y^b z^c *
+    shell_factor * sum_p coefficient_p prim_factor_p r^n e^(-alpha_p r) (n = r_power of the
+    shell), with the screening of org/qmckl_ao.org: the AOs of a nucleus vanish where
+    r^2 > cutoff * range (the AO assembly, :8515) or r > cutoff * range (the Slater shells,
+    :4970), range = max (r_power + 1) / alpha; a primitive is left out where alpha r > cutoff."""
+    cutoff = -np.log(2.0 ** (1 - s.precision))
+    P = x.shape[1]
+    out = np.zeros((P, s.ao_num))
+    k = 0
+    for a in range(s.nucl_num):
+        d = x - s.nucl_coord[:, a][:, None]
+        r = np.sqrt((d * d).sum(0))
+        shells = range(s.nucleus_index[a], s.nucleus_index[a] + s.nucleus_shell_num[a])
+        rng_a = max((s.r_power[sh] + 1) / s.exponent[p] for sh in shells
+                    for p in range(s.shell_prim_index[sh], s.shell_prim_index[sh] + s.shell_prim_num[sh]))
+        far = (r * r > cutoff * rng_a) | (r > cutoff * rng_a)
+        for sh in shells:
+            l, n = int(s.shell_ang_mom[sh]), int(s.r_power[sh])
+            rad = np.zeros(P)
+            for p in range(s.shell_prim_index[sh], s.shell_prim_index[sh] + s.shell_prim_num[sh]):
+                ar = s.exponent[p] * r
+                rad += np.where(ar > cutoff, 0.0, s.coefficient[p] * s.prim_factor[p] * r ** n * np.exp(-ar))
+            rad *= s.shell_factor[sh]
+            rad[far] = 0.0
+            for pa in range(l, -1, -1):
+                for pb in range(l - pa, -1, -1):
+                    pc = l - pa - pb
+                    out[:, k] = s.ao_factor[k] * d[0] ** pa * d[1] ** pb * d[2] ** pc * rad
+                    k += 1
+    return out
+
+
+@pytest.mark.parametrize("precision", [53, 30])
+def test_slater_ao_values_match_closed_form(precision):
+    s, x, W = S.slater_workload(walk_num=4)
+    s.precision = precision
+    pts = np.concatenate([x, _points(s, 24, 5)], axis=1)
+    got = oracle.ao_vgl(s, pts)[:, 0, :]
+    ref = _slater_closed_form(s, pts)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+
+
+def test_slater_ao_gradient_and_laplacian_match_finite_differences():
+    s, x, W = S.slater_workload(walk_num=2)
+    vgl = oracle.ao_vgl(s, x)
+    h = 1e-4
+    for d in range(3):
+        e = np.zeros((3, 1))
+        e[d] = h
+        fd = (oracle.ao_vgl(s, x + e)[:, 0, :] - oracle.ao_vgl(s, x - e)[:, 0, :]) / (2 * h)
+        assert np.abs(fd - vgl[:, 1 + d, :]).max() <= 1e-6 * np.abs(vgl[:, 1 + d, :]).max(), f"d/d{'xyz'[d]}"
+    h = 2e-3
+    lap = -6.0 * vgl[:, 0, :]
+    for d in range(3):
+        e = np.zeros((3, 1))
+        e[d] = h
+        lap = lap + oracle.ao_vgl(s, x + e)[:, 0, :] + oracle.ao_vgl(s, x - e)[:, 0, :]
+    lap /= h * h
+    assert np.abs(lap - vgl[:, 4, :]).max() <= 2e-4 * np.abs(vgl[:, 4, :]).max()
+
+
+def test_slater_shells_at_the_reference_test_point():
+    """The inputs of the reference's own Slater test (org/qmckl_ao.org:4284-4297): X =
+    (1.1, 2.2, 3.3), R = (0.1, 1.2, -2.3), five s shells with r^i e^(-0.5 i r), i = 1..5:
+    shell values from numpy, gradients and Laplacians by central finite differences of the
+    oracle's values."""
+    s, _, _ = S.slater_workload(walk_num=1)
+    five = dict(nucl_charge=np.array([1.0]), nucl_coord=np.array([[0.1], [1.2], [-2.3]]),
+                shell_ang_mom=np.zeros(5, np.int32), shell_prim_num=np.ones(5, np.int64),
+                shell_prim_index=np.arange(5, dtype=np.int64), nucleus_index=np.zeros(1, np.int64),
+                nucleus_shell_num=np.array([5], np.int64), shell_factor=np.ones(5),
+                exponent=0.5 * np.arange(1, 6), coefficient=np.ones(5), prim_factor=np.ones(5),
+                ao_factor=np.ones(5), r_power=np.arange(1, 6, dtype=np.int32))
+    t = S.System(up=1, dn=0, mo_coef=np.eye(5), type_nucl_vector=np.zeros(1, np.int64), a_vector=s.a_vector[:1],
+                 b_vector=s.b_vector, c_vector=s.c_vector[:1], rescale_en=s.rescale_en[:1], basis_type="S",
+                 precision=53, **five)
+    X = np.array([[1.1], [2.2], [3.3]])
+    r = float(np.linalg.norm(X[:, 0] - t.nucl_coord[:, 0]))
+    sh = oracle.shell_vgl(t, X)[0]
+    np.testing.assert_allclose(sh[0], r ** np.arange(1, 6) * np.exp(-0.5 * np.arange(1, 6) * r), rtol=1e-14)
+    h = 1e-5
+    for d in range(3):
+        e = np.zeros((3, 1))
+        e[d] = h
+        fd = (oracle.shell_vgl(t, X + e)[0, 0] - oracle.shell_vgl(t, X - e)[0, 0]) / (2 * h)
+        np.testing.assert_allclose(sh[1 + d], fd, rtol=1e-8)
+    h = 1e-3
+    lap = -6.0 * sh[0]
+    for d in range(3):
+        e = np.zeros((3, 1))
+        e[d] = h
+        lap = lap + oracle.shell_vgl(t, X + e)[0, 0] + oracle.shell_vgl(t, X - e)[0, 0]
+    np.testing.assert_allclose(sh[4], lap / h ** 2, rtol=1e-5)
+
+
+def test_slater_shells_on_a_nucleus():
+    """At r = 0 (org/qmckl_ao.org:4985-4995): a shell with r_power 0 has the sum of its
+    normalised coefficients as value, every other shell 0; gradients and Laplacians 0."""
+    s, _, _ = S.slater_workload(walk_num=1)
+    X = np.ascontiguousarray(s.nucl_coord[:, :1])
+    sh = oracle.shell_vgl(s, X)[0]
+    for q in range(s.nucleus_shell_num[0]):
+        p0, npr = s.shell_prim_index[q], s.shell_prim_num[q]
+        want = (s.coefficient[p0:p0 + npr] * s.prim_factor[p0:p0 + npr]).sum() if s.r_power[q] == 0 else 0.0
+        assert sh[0, q] == pytest.approx(want * s.shell_factor[q], rel=1e-15)
+        assert np.all(sh[1:, q] == 0.0)
