@@ -528,11 +528,15 @@ qmckl_exit_code qmckl_woodbury_3x3_doc(const qmckl_context context, const uint64
  * precision).  Device pointers are HIP device addresses on the context's device.
  * ====================================================================================== */
 
-/* Precision of the AO->MO product (default QMCKL_AMD_MO_FP64, the reference's arithmetic).
- * FP32: the AO VGL and the MO coefficients are rounded to FP32 and multiplied on FP32
- * matrix cores with FP32 accumulation.  BF16X3: both operands are split into three
- * bfloat16 terms and the six leading cross products are accumulated in FP32 on bf16
- * matrix cores.  AO values, distances and the whole Jastrow stay FP64 in every mode. */
+/* Precision of the AO->MO product.  Until it is set, the numerical precision decides:
+ * QMCKL_AMD_MO_FP64 (the reference's arithmetic) above 26 bits, QMCKL_AMD_MO_BF16X3 at 26
+ * bits or fewer (where the reference's optimised build computes the product in single
+ * precision, org/qmckl_mo.org:2028-2040); qmckl_amd_get_mo_precision returns the mode in use.
+ * FP32: the AO VGL and the MO coefficients are rounded to FP32 and multiplied on FP32 matrix
+ * cores, FP32 sums per 32 AOs added in FP64.  BF16X3: both operands are split into three
+ * bfloat16 terms and the six leading cross products are accumulated in FP32 on bf16 matrix
+ * cores, added in FP64 per 32 AOs.  AO values, distances and the whole Jastrow stay FP64 in
+ * every mode. */
 #define QMCKL_AMD_MO_FP64 ((int32_t)0)
 #define QMCKL_AMD_MO_FP32 ((int32_t)1)
 #define QMCKL_AMD_MO_BF16X3 ((int32_t)2)

@@ -425,7 +425,7 @@ qmckl_exit_code mo_product_args(qmckl_context handle, Context* c, double* d_out,
   if (rc != QMCKL_SUCCESS) return rc;
   const int64_t P = c->point_num, Nmo = c->mo_num;
   const DevConst& d = c->dc;
-  mode = c->mo_precision;
+  mode = mo_mode(c);
   a.coord = c->d_coord.as<double>();
   a.coord_ld = P;
   a.p0 = 0;
@@ -471,7 +471,7 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     if (e != hipSuccess) return hip_fail(handle, e, fn);
     out = c->d_mo_vgl.as<double>();
   }
-  const int mode = c->mo_precision;
+  const int mode = mo_mode(c);
   const int64_t pad = c->dc.mo_pad;
   ProfScope ps(c, SLOT_MO);
   if (mode == QMCKL_AMD_MO_FP64 && c->ao_vgl_date >= c->date) {
@@ -1073,7 +1073,7 @@ qmckl_exit_code qmckl_amd_get_mo_precision(qmckl_context context, int32_t* const
   Context* c = ctx_of(context);
   if (!c) return QMCKL_INVALID_CONTEXT;
   if (mode == nullptr) return fail(context, QMCKL_INVALID_ARG_2, "qmckl_amd_get_mo_precision", "NULL pointer");
-  *mode = c->mo_precision;
+  *mode = mo_mode(c);
   return QMCKL_SUCCESS;
 }
 

@@ -52,7 +52,7 @@ struct Snapshot {
     s.elec_num = c->elec_num;
     s.mo_num = c->mo_num;
     s.const_date = c->const_date;
-    s.mo_precision = c->mo_precision;
+    s.mo_precision = mo_mode(c);
     for (DevBuf* b : device_buffers(c)) s.buffers.emplace_back(b->ptr, b->bytes);
     return s;
   }

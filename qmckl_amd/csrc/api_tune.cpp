@@ -104,7 +104,7 @@ qmckl_exit_code qmckl_amd_autotune(const qmckl_context* contexts, const int64_t 
     qmckl_exit_code rc = device_ready(c, contexts[i], fn);
     if (rc != QMCKL_SUCCESS) return rc;
     const int64_t nct = (c->mo_num + 15) / 16, nchunk = (c->ao_num + AO_MO_KC - 1) / AO_MO_KC;
-    key[i] = Bucket(c->device, c->mo_precision, nct, pow2_ceil(nchunk), pow2_ceil((c->point_num + 15) / 16),
+    key[i] = Bucket(c->device, mo_mode(c), nct, pow2_ceil(nchunk), pow2_ceil((c->point_num + 15) / 16),
                     c->tables.slater ? 1 : 0);
   }
   for (int64_t i = 0; i < n; ++i) {

@@ -125,7 +125,9 @@ struct Context {
   std::vector<double> mo_coef;  // [mo][ao]
   uint32_t mo_uninit = 0x3;
   bool mo_provided = false;
-  int32_t mo_precision = QMCKL_AMD_MO_FP64;
+  // MO product precision set by qmckl_amd_set_mo_precision, or -1: by the numerical
+  // precision (mo_mode)
+  int32_t mo_precision = -1;
   // requested shape of the fused AO->MO product: column tiles per pass and waves per
   // workgroup (0: the default for the MO count)
   int32_t mo_tile_g = 0, mo_tile_nw = 0;
@@ -229,6 +231,14 @@ struct ProfScope {
 enum { SLOT_AO = 0, SLOT_MO = 1, SLOT_JPAIR = 2, SLOT_JEEN = 3, SLOT_JTOTAL = 4 };
 
 // ---- helpers shared by the api_*.cpp files ----------------------------------------
+// The MO product precision in use: the one set with qmckl_amd_set_mo_precision, else, as the
+// reference's optimised build switches its MO product to single precision at precision <= 26
+// (org/qmckl_mo.org:2028-2040), the 3xBF16 product there (more accurate than a single-precision
+// product and faster) and FP64 above.
+inline int32_t mo_mode(const Context* c) {
+  if (c->mo_precision >= 0) return c->mo_precision;
+  return c->precision <= 26 ? QMCKL_AMD_MO_BF16X3 : QMCKL_AMD_MO_FP64;
+}
 // A change of constant data (basis, MOs, Jastrow parameters, nuclei, precision).
 inline void touch_const(Context* c) {
   c->date += 1;

@@ -172,3 +172,31 @@ def test_tile_setter_refuses_unbuilt_shapes():
         assert _mo_tile(ctx) == (1, 4, 1)  # 7 MOs: one column tile, waves adjusted to the cap
     finally:
         q.context_destroy(ctx)
+
+
+def test_mo_precision_follows_the_numerical_precision_until_set():
+    """Unset, the MO product precision follows the numerical precision: FP64 above 26 bits,
+    3xBF16 at 26 or fewer (where the reference's optimised build switches the product to
+    single precision); once set it stays."""
+    import ctypes
+
+    s, x, W = S.workload("small", walk_num=1)
+    ctx = q.context_create()
+    try:
+        S.load(ctx, s)
+        L = q.lib()
+        m = ctypes.c_int32()
+        q.check(ctx, L.qmckl_amd_get_mo_precision(ctx, ctypes.byref(m)))
+        assert m.value == q.MO_FP64
+        q.check(ctx, L.qmckl_set_numprec_precision(ctx, 26))
+        q.check(ctx, L.qmckl_amd_get_mo_precision(ctx, ctypes.byref(m)))
+        assert m.value == q.MO_BF16X3
+        q.check(ctx, L.qmckl_set_numprec_precision(ctx, 27))
+        q.check(ctx, L.qmckl_amd_get_mo_precision(ctx, ctypes.byref(m)))
+        assert m.value == q.MO_FP64
+        q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, q.MO_FP32))
+        q.check(ctx, L.qmckl_set_numprec_precision(ctx, 20))
+        q.check(ctx, L.qmckl_amd_get_mo_precision(ctx, ctypes.byref(m)))
+        assert m.value == q.MO_FP32
+    finally:
+        q.context_destroy(ctx)

@@ -251,3 +251,21 @@ def test_slater_basis_vs_oracle():
         assert q.lib().qmckl_get_ao_basis_ao_hessian(ctx, out.ctypes.data, out.size) == q.QMCKL_FAILURE
     finally:
         q.context_destroy(ctx)
+
+
+def test_low_precision_default_product():
+    """At a numerical precision of 24 bits, with no MO precision set, the product runs in
+    3xBF16 (the reference's optimised build switches to single precision at 26 bits or
+    fewer): within the split-precision contract of the FP64 oracle at the same precision."""
+    s, x, W = S.workload("small", walk_num=16)
+    s.precision = 24
+    ctx = make_ctx(s, x, W)
+    try:
+        P, nmo = W * s.elec_num, s.mo_num_kept
+        mo = q.get_array(ctx, "qmckl_get_mo_basis_mo_vgl", P * 5 * nmo).reshape(P, 5, nmo)
+        ref = oracle.mo_vgl(oracle.kept_mo_coef(s), oracle.ao_vgl(s, x))
+        assert_split_contract(mo, ref, "precision 24, default product")
+        assert np.abs(mo - ref).max() > 0.0  # not the FP64 product
+        vgl_close(_mo(ctx, s, W, q.MO_FP64), ref, 1e-12, "precision 24, FP64 set")
+    finally:
+        q.context_destroy(ctx)
