@@ -1923,6 +1923,375 @@ __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, co
 }
 
 // ------------------------------------------------------------------------------------
+// e-e-n on FP64 matrix cores, pass-based and balanced (cord 4 and 5, systems too large for
+// the resident kernel): the products of k_jas_een_mfma, taken over two 16-nucleus tiles at
+// a time.  The (k, q) units of a tile are grouped as in the resident kernel -- pairs (q, q + 1)
+// of the same k, which share the A operand (ten matrix instructions per A), and single
+// units -- and the groups of the two tiles are dealt to 8 waves so that the two waves of
+// every SIMD (waves w and w + 4) carry the same matrix load: at C = 5 each wave one pair and
+// waves 0-3 one single (10 + 5 = 15 and 10 products per step, 25 per SIMD); at C = 4 waves
+// 0-3 one pair, waves 4-7 one single (15 per SIMD).  One wave per (k, q), as in
+// k_jas_een_mfma, puts 10 waves on 4 SIMDs, 3, 3, 2 and 2 of them.  The accumulators of a
+// wave's groups live across the passes over electrons j; the fold is that of k_jas_een_mfma.
+// ------------------------------------------------------------------------------------
+constexpr int EB_WAVES = 8;
+
+// the groups of wave w: pair (tile, index) and single (tile, index); tile -1: none
+template <int C>
+constexpr int bal_pair_tile(int w) {
+  return C == 5 ? w / 4 : (w < 4 ? w / 2 : -1);
+}
+template <int C>
+constexpr int bal_pair_idx(int w) {
+  return C == 5 ? w % 4 : w % 2;
+}
+template <int C>
+constexpr int bal_single_tile(int w) {
+  return C == 5 ? (w < 4 ? w / 2 : -1) : (w >= 4 ? (w - 4) / 2 : -1);
+}
+template <int C>
+constexpr int bal_single_idx(int w) {
+  return C == 5 ? w % 2 : (w - 4) % 2;
+}
+
+// Dynamic LDS of k_jas_een_bal, in doubles, for passes of JS electrons j.
+struct EenBalLds {
+  int JSP, xt, xj, pdG, pdR, gch, ep, r0t, st, cfl, red, tot, jrow, total;
+  __host__ __device__ EenBalLds(int JS, int C, int dim) : JSP(JS + 4) {
+    int o = 0;
+    xt = o;   o += EM * 3;                  // tile electrons
+    xj = o;   o += 3 * JSP;                 // electrons j of the pass, [3][j]
+    pdG = o;  o += JSP * EM;                // g_e(r_ij), [j][i]; 0 for j == i
+    pdR = o;  o += JSP * EM;                // 1 / r_ij, [j][i]; 0 for j == i
+    gch = o;  o += 2 * JSP * 16;            // g_a(r_j) of the two nucleus tiles, [tile][j][a]
+    ep = o;   o += 2 * (C + 5) * EM * 16;   // per tile and (i, a): g^1..g^C, h, 2 kappa/R, kappa^2
+    r0t = o;  o += EM * (C - 1) * 5;        // q = 0 sums of the tile electrons
+    st = o;   o += 2 * 16 * (C - 1);        // per tile: sum over all j of g_a(r_j)^q
+    cfl = o;  o += 2 * dim * 16;            // per tile: c-coefficients, [n][a]
+    red = o;  o += EB_WAVES * EM * 5;
+    tot = o;  o += EM * 5;                  // running (J, gl) per row, summed over nuclei
+    jrow = o; o += EM;
+    total = o;
+  }
+};
+
+// y^n for n = 1..4 (wave-uniform n) by selects: y, y^2, y^2 y, y^2 y^2 -- the products of the
+// repeated multiplication of een_mfma_pass for n <= 3, (y y)(y y) for n = 4
+__device__ __forceinline__ double een_pow4(double y, int n) {
+  const double y2 = y * y;
+  const double y3 = y2 * y, y4 = y2 * y2;
+  return n == 1 ? y : n == 2 ? y2 : n == 3 ? y3 : y4;
+}
+
+// One pass of the groups of a wave over nsteps x 4 electrons j: a pair (K; Q, Q + 1) on
+// nucleus tile gp_tile (accumulators a, b) when has_pair, and a single unit (SK, SQ) on
+// tile gs_tile (accumulator s) when has_single.  (k, q) are wave-uniform run-time values:
+// one code path for all waves keeps the register count of a wave's own work (an 8-way
+// compile-time dispatch cost about 70 registers more).
+__device__ __forceinline__ void een_bal_pass(d4 (&a)[5], d4 (&b)[5], d4 (&s)[5], int has_pair, int K, int Q,
+                                             const double* gp_tile, int has_single, int SK, int SQ,
+                                             const double* gs_tile, const double* pdG, const double* pdR,
+                                             const double* xj, const double* xi, int JSP, int nsteps, int li,
+                                             int lg) {
+#pragma unroll 1
+  for (int st = 0; st < nsteps; ++st) {
+    const int jj = 4 * st + lg;
+    const double Ge = pdG[jj * EM + li], ri = pdR[jj * EM + li];
+    const double dx = xi[0] - xj[jj], dy = xi[1] - xj[JSP + jj], dz = xi[2] - xj[2 * JSP + jj];
+    if (has_pair) {
+      const double g = gp_tile[jj * 16 + li];
+      const double Gk = een_pow4(Ge, K), gq = een_pow4(g, Q);
+      const double gq1 = gq * g;
+      const double t = Gk * ri;
+      const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        a[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, a[c], 0, 0, 0);
+        b[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq1, b[c], 0, 0, 0);
+      }
+    }
+    if (has_single) {
+      const double g = gs_tile[jj * 16 + li];
+      const double Gk = een_pow4(Ge, SK), gq = een_pow4(g, SQ);
+      const double t = Gk * ri;
+      const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
+#pragma unroll
+      for (int c = 0; c < 5; ++c) s[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, s[c], 0, 0, 0);
+    }
+  }
+}
+
+// The fold of unit (K, Q) at row register r into R (the roles of een_fold; the q = 0 sums
+// when Q = 1, the k = 0 sums when K = 1).
+template <int C, int K, int Q>
+__device__ __forceinline__ void een_bal_fold(EenRow& R, const d4 (&acc)[5], int r, const double* cf, const double* gp,
+                                             const double* r0row, const double* srow, double kee) {
+  double X[5], XR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, X0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    const d4 x = acc[c];
+    X[c] = r == 0 ? x[0] : r == 1 ? x[1] : r == 2 ? x[2] : x[3];
+  }
+  if (Q == 1)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) XR[c] = r0row[(K - 1) * 5 + c];
+  // the subtrahend is g^Q rounded (fma with 0), so that S - g^Q is exactly 0 when electron
+  // i is the only one
+  if (K == 1) X0[0] = srow[Q - 1] - __builtin_fma(gp[Q - 1], gp[1], 0.0);
+  een_fold<C, K, Q>(R, cf, 16, gp, X, XR, X0, kee);
+}
+
+// (J, gradient, Laplacian) of a row from its folded sums and the (i, a) factors e (stride es)
+__device__ __forceinline__ void een_bal_out(const EenRow& R, const double* e, int es, double out[5]) {
+  const double h0 = e[0], h1 = e[es], h2 = e[2 * es], kr = e[3 * es], ka2 = e[4 * es];
+  out[0] += R.sJ;
+  out[1] += h0 * R.s1 - R.t0[0];
+  out[2] += h1 * R.s1 - R.t0[1];
+  out[3] += h2 * R.s1 - R.t0[2];
+  out[4] += ka2 * R.s2 - kr * R.s1 + R.s3 - 2.0 * (h0 * R.t1[0] + h1 * R.t1[1] + h2 * R.t1[2]) - 2.0 * R.sL;
+}
+
+// The groups of wave W at compile time (for its folds).
+template <int C, int W>
+struct EenBalWave {
+  static constexpr int PT = bal_pair_tile<C>(W), ST = bal_single_tile<C>(W);
+  static constexpr int PK = PT >= 0 ? een_pair_k(C, bal_pair_idx<C>(W)) : 1;
+  static constexpr int PQ = PT >= 0 ? een_pair_q(C, bal_pair_idx<C>(W)) : 1;
+  static constexpr int SK = ST >= 0 ? een_single_k(C, bal_single_idx<C>(W)) : 1;
+  static constexpr int SQ = ST >= 0 ? een_single_q(C, bal_single_idx<C>(W)) : 1;
+};
+
+// The folds of wave W's groups at row register r (row il = lg + 4 r) into out[5].
+template <int C, int W = 0>
+__device__ __forceinline__ void een_bal_dispatch_fold(int wave, const d4 (&pa)[5], const d4 (&pb)[5],
+                                                      const d4 (&sa)[5], int r, int il, int li, int a0, int Nn,
+                                                      int ntile, const double* ep, const double* cfl,
+                                                      const double* r0t, const double* st, double kee,
+                                                      double out[5]) {
+  if constexpr (W < EB_WAVES) {
+    if (wave == W) {
+      typedef EenBalWave<C, W> B;
+      constexpr int K1 = C - 1, DIM = Lkm<C>::dim, EPS = EM * 16;
+      auto group = [&](int tile, auto fold) {
+        if (tile >= ntile || a0 + 16 * tile + li >= Nn) return;
+        const double* e = ep + tile * (C + 5) * EPS + il * 16 + li;
+        double gp[C + 1];
+        gp[0] = 1.0;
+#pragma unroll
+        for (int p = 1; p <= C; ++p) gp[p] = e[(p - 1) * EPS];
+        EenRow R;
+        fold(R, gp, cfl + tile * DIM * 16 + li, st + tile * 16 * K1 + li * K1);
+        een_bal_out(R, e + C * EPS, EPS, out);
+      };
+      if constexpr (B::PT >= 0)
+        group(B::PT, [&](EenRow& R, const double* gp, const double* cf, const double* srow) {
+          een_bal_fold<C, B::PK, B::PQ>(R, pa, r, cf, gp, r0t + il * K1 * 5, srow, kee);
+          een_bal_fold<C, B::PK, B::PQ + 1>(R, pb, r, cf, gp, r0t + il * K1 * 5, srow, kee);
+        });
+      if constexpr (B::ST >= 0)
+        group(B::ST, [&](EenRow& R, const double* gp, const double* cf, const double* srow) {
+          een_bal_fold<C, B::SK, B::SQ>(R, sa, r, cf, gp, r0t + il * K1 * 5, srow, kee);
+        });
+      return;
+    }
+    een_bal_dispatch_fold<C, W + 1>(wave, pa, pb, sa, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
+  }
+}
+
+template <int C>
+__device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasOut& o, const double* __restrict__ R0,
+                                                 const double* __restrict__ gN, const double* __restrict__ Sg,
+                                                 double* __restrict__ Jt, int ntiles, int JS, const int64_t b) {
+  static_assert(C == 4 || C == 5, "balanced pass-based kernel: cord 4 and 5");
+  constexpr int K1 = C - 1;
+  constexpr int NT = 64 * EB_WAVES;
+  constexpr int DIM = Lkm<C>::dim;
+  constexpr int EPS = EM * 16;
+  extern __shared__ double lds[];
+  const EenBalLds L(JS, C, DIM);
+  double* xt = lds + L.xt;
+  double* xj = lds + L.xj;
+  double* pdG = lds + L.pdG;
+  double* pdR = lds + L.pdR;
+  double* gch = lds + L.gch;
+  double* ep = lds + L.ep;
+  double* r0t = lds + L.r0t;
+  double* st = lds + L.st;
+  double* cfl = lds + L.cfl;
+  double* red = lds + L.red;
+  double* tot = lds + L.tot;
+  double* jrow = lds + L.jrow;
+  const int Ne = (int)v.elec_num, Nn = (int)v.nucl_num;
+  int64_t w;
+  int tile;
+  een_block_work(b, v.walk_num, ntiles, w, tile);
+  const int i0 = tile * EM;
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double kee = v.rescale_ee;
+  const double* coord = v.coord;
+  const int64_t cld = v.coord_ld;
+  const int npass = (Ne + JS - 1) / JS;
+  for (int t = tid; t < EM * 3; t += NT) {
+    const int il = t / 3, d = t - 3 * il;
+    xt[t] = (i0 + il < Ne) ? coord[w * Ne + i0 + il + d * cld] : 0.0;
+  }
+  for (int t = tid; t < EM * K1 * 5; t += NT) {
+    const int il = t / (K1 * 5);
+    r0t[t] = (i0 + il < Ne) ? R0[(w * Ne + i0) * K1 * 5 + t] : 0.0;
+  }
+  for (int t = tid; t < EM * 5; t += NT) tot[t] = 0.0;
+  __syncthreads();
+  const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
+  // this wave's groups: pair tile and (k, q), single tile and (k, q); tile -1: none
+  int g_pt = -1, g_pk = 1, g_pq = 1, g_st = -1, g_sk = 1, g_sq = 1;
+  for (int wv = 0; wv < EB_WAVES; ++wv)
+    if (wv == wave) {
+      g_pt = bal_pair_tile<C>(wv);
+      g_st = bal_single_tile<C>(wv);
+      if (g_pt >= 0) {
+        g_pk = een_pair_k(C, bal_pair_idx<C>(wv));
+        g_pq = een_pair_q(C, bal_pair_idx<C>(wv));
+      }
+      if (g_st >= 0) {
+        g_sk = een_single_k(C, bal_single_idx<C>(wv));
+        g_sq = een_single_q(C, bal_single_idx<C>(wv));
+      }
+    }
+
+  for (int a0 = 0; a0 < Nn; a0 += 32) {
+    const int ntile = a0 + 16 < Nn ? 2 : 1;
+    // per-tile constants of the two tiles: c-coefficients, the k = 0 sums, and per (i, a)
+    // the powers of g_a(r_i) and the factors of their derivatives
+    for (int t = tid; t < 2 * DIM * 16; t += NT) {
+      const int tt = t / (DIM * 16), n = (t >> 4) % DIM, al = t & 15, a = a0 + 16 * tt + al;
+      cfl[t] = (a < Nn) ? v.cfull[(int64_t)n * Nn + a] : 0.0;
+    }
+    for (int t = tid; t < 2 * 16 * K1; t += NT) {
+      const int a = a0 + t / K1;
+      st[t] = (a < Nn) ? Sg[(w * Nn + a0) * K1 + t] : 0.0;
+    }
+    for (int t = tid; t < 2 * EPS; t += NT) {
+      const int tt = t / EPS, u = t - tt * EPS, il = u >> 4, al = u & 15, i = i0 + il, a = a0 + 16 * tt + al;
+      double gp[C + 1], h[3] = {0.0, 0.0, 0.0}, kr = 0.0, ka2 = 0.0;
+#pragma unroll
+      for (int p = 0; p <= C; ++p) gp[p] = 0.0;
+      if (i < Ne && a < Nn) {
+        const double ka = v.kappa_en[v.type_nucl[a]];
+        const double dx = xt[il * 3] - v.nucl_coord[a], dy = xt[il * 3 + 1] - v.nucl_coord[a + Nn],
+                     dz = xt[il * 3 + 2] - v.nucl_coord[a + 2 * Nn];
+        const double R = sqrt(dx * dx + dy * dy + dz * dz);
+        const double rinv = 1.0 / R;
+        const double g1 = gN[gn_index(w, a, i, gn_na(Nn), gn_jp(Ne))];  // exp(-kappa_a R)
+        gp[1] = g1;
+#pragma unroll
+        for (int p = 2; p <= C; ++p) gp[p] = gp[p - 1] * g1;
+        h[0] = -ka * (dx * rinv);
+        h[1] = -ka * (dy * rinv);
+        h[2] = -ka * (dz * rinv);
+        kr = 2.0 * ka * rinv;
+        ka2 = ka * ka;
+      }
+      double* e = ep + tt * (C + 5) * EPS + u;
+#pragma unroll
+      for (int p = 1; p <= C; ++p) e[(p - 1) * EPS] = gp[p];
+      e[(C + 0) * EPS] = h[0];
+      e[(C + 1) * EPS] = h[1];
+      e[(C + 2) * EPS] = h[2];
+      e[(C + 3) * EPS] = kr;
+      e[(C + 4) * EPS] = ka2;
+    }
+
+    d4 pa[5], pb[5], sa[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) pa[c] = pb[c] = sa[c] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int pass = 0; pass < npass; ++pass) {
+      const int j0 = pass * JS;
+      const int jn = min(JS, Ne - j0);
+      const int jn4 = (jn + 3) & ~3;
+      if (npass > 1 || a0 == 0) {
+        if (npass > 1) __syncthreads();  // previous pass done with the pair data
+        for (int t = tid; t < 3 * jn4; t += NT) {
+          const int d = t / jn4, jj = t - d * jn4;
+          xj[d * L.JSP + jj] = (jj < jn) ? coord[w * Ne + j0 + jj + d * cld] : 0.0;
+        }
+        for (int t = tid; t < EM * jn4; t += NT) {
+          const int il = t & (EM - 1), jj = t / EM;
+          const int i = i0 + il, j = j0 + jj;
+          double Ge = 0.0, ri = 0.0;
+          if (i < Ne && jj < jn && i != j) {
+            const double dx = xt[il * 3] - coord[w * Ne + j];
+            const double dy = xt[il * 3 + 1] - coord[w * Ne + j + cld];
+            const double dz = xt[il * 3 + 2] - coord[w * Ne + j + 2 * cld];
+            double d;
+            dist_and_inverse(dx * dx + dy * dy + dz * dz, d, ri);
+            Ge = exp_neg_deg<13, true>(-kee * d);
+          }
+          pdG[t] = Ge;  // [jj][il]
+          pdR[t] = ri;
+        }
+      }
+      for (int t = tid; t < 2 * 16 * jn4; t += NT) {  // consecutive threads: consecutive doubles
+        const int tt = t / (16 * jn4), u = t - tt * 16 * jn4, jj = u >> 4, al = u & 15;
+        const int a = a0 + 16 * tt + al;
+        gch[tt * L.JSP * 16 + jj * 16 + al] =
+            (a < Nn && jj < jn) ? gN[gn_index(w, a, j0 + jj, gn_na(Nn), gn_jp(Ne))] : 0.0;
+      }
+      __syncthreads();
+      een_bal_pass(pa, pb, sa, g_pt >= 0 && g_pt < ntile, g_pk, g_pq, gch + max(g_pt, 0) * L.JSP * 16,
+                   g_st >= 0 && g_st < ntile, g_sk, g_sq, gch + max(g_st, 0) * L.JSP * 16, pdG, pdR, xj, xi,
+                   L.JSP, jn4 >> 2, li, lg);
+      if (pass + 1 < npass) __syncthreads();  // before gch of the next pass is written
+    }
+
+    // Epilogue: lane (li, lg) holds rows lg + 4 r (register r), column a = a0 + 16 tile + li.
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      const int il = lg + 4 * r, i = i0 + il;
+      double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (i < Ne) een_bal_dispatch_fold<C>(wave, pa, pb, sa, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
+      // sum over the 16 nuclei of the tile(s): lanes li = 0..15 of this row group
+#pragma unroll
+      for (int c = 0; c < 5; ++c) out[c] = row_sum16(out[c]);
+      if (li == 0)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) red[(wave * EM + il) * 5 + c] = out[c];
+    }
+    __syncthreads();
+    for (int t = tid; t < EM * 5; t += NT) {
+      double x = tot[t];
+#pragma unroll
+      for (int wv = 0; wv < EB_WAVES; ++wv) x += red[wv * EM * 5 + t];
+      tot[t] = x;
+    }
+    // the next tile pair rewrites cfl, st, ep and gch only after this barrier
+    __syncthreads();
+  }
+  for (int t = tid; t < EM * 5; t += NT) {
+    const int il = t / 5, c = t - 5 * il, i = i0 + il;
+    if (c == 0)
+      jrow[il] = (i < Ne) ? tot[t] : 0.0;
+    else if (i < Ne)
+      o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = tot[t];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sj = 0.0;
+    for (int il = 0; il < EM; ++il) sj += jrow[il];
+    Jt[w * ntiles + tile] = sj;
+  }
+}
+
+template <int C>
+__global__ void __launch_bounds__(64 * EB_WAVES) k_jas_een_bal(JasDevView v, JasOut o, const double* __restrict__ R0,
+                                                              const double* __restrict__ gN,
+                                                              const double* __restrict__ Sg, double* __restrict__ Jt,
+                                                              int ntiles, int JS) {
+  jas_een_bal_body<C>(v, o, R0, gN, Sg, Jt, ntiles, JS, blockIdx.x);
+}
+
+// ------------------------------------------------------------------------------------
 // Grouped launches over the Jastrow evaluations of several contexts (a batch of molecules):
 // one launch per kernel (and cord), in which workgroup b serves job m, the one whose block
 // range [starts[m], starts[m + 1]) holds b, as that job's workgroup b - starts[m] of
@@ -1966,6 +2335,14 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma_batch(cons
   jas_een_mfma_body<C>(J.v, J.o, J.R0, J.gN, J.Sg, J.Jt, J.ntiles, J.JC, (int64_t)blockIdx.x - starts[m]);
 }
 
+template <int C>
+__global__ void __launch_bounds__(64 * EB_WAVES) k_jas_een_bal_batch(const JasJob* __restrict__ jobs,
+                                                                    const int* __restrict__ starts, int njobs) {
+  const int m = job_of(starts, njobs, blockIdx.x);
+  const JasJob& J = jobs[m];
+  jas_een_bal_body<C>(J.v, J.o, J.R0, J.gN, J.Sg, J.Jt, J.ntiles, J.JC, (int64_t)blockIdx.x - starts[m]);
+}
+
 // The persistent resident kernel over several contexts: each context gets its own share of
 // the workgroups (one per CU), in proportion to its work, and runs its (walker, tile) items
 // on them as a launch of its own would.
@@ -1987,6 +2364,7 @@ __global__ void __launch_bounds__(kBlock) k_jas_total_batch(const JasJob* __rest
 struct EenShape {
   bool mfma;  // matrix-core kernel (cord <= EMFMA_MAX_C)
   bool res;   // its resident variant (everything of a walker tile in LDS)
+  bool bal;   // else, at cord 4 and 5, the balanced pass-based variant
   int TI, JC, ntiles;
   size_t lds;
 };
@@ -2002,8 +2380,17 @@ EenShape een_shape(const JasDevView& v) {
     const int C = (int)v.cord;
     s.TI = EM;
     const int ne4 = (Ne + 3) & ~3;
+    const size_t res_lds0 = sizeof(double) * (size_t)EenResLds(gn_jp(Ne), 16 * gn_na(Nn), C, (int)v.dim_c).total;
+    // the balanced variant where k_jas_een_mfma needs more than one pass over j as well (it
+    // holds fewer electrons per pass: with one pass, about 230-300 electrons at C = 5, the
+    // one-wave-per-unit kernel builds the tile's pair records once and is as fast or faster)
+    const bool mfma_one_pass =
+        sizeof(double) * (size_t)EenMfmaLds(ne4, C, (int)v.dim_c, C * (C - 1) / 2).total <= 160 * 1024;
+    s.bal = res_lds0 > 160 * 1024 && (C == 4 || C == 5) && !mfma_one_pass;
     auto fits = [&](int js) {
-      return sizeof(double) * (size_t)EenMfmaLds(js, C, (int)v.dim_c, C * (C - 1) / 2).total <= 160 * 1024;
+      const size_t n = s.bal ? (size_t)EenBalLds(js, C, (int)v.dim_c).total
+                             : (size_t)EenMfmaLds(js, C, (int)v.dim_c, C * (C - 1) / 2).total;
+      return sizeof(double) * n <= 160 * 1024;
     };
     int npass = 1, js = ne4;
     while (!fits(js) && js > 4) {
@@ -2012,12 +2399,14 @@ EenShape een_shape(const JasDevView& v) {
     }
     s.JC = js;
     s.ntiles = (Ne + EM - 1) / EM;
-    const size_t res_lds = sizeof(double) * (size_t)EenResLds(gn_jp(Ne), 16 * gn_na(Nn), C, (int)v.dim_c).total;
-    s.res = res_lds <= 160 * 1024;
-    s.lds = s.res ? res_lds : sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, C * (C - 1) / 2).total;
+    s.res = res_lds0 <= 160 * 1024;
+    s.lds = s.res   ? res_lds0
+            : s.bal ? sizeof(double) * (size_t)EenBalLds(s.JC, C, (int)v.dim_c).total
+                    : sizeof(double) * (size_t)EenMfmaLds(s.JC, C, (int)v.dim_c, C * (C - 1) / 2).total;
     return s;
   }
   s.res = false;
+  s.bal = false;
   s.TI = kBlock / Nn;
   if (s.TI < 1) s.TI = 1;
   if (s.TI > Ne) s.TI = Ne;
@@ -2332,12 +2721,18 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       if (ea == hipSuccess)                                                                                    \
         ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_res<CC>),                            \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
+      if (ea == hipSuccess && (CC == 4 || CC == 5))                                                            \
+        ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_bal<CC == 4 ? 4 : 5>),                \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
       if (ea != hipSuccess) return ea;                                                                         \
       attr_set = true;                                                                                         \
     }                                                                                                          \
     if (sh.res)                                                                                                \
       hipLaunchKernelGGL(k_jas_een_res<CC>, dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES),     \
                          sh.lds, s, v, o, sv.Jt, sv.JtP, sh.ntiles);                                           \
+    else if (sh.bal)                                                                                           \
+      hipLaunchKernelGGL(k_jas_een_bal<CC == 4 ? 4 : 5>, grid, dim3(64 * EB_WAVES), sh.lds, s, v, o, sv.R0,      \
+                         sv.gN, sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                \
     else                                                                                                       \
       hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,    \
                          sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                       \
@@ -2388,7 +2783,7 @@ hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_
 // ---- grouped launches of a batch of contexts -------------------------------------------
 namespace {
 
-enum { JB_PAIR = 0, JB_EEN_VALU = 1, JB_EEN_MFMA = 2, JB_EEN_RES = 3, JB_TOTAL = 4 };
+enum { JB_PAIR = 0, JB_EEN_VALU = 1, JB_EEN_MFMA = 2, JB_EEN_BAL = 3, JB_EEN_RES = 4, JB_TOTAL = 5 };
 
 template <class K>
 hipError_t allow_lds(K kernel) {
@@ -2416,6 +2811,13 @@ hipError_t launch_batch_c(const JasBatchLaunch& l, hipStream_t s) {
       if constexpr (C >= 2 && C <= EMFMA_MAX_C) {
         if ((e = allow_lds(&k_jas_een_mfma_batch<C>)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_jas_een_mfma_batch<C>, g, b, l.lds, s, l.d_jobs, l.d_starts, l.njobs);
+        break;
+      }
+      return hipErrorInvalidValue;
+    case JB_EEN_BAL:
+      if constexpr (C == 4 || C == 5) {
+        if ((e = allow_lds(&k_jas_een_bal_batch<C>)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_jas_een_bal_batch<C>, g, b, l.lds, s, l.d_jobs, l.d_starts, l.njobs);
         break;
       }
       return hipErrorInvalidValue;
@@ -2507,6 +2909,8 @@ hipError_t jastrow_batch_build(const std::vector<std::pair<JasDevView, JasOut>>&
       if (fused)
         add(JB_EEN_RES, 0, 64 * EQ_WAVES, sh.lds,
             (double)W * sh.ntiles * gn_jp(Ne) * 16.0 * gn_na(v.nucl_num));
+      else if (sh.bal)
+        add(JB_EEN_BAL, W * sh.ntiles, 64 * EB_WAVES, sh.lds, 0.0);
       else if (sh.mfma)
         add(JB_EEN_MFMA, W * sh.ntiles, 64 * (C * (C - 1) / 2), sh.lds, 0.0);
       else

@@ -267,15 +267,18 @@ def test_jastrow_synthetic_vs_oracle(name, W):
     (20, 19, 18, 5, 5, 2), (20, 19, 18, 3, 4, 3), (17, 24, 16, 5, 5, 4), (33, 40, 39, 2, 6, 6),
     (20, 19, 18, 5, 5, 7), (9, 12, 11, 4, 3, 8), (3, 3, 2, 5, 5, 5), (1, 1, 0, 5, 5, 5),
     (25, 30, 29, 5, 5, 5), (32, 20, 20, 4, 4, 4), (18, 21, 20, 5, 5, 3),
-    (40, 120, 120, 5, 5, 5), (20, 200, 200, 5, 5, 6)])
+    (40, 120, 120, 5, 5, 5), (20, 200, 200, 5, 5, 6), (37, 180, 170, 5, 5, 5), (50, 200, 190, 4, 4, 4)])
 def test_jastrow_orders_and_shapes_vs_oracle(nn, up, dn, aord, bord, cord):
     """Every e-e-n order the kernels are built for (2..8), with electron and nucleus counts
     off the 16-wide tiles and the 32-electron chunks, down to a single electron.  In the
     LDS-resident e-e-n kernel a last nucleus tile of at most 4 nuclei runs on the 4x4x4
     products (20, 17, 33, 3, 1, 18 nuclei), with both units of a pair in one product when it
     holds at most 2 (17, 33, 1, 18), one of 9 or 16 nuclei on the 16x16x4 ones (25, 32).
-    The last two shapes do not fit the resident kernel: 240 electrons x 40 nuclei run the
-    pass-based kernel in one pass over j, 400 electrons in two passes."""
+    The last four shapes do not fit the resident kernel: 240 electrons x 40 nuclei run the
+    pass-based kernel (one wave per (k, q) unit) in one pass over j, 400 electrons at cord 6
+    the same kernel in two passes; 350 electrons x 37 nuclei (cord 5) and 390 x 50 (cord 4)
+    the balanced pass-based kernel in two passes, the last of its pairs of nucleus tiles
+    holding one tile (of 5 and 2 nuclei)."""
     s, x, W = S.jastrow_workload(nn, up, dn, aord, bord, cord, walk_num=8 if up + dn < 200 else 2)
     ctx = make_ctx(s, x, W, mo=False)
     _compare_jastrow(_jastrow_gpu(ctx, W, s.elec_num), oracle.jastrow(s, x, W), s.name)
