@@ -140,16 +140,16 @@ def main():
                 for k in ("ee", "en", "een"))
         return -0.5 * (g[:, 3] + g[:, 0] ** 2 + g[:, 1] ** 2 + g[:, 2] ** 2).sum(axis=1)
 
-    def kernel_times(mode, n=3):
+    def kernel_times(mode, n=3, fn=step):
         q.check(ctx, L.qmckl_amd_set_mo_precision(ctx, mode))
-        step()
+        fn()
         q.synchronize(ctx)
         ms = np.zeros(5)
         cnt = np.zeros(5, np.int64)
         q.check(ctx, L.qmckl_amd_set_profiling(ctx, 1))
         q.check(ctx, L.qmckl_amd_get_kernel_times(ctx, q.ptr(ms), q.ptr(cnt), 5))
         for _ in range(n):
-            step()
+            fn()
         q.check(ctx, L.qmckl_amd_get_kernel_times(ctx, q.ptr(ms), q.ptr(cnt), 5))
         q.check(ctx, L.qmckl_amd_set_profiling(ctx, 0))
         return {SLOTS[i]: {"ms_per_step": ms[i] / n, "launches_per_step": int(cnt[i]) // n} for i in range(5)}
@@ -167,6 +167,16 @@ def main():
     log(f"{args.mode}: {tsp * 1e3:.3f} ms/step")
     ksp = kernel_times(modes[args.mode])
     q.synchronize(ctx)
+    # qmckl_get_mo_basis_mo_value's own path: the values-only product (no AO derivatives)
+    mv = q.DeviceBuffer(8 * P * Nmo)
+
+    def value_step():
+        q.check(ctx, L.qmckl_amd_set_electron_coord_device(ctx, b"T", W, xd.ptr, x.size))
+        q.check(ctx, L.qmckl_amd_get_mo_basis_mo_value_device(ctx, mv.ptr, P * Nmo))
+
+    kval = kernel_times(modes[args.mode], fn=value_step)["mo_gemm"]
+    mv.free()
+    log(f"values-only MO product ({args.mode}): {kval['ms_per_step']:.3f} ms")
     # MO error of the split product per VGL component: max |split - FP64| / max |FP64| over the
     # component (the contract: <= 1e-7 value and gradient, <= 1e-6 Laplacian), and the
     # largest per-entry relative error |split - FP64| / |FP64| over the entries with
@@ -257,6 +267,9 @@ def main():
                       "mo_product": mo_rate(costs, W, k64, "fp64")},
         "split_path": {"kernels": ksp, "speedup_vs_fp64": t64 / tsp,
                        "mo_product": mo_rate(costs, W, ksp, args.mode)},
+        "mo_values_only": {"ms_per_launch": kval["ms_per_step"] / max(1, kval["launches_per_step"]),
+                           "note": "qmckl_get_mo_basis_mo_value without a current MO VGL: AO values "
+                                   "without derivatives, a fifth of the product's matrix work"},
         "mo_err_vs_fp64_max_normalised": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
         "mo_rel_err_vs_fp64_above_1e-3_of_max": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_rel)),
         "jastrow_max_abs_dJ_vs_fp64": dJ,

@@ -576,6 +576,11 @@ qmckl_exit_code qmckl_amd_get_ao_basis_ao_vgl_device(qmckl_context context, doub
                                                      const int64_t size_max);
 qmckl_exit_code qmckl_amd_get_mo_basis_mo_vgl_device(qmckl_context context, double* const d_mo_vgl,
                                                      const int64_t size_max);
+/* MO values [point][mo] alone (qmckl_get_mo_basis_mo_value into device memory): the value
+ * rows of a current MO VGL, else a values-only product (no AO derivatives, a fifth of the
+ * matrix work) equal bit for bit to the value rows of the full evaluation. */
+qmckl_exit_code qmckl_amd_get_mo_basis_mo_value_device(qmckl_context context, double* const d_mo_value,
+                                                       const int64_t size_max);
 qmckl_exit_code qmckl_amd_get_jastrow_champ_value_device(qmckl_context context, double* const d_value,
                                                          const int64_t size_max);
 qmckl_exit_code qmckl_amd_get_jastrow_champ_gl_device(qmckl_context context, double* const d_gl,

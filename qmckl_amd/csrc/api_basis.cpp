@@ -452,6 +452,7 @@ qmckl_exit_code mo_product_args(qmckl_context handle, Context* c, double* d_out,
   a.N = Nmo;
   a.cutoff = ao_view(c).cutoff;
   a.slater = c->tables.slater ? 1 : 0;
+  a.vo = 0;
   return QMCKL_SUCCESS;
 }
 
@@ -491,6 +492,43 @@ qmckl_exit_code provide_mo_vgl(qmckl_context handle, Context* c, double* d_out, 
     if (rc != QMCKL_SUCCESS) return rc;
   }
   if (d_out == nullptr) c->mo_vgl_date = c->date;
+  return QMCKL_SUCCESS;
+}
+
+// MO values [point][mo] into d_out (or into the context buffer when d_out is null);
+// org/qmckl_mo.org:988-1100.  The value rows of the MO VGL when it is current (as the
+// reference takes them), else the values-only product: AO values without derivatives and a
+// fifth of the matrix work, the same sums bit for bit.  With the cusp correction, or the
+// FP64 AO VGL resident, the full evaluation gives them.
+qmckl_exit_code provide_mo_value(qmckl_context handle, Context* c, double* d_out, const char* fn) {
+  if (!c->mo_provided) return fail(handle, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (d_out == nullptr && c->mo_value_date >= c->date) return QMCKL_SUCCESS;
+  const int64_t P = c->point_num, Nmo = c->mo_num;
+  double* out = d_out;
+  hipError_t e = hipSuccess;
+  if (out == nullptr) {
+    e = c->d_mo_value.reserve(sizeof(double) * std::max<int64_t>(P, 1) * Nmo);
+    if (e != hipSuccess) return hip_fail(handle, e, fn);
+    out = c->d_mo_value.as<double>();
+  }
+  const bool via_vgl = c->mo_vgl_date >= c->date || !c->r_cusp.empty() ||
+                       (mo_mode(c) == QMCKL_AMD_MO_FP64 && c->ao_vgl_date >= c->date);
+  if (via_vgl) {
+    qmckl_exit_code rc = provide_mo_vgl(handle, c, nullptr, fn);
+    if (rc != QMCKL_SUCCESS) return rc;
+    e = hipMemcpy2DAsync(out, sizeof(double) * Nmo, c->d_mo_vgl.ptr, sizeof(double) * 5 * Nmo, sizeof(double) * Nmo,
+                         P, hipMemcpyDeviceToDevice, c->stream);
+  } else {
+    AoMoArgs a;
+    int mode = 0;
+    qmckl_exit_code rc = mo_product_args(handle, c, out, fn, a, mode);
+    if (rc != QMCKL_SUCCESS) return rc;
+    a.vo = 1;
+    ProfScope ps(c, SLOT_MO);
+    e = launch_ao_mo(a, mode, c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  if (d_out == nullptr) c->mo_value_date = c->date;
   return QMCKL_SUCCESS;
 }
 
@@ -1012,13 +1050,9 @@ qmckl_exit_code qmckl_get_mo_basis_mo_value(qmckl_context context, double* const
   if (mo_value == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
   const int64_t n = c->point_num * c->mo_num;
   if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
-  qmckl_exit_code rc = provide_mo_vgl(context, c, nullptr, fn);
+  qmckl_exit_code rc = provide_mo_value(context, c, nullptr, fn);
   if (rc != QMCKL_SUCCESS) return rc;
-  hipError_t e = hipMemcpy2DAsync(mo_value, sizeof(double) * c->mo_num, c->d_mo_vgl.ptr, sizeof(double) * 5 * c->mo_num,
-                                  sizeof(double) * c->mo_num, c->point_num, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return hip_fail(context, e, fn);
-  return QMCKL_SUCCESS;
+  return copy_out(c, context, c->d_mo_value.ptr, mo_value, sizeof(double) * n, fn);
 }
 
 qmckl_exit_code qmckl_get_mo_basis_mo_value_inplace(qmckl_context context, double* const mo_value, const int64_t size_max) {
@@ -1033,6 +1067,7 @@ qmckl_exit_code qmckl_amd_set_mo_precision(qmckl_context context, const int32_t 
   std::lock_guard<std::recursive_mutex> g(c->mutex);
   c->mo_precision = mode;
   c->mo_vgl_date = 0;
+  c->mo_value_date = 0;
   return QMCKL_SUCCESS;
 }
 
@@ -1087,6 +1122,19 @@ qmckl_exit_code qmckl_amd_get_mo_basis_mo_vgl_device(qmckl_context context, doub
   const int64_t n = c->point_num * 5 * c->mo_num;
   if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
   return provide_mo_vgl(context, c, d_mo_vgl, fn);
+}
+
+qmckl_exit_code qmckl_amd_get_mo_basis_mo_value_device(qmckl_context context, double* const d_mo_value,
+                                                       const int64_t size_max) {
+  const char* fn = "qmckl_amd_get_mo_basis_mo_value_device";
+  Context* c = ctx_of(context);
+  if (!c) return QMCKL_INVALID_CONTEXT;
+  std::lock_guard<std::recursive_mutex> g(c->mutex);
+  if (!c->mo_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
+  if (d_mo_value == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
+  const int64_t n = c->point_num * c->mo_num;
+  if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "input array too small");
+  return provide_mo_value(context, c, d_mo_value, fn);
 }
 
 }  // extern "C"

@@ -199,7 +199,7 @@ std::vector<DevBuf*> device_buffers(Context* c) {
                     &d.ao_factor, &d.nucl_range, &d.nucl_coord, &d.coef_t, &d.coef_ch64,
                     &d.coef_ch32, &d.coef_chbf, &d.ao_desc, &d.ao_desc_off, &d.jas_a, &d.jas_b,
                     &d.jas_cfull, &d.jas_cfull_pad, &d.jas_nuc_pad, &d.jas_type, &d.jas_kappa_en,
-                    &c->d_coord, &c->d_ao_vgl, &c->d_ao_work, &c->d_mo_vgl, &c->d_ee_dist,
+                    &c->d_coord, &c->d_ao_vgl, &c->d_ao_work, &c->d_mo_vgl, &c->d_mo_value, &c->d_ee_dist,
                     &c->d_en_dist, &c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl,
                     &c->d_feen_gl, &c->d_jvalue, &c->d_jgl, &c->d_jscratch, &c->d_jint,
                     &d.jas_asymp_a, &d.prim_nucl_c, &d.prim_expo_c, &d.jas_charge,

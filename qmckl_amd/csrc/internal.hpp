@@ -159,6 +159,8 @@ struct Context {
   DevBuf d_ao_work;  // chunk scratch for the AO->MO product
   DevBuf d_mo_vgl;   // [P][5][mo]
   uint64_t mo_vgl_date = 0;
+  DevBuf d_mo_value;  // [P][mo]
+  uint64_t mo_value_date = 0;
 
   DevBuf d_ee_dist, d_en_dist;
   uint64_t ee_dist_date = 0, en_dist_date = 0;
@@ -335,6 +337,7 @@ struct AoMoArgs {
   int b_chunk_bytes, G, ngroup;
   int nw;      // waves per workgroup: 4 (G <= 8) or 8 (5 <= G <= 16)
   int slater;  // Slater shells (the shell records carry r_power)
+  int vo;      // the MO values alone into C [point][mo] (single launches only, not batches)
   double* C;
   int64_t ldc, N;
   double cutoff;

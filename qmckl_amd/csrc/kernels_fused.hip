@@ -158,6 +158,13 @@ __device__ __forceinline__ void chunk_ao_vgl(double x, double y, double z, int p
   r[3] = pv[3] * s0 + pv[0] * s3;
   r[4] = pv[4] * s0 + pv[0] * s4 + 2.0 * (pv[1] * s1 + pv[2] * s2 + pv[3] * s3);
 }
+// The value alone of chunk_ao_vgl (the same products in the same order).
+__device__ __forceinline__ double chunk_ao_value(double x, double y, double z, int pa, int pb, int pc,
+                                                 const double* rs) {
+  double pv[5];
+  monomial_vgl(x + 1.e-20, y + 1.e-20, z + 1.e-20, pa, pb, pc, pv);
+  return pv[0] * rs[0];
+}
 // The kernel.  TP = 16 points per workgroup (5 row tiles, point-major rows 5 p + c), KC = 32
 // AOs per chunk.  The 5 x G products of a chunk (row tile r, column tile j) go to the waves
 // as: wave w takes
@@ -165,7 +172,6 @@ __device__ __forceinline__ void chunk_ao_vgl(double x, double y, double z, int p
 // (r, 4) are spread over the waves by r mod 4 (at most 7 accumulator tiles per wave).
 constexpr int TP = 16;
 constexpr int ROWS = 5 * TP;   // product rows per workgroup (point-major, row = 5 p + c)
-constexpr int RT = ROWS / 16;  // 16-row MFMA tiles
 constexpr int SH = 6;          // doubles per staged (point, shell): 5 radial VGL + padding
 // points between those of one thread in the AO step, with NW waves per workgroup
 template <int NW>
@@ -182,20 +188,11 @@ constexpr int KP16 = AO_MO_BF_LD;
 constexpr int LDA64 = KC + 2;
 constexpr int LDA32 = KC + 4;
 
-template <int MODE>
-struct TileA;
-template <>
-struct TileA<0> {
-  static constexpr size_t bytes = sizeof(double) * ROWS * LDA64;
-};
-template <>
-struct TileA<1> {
-  static constexpr size_t bytes = sizeof(float) * ROWS * LDA32;
-};
-template <>
-struct TileA<2> {
-  static constexpr size_t bytes = sizeof(uint16_t) * 3 * ROWS * KP16;
-};
+// Bytes of the operand tile of NR rows (ROWS, or TP for the values alone)
+template <int MODE, int NR>
+constexpr size_t tile_bytes() {
+  return MODE == 0 ? sizeof(double) * NR * LDA64 : MODE == 1 ? sizeof(float) * NR * LDA32 : sizeof(uint16_t) * 3 * NR * KP16;
+}
 
 // six leading cross products of a (3 planes) x b (3 planes) in FP32: the leading one
 // (hi x hi) into `hi`, the five smaller ones, smallest first, into `lo`
@@ -209,9 +206,9 @@ __device__ __forceinline__ void bf16x3_product(const bf8 (&a)[3], const bf8 (&b)
   lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], lo, 0, 0, 0);
 }
 
-// Product of the operand tile's row tile r with column tile j of the chunk's C_t, added
-// into acc (FP64 for every mode).
-template <int MODE, int G>
+// Product of row tile r of the operand tile (NR rows) with column tile j of the chunk's C_t,
+// added into acc (FP64 for every mode).
+template <int MODE, int G, int NR>
 __device__ __forceinline__ void tile_product(d4& acc, const void* at, const void* bchunk, int r, int j, int li,
                                              int lg) {
   constexpr int NG16 = 16 * G;
@@ -238,7 +235,7 @@ __device__ __forceinline__ void tile_product(d4& acc, const void* at, const void
     bf8 a[3], b[3];
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
-      a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(A + pl * ROWS * KP16 + (16 * r + li) * KP16 + 8 * lg));
+      a[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(A + pl * NR * KP16 + (16 * r + li) * KP16 + 8 * lg));
       b[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const s8*>(B + ((pl * NG16) + 16 * j + li) * KP16 + 8 * lg));
     }
     f4 hi, lo;
@@ -260,8 +257,15 @@ constexpr bool second_tile() {
 
 // Evaluates point tile `tile` x column group `group` of the product described by A (SL:
 // Slater shells, a compile-time branch so that the Gaussian kernels keep their registers).
-template <int MODE, int G, int NW, bool SL>
+// VO: the MO values alone (org/qmckl_mo.org:988-1100): the radial and AO values without
+// their derivatives, a tile of 16 rows (one per point) and one row tile per column tile,
+// results [point][mo]; each value is the same sum of the same products as the value row of
+// the full evaluation, so both agree bit for bit.
+template <int MODE, int G, int NW, bool SL, bool VO>
 __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int tile, const int group, double* lds) {
+  constexpr int NC = VO ? 1 : 5;          // components per point
+  constexpr int NR = NC * TP;             // operand tile rows (row = NC p + c)
+  constexpr int RT = NR / 16;             // 16-row MFMA tiles
   constexpr int NMAIN = G < NW ? G : NW;  // waves with a first column tile
   constexpr bool TWO = second_tile<G, NW>();
   constexpr int QS = qstep<NW>();
@@ -272,7 +276,7 @@ __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int til
   double* xt = lds + 2 * A.desc_max;                 // [TP][3]
   int64_t* pidx = reinterpret_cast<int64_t*>(xt + 3 * TP);  // [TP] point of each tile row
   double* sh = xt + 4 * TP;                          // [TP][ns_max][SH]
-  void* at = sh + TP * A.ns_max * SH;                // operand tile (TileA<MODE>)
+  void* at = sh + TP * A.ns_max * SH;                // operand tile (tile_bytes<MODE, NR>)
   const int64_t tp0 = A.p0 + (int64_t)tile * TP;  // first position p' of the tile
   const int ntp = (int)min((int64_t)TP, A.p0 + A.np - tp0);
   const char* bsrc = static_cast<const char*>(A.B) + (int64_t)group * A.nchunk * A.b_chunk_bytes;
@@ -314,12 +318,12 @@ __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int til
       if (q < ntp) live |= (int)lv;
       double* o = sh + (q * A.ns_max + s) * SH;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) o[k] = r[k];
+      for (int k = 0; k < NC; ++k) o[k] = r[k];
     }
     // every shell of the chunk screened at every point of the tile: all its AO values are
     // exact zeros and add nothing to the products (rows past the tile's points are zero too)
     if (!__syncthreads_or(live)) continue;
-    // 2. AO VGL of the chunk into the operand tile, row = 5 p + c: a thread keeps AO
+    // 2. AO VGL of the chunk into the operand tile, row = NC p + c: a thread keeps AO
     //    kk = tid mod KC (its descriptor decoded once per chunk) for the points
     //    q = tid / KC + QS m
     const int kk = tid & (KC - 1);
@@ -329,22 +333,26 @@ __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int til
     const double nxa = c.nuc[4 * anl], nya = c.nuc[4 * anl + 1], nza = c.nuc[4 * anl + 2];
     for (int q = tid / KC; q < TP; q += QS) {
       double r[5];
-      chunk_ao_vgl(xt[3 * q] - nxa, xt[3 * q + 1] - nya, xt[3 * q + 2] - nza, pa, pb, pc,
-                   sh + (q * A.ns_max + ash) * SH, r);
+      if constexpr (VO)
+        r[0] = chunk_ao_value(xt[3 * q] - nxa, xt[3 * q + 1] - nya, xt[3 * q + 2] - nza, pa, pb, pc,
+                              sh + (q * A.ns_max + ash) * SH);
+      else
+        chunk_ao_vgl(xt[3 * q] - nxa, xt[3 * q + 1] - nya, xt[3 * q + 2] - nza, pa, pb, pc,
+                     sh + (q * A.ns_max + ash) * SH, r);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < NC; ++k) {
         if constexpr (MODE == 0) {
-          static_cast<double*>(at)[(5 * q + k) * LDA64 + kk] = r[k];
+          static_cast<double*>(at)[(NC * q + k) * LDA64 + kk] = r[k];
         } else if constexpr (MODE == 1) {
-          static_cast<float*>(at)[(5 * q + k) * LDA32 + kk] = (float)r[k];
+          static_cast<float*>(at)[(NC * q + k) * LDA32 + kk] = (float)r[k];
         } else {
           uint16_t h, m, l;
           split3(r[k], h, m, l);
           uint16_t* T = static_cast<uint16_t*>(at);
-          const int idx = (5 * q + k) * KP16 + kk;
+          const int idx = (NC * q + k) * KP16 + kk;
           T[idx] = h;
-          T[ROWS * KP16 + idx] = m;
-          T[2 * ROWS * KP16 + idx] = l;
+          T[NR * KP16 + idx] = m;
+          T[2 * NR * KP16 + idx] = l;
         }
       }
     }
@@ -354,15 +362,15 @@ __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int til
     const void* bch = bsrc + (int64_t)ch * A.b_chunk_bytes;
     if (has1)
 #pragma unroll
-      for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc1[r], at, bch, r, wave, li, lg);
+      for (int r = 0; r < RT; ++r) tile_product<MODE, G, NR>(acc1[r], at, bch, r, wave, li, lg);
     if constexpr (TWO) {
       if (has2)
 #pragma unroll
-        for (int r = 0; r < RT; ++r) tile_product<MODE, G>(acc2[r], at, bch, r, wave + NW, li, lg);
+        for (int r = 0; r < RT; ++r) tile_product<MODE, G, NR>(acc2[r], at, bch, r, wave + NW, li, lg);
     } else if constexpr (NW == 4 && G == 5) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
-        if (wave + 4 * x < RT) tile_product<MODE, G>(acc2[x], at, bch, wave + 4 * x, 4, li, lg);
+        if (wave + 4 * x < RT) tile_product<MODE, G, NR>(acc2[x], at, bch, wave + 4 * x, 4, li, lg);
     }
   }
   // result rows: f64 MFMA register k holds row lg + 4 k of the tile, f32 (and bf16) 4 lg + k
@@ -372,8 +380,8 @@ __device__ __forceinline__ void ao_mo_tile_body(const AoMoArgs& A, const int til
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int row = 16 * r + (MODE == 0 ? lg + 4 * k : 4 * lg + k);
-      const int q = row / 5;
-      if (row < 5 * ntp) A.C[(pidx[q] * 5 + (row - 5 * q)) * A.ldc + col] = v[k];
+      const int q = row / NC;
+      if (row < NC * ntp) A.C[(pidx[q] * NC + (row - NC * q)) * A.ldc + col] = v[k];
     }
   };
   if (has1)
@@ -397,10 +405,10 @@ constexpr int ao_mo_min_blocks() {
   return NW == 8 ? 1 : (G > 5 ? 2 : (MODE == 0 ? 3 : 4));
 }
 
-template <int MODE, int G, int NW, bool SL>
+template <int MODE, int G, int NW, bool SL, bool VO>
 __global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>())) k_ao_mo_tile(AoMoArgs A) {
   extern __shared__ __attribute__((aligned(16))) double lds[];
-  ao_mo_tile_body<MODE, G, NW, SL>(A, blockIdx.x, blockIdx.y, lds);
+  ao_mo_tile_body<MODE, G, NW, SL, VO>(A, blockIdx.x, blockIdx.y, lds);
 }
 
 // One launch over the products of several contexts (a batch of molecules): workgroup b
@@ -416,7 +424,7 @@ __global__ void __launch_bounds__(64 * NW, (ao_mo_min_blocks<MODE, G, NW>()))
   const int m = __builtin_amdgcn_readfirstlane((int)(it & 0xffffu));
   const int tile = __builtin_amdgcn_readfirstlane((int)((it >> 16) & 0xffffffffu));
   const int group = __builtin_amdgcn_readfirstlane((int)(it >> 48));
-  ao_mo_tile_body<MODE, G, NW, SL>(args[m], tile, group, lds);
+  ao_mo_tile_body<MODE, G, NW, SL, false>(args[m], tile, group, lds);
 }
 
 // Dynamic LDS limit of a kernel: the CU's 160 KiB less its static LDS (the block-wide OR of
@@ -436,13 +444,13 @@ hipError_t prepare_lds(K kernel, size_t lds, size_t& max_dyn) {
   return lds > max_dyn ? hipErrorInvalidValue : hipSuccess;
 }
 
-template <int MODE, int G, int NW, bool SL>
+template <int MODE, int G, int NW, bool SL, bool VO>
 hipError_t launch_g(const AoMoArgs& a, size_t lds, hipStream_t s) {
   static size_t max_dyn = 0;
-  hipError_t e = prepare_lds(&k_ao_mo_tile<MODE, G, NW, SL>, lds, max_dyn);
+  hipError_t e = prepare_lds(&k_ao_mo_tile<MODE, G, NW, SL, VO>, lds, max_dyn);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)((a.np + TP - 1) / TP), (unsigned)a.ngroup);
-  hipLaunchKernelGGL((k_ao_mo_tile<MODE, G, NW, SL>), grid, dim3(64 * NW), lds, s, a);
+  hipLaunchKernelGGL((k_ao_mo_tile<MODE, G, NW, SL, VO>), grid, dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
@@ -493,7 +501,7 @@ struct RunOne {
   size_t lds;
   hipStream_t s;
   template <int M, int G, int NW, bool SL>
-  hipError_t run() { return launch_g<M, G, NW, SL>(a, lds, s); }
+  hipError_t run() { return a.vo ? launch_g<M, G, NW, SL, true>(a, lds, s) : launch_g<M, G, NW, SL, false>(a, lds, s); }
 };
 
 struct RunBatch {
@@ -509,8 +517,9 @@ struct RunBatch {
 }  // namespace
 
 size_t ao_mo_lds_bytes(const AoMoArgs& a, int mode) {
-  return sizeof(double) * (2 * (size_t)a.desc_max + 4 * TP + (size_t)TP * a.ns_max * SH) +
-         (mode == 0 ? TileA<0>::bytes : mode == 1 ? TileA<1>::bytes : TileA<2>::bytes);
+  const size_t tile = a.vo ? (mode == 0 ? tile_bytes<0, TP>() : mode == 1 ? tile_bytes<1, TP>() : tile_bytes<2, TP>())
+                           : (mode == 0 ? tile_bytes<0, ROWS>() : mode == 1 ? tile_bytes<1, ROWS>() : tile_bytes<2, ROWS>());
+  return sizeof(double) * (2 * (size_t)a.desc_max + 4 * TP + (size_t)TP * a.ns_max * SH) + tile;
 }
 
 bool ao_mo_variant_ok(int mode, int G, int NW, int slater) {
@@ -548,8 +557,8 @@ hipError_t mo_batch_build(const std::vector<std::pair<AoMoArgs, int>>& in, MoBat
     const AoMoArgs& a = in[i].first;
     const int mode = in[i].second;
     if (a.np <= 0) continue;
-    if (!ao_mo_variant_ok(mode, a.G, a.nw, a.slater) || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC || a.desc_max % 2 ||
-        a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te ||
+    if (!ao_mo_variant_ok(mode, a.G, a.nw, a.slater) || a.vo || a.nchunk < 1 || a.ns_max < 1 || a.ns_max > KC ||
+        a.desc_max % 2 || a.b_chunk_bytes % 16 || a.tw < 1 || a.te < 1 || a.p0 < 0 || a.p0 + a.np > a.tw * a.te ||
         ao_mo_lds_bytes(a, mode) > 160 * 1024)
       return hipErrorInvalidValue;
     by_shape[std::make_tuple(mode, a.G, a.nw, a.slater)].push_back(i);

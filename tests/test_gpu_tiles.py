@@ -99,3 +99,49 @@ def test_autotune_picks_built_shapes_and_keeps_results():
     finally:
         for ctx in ctxs:
             q.context_destroy(ctx)
+
+
+VALUE_CASES = {
+    # workload, shapes (col tiles, waves; 0 = default) to run the values-only product with
+    "medium": (lambda: S.workload("medium", walk_num=4), [(0, 0), (8, 8), (2, 4)]),
+    "molecule-300e": (lambda: S.molecule_workload(300, walk_num=4), [(0, 0), (4, 4)]),
+    "slater": (lambda: S.slater_workload(walk_num=8), [(0, 0), (1, 4)]),
+}
+
+
+@pytest.mark.parametrize("name", list(VALUE_CASES))
+def test_values_only_product(name):
+    """qmckl_get_mo_basis_mo_value with no current MO VGL runs the values-only product (AO
+    values without derivatives, one row tile per column tile): in every precision and shape
+    its values equal the value rows of the full MO VGL bit for bit, the device getter gives
+    the same, and the FP64 values match the oracle to 1e-12 of scale."""
+    make, shapes = VALUE_CASES[name]
+    s, x, W = make()
+    P, nmo = W * s.elec_num, s.mo_num_kept
+    print(f"{name}: Ne={s.elec_num} Nao={s.ao_num} Nmo={nmo} W={W}", flush=True)
+    ctx = q.context_create()
+    dev = q.DeviceBuffer(8 * P * nmo)
+    try:
+        S.load(ctx, s)
+        ref = oracle.mo_vgl(oracle.kept_mo_coef(s), oracle.ao_vgl(s, x))[:, 0, :]
+        for mode in (q.MO_FP64, q.MO_FP32, q.MO_BF16X3):
+            q.check(ctx, q.lib().qmckl_amd_set_mo_precision(ctx, mode))
+            for g, w in shapes:
+                q.check(ctx, q.lib().qmckl_amd_set_mo_tile(ctx, g, w))
+                S.set_coords(ctx, s, x, W)  # no current MO VGL: the values-only product runs
+                val = q.get_array(ctx, "qmckl_get_mo_basis_mo_value", P * nmo).reshape(P, nmo)
+                S.set_coords(ctx, s, x, W)
+                q.check(ctx, q.lib().qmckl_amd_get_mo_basis_mo_value_device(ctx, dev.ptr, P * nmo))
+                q.check(ctx, q.lib().qmckl_amd_synchronize(ctx))
+                dval = dev.to_array((P, nmo))
+                vgl = _mo(ctx, s, W)
+                print(f"  mode {mode} shape {_shape(ctx)}: max |value - vgl row| "
+                      f"{np.abs(val - vgl[:, 0, :]).max():.1e}", flush=True)
+                assert np.array_equal(val, vgl[:, 0, :]), f"{name} mode {mode} shape {(g, w)}"
+                assert np.array_equal(dval, val)
+                if mode == q.MO_FP64:
+                    assert np.abs(val - ref).max() <= 1e-12 * np.abs(ref).max()
+        q.check(ctx, q.lib().qmckl_amd_set_mo_tile(ctx, 0, 0))
+    finally:
+        dev.free()
+        q.context_destroy(ctx)
