@@ -177,6 +177,15 @@ def main():
     kval = kernel_times(modes[args.mode], fn=value_step)["mo_gemm"]
     mv.free()
     log(f"values-only MO product ({args.mode}): {kval['ms_per_step']:.3f} ms")
+
+    # qmckl_get_jastrow_champ_value's own path: the values-only e-e-n evaluation (after the
+    # first call, which follows GL requests and so evaluates everything)
+    def jvalue_step():
+        q.check(ctx, L.qmckl_amd_set_electron_coord_device(ctx, b"T", W, xd.ptr, x.size))
+        q.check(ctx, L.qmckl_amd_get_jastrow_champ_value_device(ctx, val.ptr, W))
+
+    kjv = kernel_times(modes[args.mode], fn=jvalue_step)["jastrow_een"]
+    log(f"values-only Jastrow: {kjv['ms_per_step']:.3f} ms")
     # MO error of the split product per VGL component: max |split - FP64| / max |FP64| over the
     # component (the contract: <= 1e-7 value and gradient, <= 1e-6 Laplacian), and the
     # largest per-entry relative error |split - FP64| / |FP64| over the entries with
@@ -270,6 +279,10 @@ def main():
         "mo_values_only": {"ms_per_launch": kval["ms_per_step"] / max(1, kval["launches_per_step"]),
                            "note": "qmckl_get_mo_basis_mo_value without a current MO VGL: AO values "
                                    "without derivatives, a fifth of the product's matrix work"},
+        "jastrow_values_only": {"een_ms_per_launch": kjv["ms_per_step"] / max(1, kjv["launches_per_step"]),
+                                "note": "qmckl_get_jastrow_champ_value alone: the value component of the "
+                                        "e-e-n products only, no GL (LDS-resident kernel sizes; else the "
+                                        "full evaluation)"},
         "mo_err_vs_fp64_max_normalised": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
         "mo_rel_err_vs_fp64_above_1e-3_of_max": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_rel)),
         "jastrow_max_abs_dJ_vs_fp64": dJ,

@@ -316,6 +316,10 @@ qmckl_exit_code qmckl_get_jastrow_champ_factor_een_gl(qmckl_context context, dou
 /* org/qmckl_jastrow_champ.org:14403 */
 qmckl_exit_code qmckl_get_jastrow_champ_factor_een_grad(qmckl_context context, double* const factor_een_grad,
                                                         const int64_t size_max);
+/* The value and factor getters evaluate the values alone (no GL; one of the five e-e-n
+ * products per step, where the LDS-resident e-e-n kernel applies) unless the caller asked for
+ * the GL after its previous value request, in which case they run the full evaluation that
+ * the GL request will read; the values are the same bit for bit either way. */
 qmckl_exit_code qmckl_get_jastrow_champ_value(qmckl_context context, double* const value, const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_gl(qmckl_context context, double* const gl, const int64_t size_max);
 qmckl_exit_code qmckl_get_jastrow_champ_grad(qmckl_context context, double* const grad, const int64_t size_max);

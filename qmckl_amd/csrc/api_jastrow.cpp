@@ -165,7 +165,7 @@ qmckl_exit_code get_result(qmckl_context context, Which which, double* out, int6
   const bool is_gl = which == F_EE_GL || which == F_EN_GL || which == F_EEN_GL || which == J_GL;
   const int64_t n = is_gl ? W * 4 * Ne : W;
   if (size_max < n) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
-  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  qmckl_exit_code rc = is_gl ? provide_jastrow(context, c, fn) : provide_jastrow_value(context, c, fn);
   if (rc != QMCKL_SUCCESS) return rc;
   const DevBuf* src[] = {&c->d_fee, &c->d_fen, &c->d_feen, &c->d_fee_gl, &c->d_fen_gl, &c->d_feen_gl, &c->d_jvalue, &c->d_jgl};
   return copy_out(c, context, src[which]->ptr, out, n, fn);
@@ -486,7 +486,18 @@ bool jastrow_view_resident(Context* c, int64_t walk_num, const double* coord, Ja
 
 // Value, factors and their gradients/Laplacians of all walkers in one pass over the
 // device; org/qmckl_jastrow_champ.org:16481-16560 (value) and :16887-16960 (gl).
+namespace {
+qmckl_exit_code provide_jastrow_full(qmckl_context handle, Context* c, const char* fn);
+}
+
 qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn) {
+  c->jgl_asked_date = c->date;
+  return provide_jastrow_full(handle, c, fn);
+}
+
+namespace {
+// Factors, e^J and every GL in one evaluation (pair, e-e-n and total kernels).
+qmckl_exit_code provide_jastrow_full(qmckl_context handle, Context* c, const char* fn) {
   if (c->jastrow_date >= c->date) return QMCKL_SUCCESS;
   qmckl_exit_code rc = jas_ready(c, handle, fn);
   if (rc != QMCKL_SUCCESS) return rc;
@@ -525,6 +536,44 @@ qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn
   }
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   c->jastrow_date = c->date;
+  return QMCKL_SUCCESS;
+}
+}  // namespace
+
+// The factors ee, en, een and e^J alone, as the reference's value providers give them
+// (org/qmckl_jastrow_champ.org:16700-16820: the value from the three factors, no GL): one
+// resident e-e-n launch on the value component of its products only, bit-identical to the
+// values of the full evaluation.  Where that kernel does not apply, and when the caller asked
+// for the GL after its previous value request (it will most likely again: a QMC step reads
+// both), the full evaluation, so that the GL request that follows finds it done.
+qmckl_exit_code provide_jastrow_value(qmckl_context handle, Context* c, const char* fn) {
+  if (c->jastrow_date >= c->date || c->jvalue_date >= c->date) return QMCKL_SUCCESS;
+  const bool gl_too = c->jgl_asked_date != 0 && c->jgl_asked_date >= c->jval_asked_date;
+  c->jval_asked_date = c->date;
+  qmckl_exit_code rc = jas_ready(c, handle, fn);
+  if (rc != QMCKL_SUCCESS) return rc;
+  const JasDevView v = jas_view(c);
+  if (gl_too || !jastrow_values_only_ok(v)) return provide_jastrow_full(handle, c, fn);
+  const int64_t W = c->walk_num;
+  hipError_t e = hipSuccess;
+  DevBuf* w1[] = {&c->d_fee, &c->d_fen, &c->d_feen, &c->d_jvalue};
+  for (DevBuf* b : w1)
+    if (e == hipSuccess) e = b->reserve(sizeof(double) * W);
+  if (e == hipSuccess) e = c->d_jscratch.reserve(sizeof(double) * std::max<int64_t>(jastrow_scratch_doubles(v), 1));
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  JasOut o;
+  o.fee = c->d_fee.as<double>();
+  o.fen = c->d_fen.as<double>();
+  o.feen = c->d_feen.as<double>();
+  o.fee_gl = o.fen_gl = o.feen_gl = o.gl = nullptr;  // not touched by the values-only kernels
+  o.value = c->d_jvalue.as<double>();
+  o.scratch = c->d_jscratch.as<double>();
+  {
+    ProfScope ps(c, SLOT_JEEN);
+    e = launch_jastrow_values(v, o, c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(handle, e, fn);
+  c->jvalue_date = c->date;
   return QMCKL_SUCCESS;
 }
 
@@ -990,7 +1039,7 @@ qmckl_exit_code qmckl_amd_get_jastrow_champ_value_device(qmckl_context context, 
   if (d_value == nullptr) return fail(context, QMCKL_INVALID_ARG_2, fn, "NULL pointer");
   if (!c->jas_provided) return fail(context, QMCKL_NOT_PROVIDED, fn, nullptr);
   if (size_max < c->walk_num) return fail(context, QMCKL_INVALID_ARG_3, fn, "Array too small");
-  qmckl_exit_code rc = provide_jastrow(context, c, fn);
+  qmckl_exit_code rc = provide_jastrow_value(context, c, fn);
   if (rc != QMCKL_SUCCESS) return rc;
   hipError_t e = hipMemcpyAsync(d_value, c->d_jvalue.ptr, sizeof(double) * c->walk_num, hipMemcpyDeviceToDevice, c->stream);
   if (e != hipSuccess) return hip_fail(context, e, fn);

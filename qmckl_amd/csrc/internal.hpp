@@ -171,6 +171,11 @@ struct Context {
   DevBuf d_jvalue, d_jgl;                // [W], [W][4][Ne]
   DevBuf d_jscratch;                     // partial sums
   uint64_t jastrow_date = 0;
+  uint64_t jvalue_date = 0;  // the factors and e^J alone (values-only evaluation)
+  // dates of the last GL request (any consumer of the full evaluation) and of the last
+  // values-only request that evaluated: a caller that asked for the GL after its previous
+  // value request gets the full evaluation on the next one
+  uint64_t jgl_asked_date = 0, jval_asked_date = 0;
   // Jastrow intermediates (materialised only when asked for)
   DevBuf d_jint;
   // derivative getters: intermediates (JasInter kinds, then e-n distances and the force
@@ -272,6 +277,8 @@ struct AoMoArgs;
 qmckl_exit_code mo_product_args(qmckl_context handle, Context* c, double* d_out, const char* fn, AoMoArgs& a,
                                 int& mode);
 qmckl_exit_code provide_jastrow(qmckl_context handle, Context* c, const char* fn);
+// The factors and e^J alone, without the GL where the values-only kernel applies
+qmckl_exit_code provide_jastrow_value(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_ee_distance(qmckl_context handle, Context* c, const char* fn);
 qmckl_exit_code provide_en_distance(qmckl_context handle, Context* c, const char* fn);
 // e_kin, e_pot, e_local, V_ee, V_en per walker in c->d_energy ([5][W]); kinetic: with the
@@ -417,6 +424,10 @@ JasOut jastrow_out(Context* c);
 hipError_t launch_jastrow_pair(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t s);
 hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_t s);
+// The factors and e^J alone (no GL) in one resident e-e-n launch and the total; only where
+// that kernel applies (jastrow_values_only_ok), else the full evaluation gives them.
+bool jastrow_values_only_ok(const JasDevView& v);
+hipError_t launch_jastrow_values(const JasDevView& v, const JasOut& o, hipStream_t s);
 // Grouped Jastrow launches over a batch of contexts (one launch per kernel and cord): built
 // once (device job tables allocated), launched per step, freed with the batch.
 struct JasBatchLaunch {

@@ -1155,10 +1155,12 @@ constexpr unsigned een_role_mask() {
 // VALUE_ONLY (the X[0][Q] roles, k = 0): the terms without kk.  F (een_role_mask) drops the
 // terms whose role sums are zero by construction: they would add exact zeros, which the
 // compiler must otherwise compute (kk or X could be infinite).
-template <bool VALUE_ONLY, unsigned F = 7u>
+// JONLY: J alone (the values-only evaluation), the same additions into o[0] in the same order.
+template <bool VALUE_ONLY, unsigned F = 7u, bool JONLY = false>
 __device__ __forceinline__ void een_apply(double (&o)[5], const EenTs& s, const double* X, double kk, const EenEp& e) {
   if constexpr ((F & 1u) != 0) {
     if constexpr ((F & 2u) != 0) o[0] += s.TJ * X[0];
+    if constexpr (JONLY) return;
     if constexpr ((F & 4u) != 0) {
       const double t1x = s.T1 * X[0];
       o[1] += e.h0 * t1x;
@@ -1258,8 +1260,9 @@ struct EenAccT<EEN_ROWS> {
 template <int MODE>
 using EenAcc = typename EenAccT<MODE>::type;
 
-template <int K, int Q, int NQ>
-__device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][5][2], const double* jr, const double* gt,
+// NC: components of the products, 5 (value, gradient, Laplacian sums) or 1 (the value alone)
+template <int K, int Q, int NQ, int NC>
+__device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][NC][2], const double* jr, const double* gt,
                                                   const double* xt, int nsteps, int li, int lg) {
   const int n4 = li & 3;
   const double* pj = jr + lg * JREC;
@@ -1286,7 +1289,7 @@ __device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][5][2], const
     for (int n = 0; n < NQ; ++n) {
       if (n > 0) gq *= g;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
+      for (int c = 0; c < NC; ++c) {
         acc[n][c][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(Aa[c], gq, acc[n][c][0], 0, 0, 0);
         acc[n][c][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(Ab[c], gq, acc[n][c][1], 0, 0, 0);
       }
@@ -1294,8 +1297,8 @@ __device__ __forceinline__ void een_res_pass_rows(double (&acc)[NQ][5][2], const
   }
 }
 
-template <int K, int Q, int NQ, int MODE>
-__device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const double* jr, const double* gt,
+template <int K, int Q, int NQ, int MODE, int NC>
+__device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][NC], const double* jr, const double* gt,
                                              const double (&xi)[3], int nsteps, int li, int lg) {
   constexpr bool MINI = MODE == EEN_MINI || MODE == EEN_MINI2;
   constexpr bool MINI2 = MODE == EEN_MINI2;
@@ -1328,7 +1331,7 @@ __device__ __forceinline__ void een_res_pass(EenAcc<MODE> (&acc)[NQ][5], const d
     for (int n = 0; n < NQ; ++n) {
       if (n > 0) gq *= g;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
+      for (int c = 0; c < NC; ++c) {
         if constexpr (MINI)
           acc[n][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[c], gq, acc[n][c], 0, 0, 0);
         else
@@ -1348,27 +1351,27 @@ struct EenResCtx {
 
 // The fold of unit (K, Q) of row il: its products X with the m roles, X[K][0] (the q = 0 sums
 // of the tile electrons) with the r roles when Q = 1, X[0][Q] with the z roles when K = 1.
-template <int C, int K, int Q>
+template <int C, int K, int Q, bool JONLY>
 __device__ __forceinline__ void een_fold_unit(const EenResCtx& x, int il, const double (&X)[5], const EenTs& tm,
                                               const EenTs& tr, const EenTs& tz, const double* gp, double kk,
                                               const EenEp& e, double (&out)[5]) {
   constexpr int K1 = C - 1;
-  een_apply<false, een_role_mask<C, K, Q, 0>()>(out, tm, X, kk, e);
+  een_apply<false, een_role_mask<C, K, Q, 0>(), JONLY>(out, tm, X, kk, e);
   if constexpr (Q == 1) {
     const double* XR = x.r0t + il * K1 * 5 + (K - 1) * 5;
-    een_apply<false, een_role_mask<C, K, Q, 1>()>(out, tr, XR, kk, e);
+    een_apply<false, een_role_mask<C, K, Q, 1>(), JONLY>(out, tr, XR, kk, e);
   }
   if constexpr (K == 1) {
     // X[0][Q], value only; g^Q rounded and kept out of contraction (see een_dispatch_fold)
     const double X0[1] = {x.st[x.a * K1 + Q - 1] - __builtin_fma(gp[Q - 1], gp[1], 0.0)};
-    een_apply<true, een_role_mask<C, K, Q, 2>()>(out, tz, X0, 0.0, e);
+    een_apply<true, een_role_mask<C, K, Q, 2>(), JONLY>(out, tz, X0, 0.0, e);
   }
 }
 
 // Fold of row r of the products (X, 5 components per unit of the group) of lane column
 // x.a into out: the role sums of each unit applied to its products, and with q = 1 or k = 1
 // the roles on the q = 0 and k = 0 sums.  gcol is the lane's column in the tile.
-template <int C, int K, int Q, int NQ>
+template <int C, int K, int Q, int NQ, bool JONLY>
 __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int gcol, const double (&X)[NQ][5],
                                                  double (&out)[5]) {
   const int i = min(x.i0 + il, x.Ne - 1);
@@ -1390,11 +1393,11 @@ __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int
     EenTs tm, tr, tz;
     if (n == 0) {
       een_tsums<C, K, Q>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
-      een_fold_unit<C, K, Q>(x, il, X[n], tm, tr, tz, gp, kk, e, out);
+      een_fold_unit<C, K, Q, JONLY>(x, il, X[n], tm, tr, tz, gp, kk, e, out);
     } else {
       constexpr int Q1 = NQ > 1 ? Q + 1 : Q;
       een_tsums<C, K, Q1>(tm, tr, tz, x.cfl + x.a, x.NA16, gp);
-      een_fold_unit<C, K, Q1>(x, il, X[n], tm, tr, tz, gp, kk, e, out);
+      een_fold_unit<C, K, Q1, JONLY>(x, il, X[n], tm, tr, tz, gp, kk, e, out);
     }
   }
 }
@@ -1405,65 +1408,68 @@ __device__ __forceinline__ void een_res_fold_row(const EenResCtx& x, int il, int
 // the row or the nucleus, so the rows interleave: a padding nucleus has zero c-coefficients,
 // kappa, 1/R, products and k = 0 sums and adds exactly 0; a row past the last electron is
 // discarded by the caller.
-template <int C, int K, int Q, int NQ, int MODE>
+template <int C, int K, int Q, int NQ, int MODE, int NC>
 __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (&xi)[3], double (&ra)[4][5]) {
   constexpr bool MINI = MODE == EEN_MINI;
+  constexpr bool JONLY = NC == 1;
   if constexpr (MODE == EEN_ROWS) {
-    double acc[NQ][5][2];
+    double acc[NQ][NC][2];
 #pragma unroll
     for (int n = 0; n < NQ; ++n)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) acc[n][c][0] = acc[n][c][1] = 0.0;
-    een_res_pass_rows<K, Q, NQ>(acc, x.jr, x.gt, x.xt, x.nsteps, x.li, x.lg);
+      for (int c = 0; c < NC; ++c) acc[n][c][0] = acc[n][c][1] = 0.0;
+    een_res_pass_rows<K, Q, NQ, NC>(acc, x.jr, x.gt, x.xt, x.nsteps, x.li, x.lg);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       double X[NQ][5];
 #pragma unroll
       for (int n = 0; n < NQ; ++n)
 #pragma unroll
-        for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c][h];
-      een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * h, x.li, X, ra[h]);
+        for (int c = 0; c < 5; ++c) X[n][c] = c < NC ? acc[n][c < NC ? c : 0][h] : 0.0;
+      een_res_fold_row<C, K, Q, NQ, JONLY>(x, x.lg + 4 * h, x.li, X, ra[h]);
     }
     return;
   } else if constexpr (MODE == EEN_MINI2) {
-    double acc[1][5] = {{0.0, 0.0, 0.0, 0.0, 0.0}};
-    een_res_pass<K, Q, 1, EEN_MINI2>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+    double acc[1][NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[0][c] = 0.0;
+    een_res_pass<K, Q, 1, EEN_MINI2, NC>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
     double X[2][5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      X[0][c] = acc[0][c];
-      X[1][c] = dpp_f64<DPP_XOR2>(acc[0][c]);  // column n ^ 2: power q + 1 of the same nucleus
+      X[0][c] = c < NC ? acc[0][c < NC ? c : 0] : 0.0;
+      X[1][c] = c < NC ? dpp_f64<DPP_XOR2>(acc[0][c < NC ? c : 0]) : 0.0;  // column n ^ 2: power q + 1
     }
     double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    een_res_fold_row<C, K, Q, 2>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, out);
+    een_res_fold_row<C, K, Q, 2, JONLY>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, out);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) ra[r][c] += (r == (x.li >> 2)) ? out[c] : 0.0;
+      for (int c = 0; c < NC; ++c) ra[r][c] += (r == (x.li >> 2)) ? out[c] : 0.0;
   } else {
-  EenAcc<MODE> acc[NQ][5];
+  EenAcc<MODE> acc[NQ][NC];
 #pragma unroll
   for (int n = 0; n < NQ; ++n)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if constexpr (MINI)
         acc[n][c] = 0.0;
       else
         acc[n][c] = d4{0.0, 0.0, 0.0, 0.0};
     }
-  een_res_pass<K, Q, NQ, MODE>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
+  een_res_pass<K, Q, NQ, MODE, NC>(acc, x.jr, x.gt, xi, x.nsteps, x.li, x.lg);
   if constexpr (MINI) {
     double X[NQ][5];
 #pragma unroll
     for (int n = 0; n < NQ; ++n)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c];
+      for (int c = 0; c < 5; ++c) X[n][c] = c < NC ? acc[n][c < NC ? c : 0] : 0.0;
     double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, out);
+    een_res_fold_row<C, K, Q, NQ, JONLY>(x, x.lg + 4 * (x.li >> 2), x.li & 3, X, out);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) ra[r][c] += (r == (x.li >> 2)) ? out[c] : 0.0;
+      for (int c = 0; c < NC; ++c) ra[r][c] += (r == (x.li >> 2)) ? out[c] : 0.0;
   } else {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -1471,14 +1477,14 @@ __device__ __forceinline__ void een_res_group(const EenResCtx& x, const double (
 #pragma unroll
       for (int n = 0; n < NQ; ++n)
 #pragma unroll
-        for (int c = 0; c < 5; ++c) X[n][c] = acc[n][c][r];
-      een_res_fold_row<C, K, Q, NQ>(x, x.lg + 4 * r, x.li, X, ra[r]);
+        for (int c = 0; c < 5; ++c) X[n][c] = c < NC ? acc[n][c < NC ? c : 0][r] : 0.0;
+      een_res_fold_row<C, K, Q, NQ, JONLY>(x, x.lg + 4 * r, x.li, X, ra[r]);
     }
   }
   }
 }
 
-template <int C, int G = 0>
+template <int C, int NC, int G = 0>
 __device__ __forceinline__ void een_res_dispatch(int g, int mode, const EenResCtx& x, const double (&xi)[3],
                                                  double (&ra)[4][5]) {
   constexpr int NPR = een_npair(C);
@@ -1489,19 +1495,19 @@ __device__ __forceinline__ void een_res_dispatch(int g, int mode, const EenResCt
       constexpr int NQ = G < NPR ? 2 : 1;
       if constexpr (NQ == 2) {
         if (mode == EEN_MINI2) {
-          een_res_group<C, K, Q, NQ, EEN_MINI2>(x, xi, ra);
+          een_res_group<C, K, Q, NQ, EEN_MINI2, NC>(x, xi, ra);
           return;
         }
       }
       if (mode == EEN_MINI || mode == EEN_MINI2)
-        een_res_group<C, K, Q, NQ, EEN_MINI>(x, xi, ra);
+        een_res_group<C, K, Q, NQ, EEN_MINI, NC>(x, xi, ra);
       else if (mode == EEN_ROWS)
-        een_res_group<C, K, Q, NQ, EEN_ROWS>(x, xi, ra);
+        een_res_group<C, K, Q, NQ, EEN_ROWS, NC>(x, xi, ra);
       else
-        een_res_group<C, K, Q, NQ, EEN_FULL>(x, xi, ra);
+        een_res_group<C, K, Q, NQ, EEN_FULL, NC>(x, xi, ra);
       return;
     }
-    een_res_dispatch<C, G + 1>(g, mode, x, xi, ra);
+    een_res_dispatch<C, NC, G + 1>(g, mode, x, xi, ra);
   }
 }
 
@@ -1541,10 +1547,14 @@ __device__ __forceinline__ void een_butterfly(double (&vec)[N], int l32) {
 // the per-walker inputs (g_a(r_j) of every nucleus, x_j, the q sums) are copied to LDS once
 // per walker instead of once per tile, the constant ones (nucleus table, c-coefficients)
 // once per workgroup, and no workgroup launch separates two tiles.
-template <int C>
+// VAL: the values alone (e-e, e-n and e-e-n factors): the products of the value component
+// only (one matrix instruction of five per step), the fold of J alone, no GL written; the
+// values equal those of the full evaluation bit for bit (the same operations on them).
+template <int C, bool VAL>
 __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasOut& o, double* __restrict__ Jt,
                                                  double* __restrict__ JtP, int ntiles, const int bx, const int gx) {
   constexpr int K1 = C - 1;
+  constexpr int NC = VAL ? 1 : 5;
   constexpr int NT = 64 * EQ_WAVES;
   constexpr int DIM = Lkm<C>::dim;
   extern __shared__ double lds[];
@@ -1772,7 +1782,7 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
         const int idx = l32 * (NVP / 32) + m;
         if (idx < 5 * K1) {
           r0t[il * K1 * 5 + idx] = vec[m];
-        } else if (idx < NV && i < Ne) {
+        } else if (!VAL && idx < NV && i < Ne) {
           double* gl = (idx < 5 * K1 + 4 ? o.fee_gl : o.fen_gl) + w * 4 * Ne;
           gl[(int64_t)((idx - 5 * K1) & 3) * Ne + i] = vec[m];
         }
@@ -1829,19 +1839,20 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
       x.ny = nuc[NA16 + x.a];
       x.nz = nuc[2 * NA16 + x.a];
       x.ka = nuc[3 * NA16 + x.a];
-      een_res_dispatch<C>(g, mode, x, xi, ra);
+      een_res_dispatch<C, NC>(g, mode, x, xi, ra);
     }
     // 4. sums over the nuclei (lanes li) and the waves
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const double y = row_sum16(ra[r][c]);
         if (li == 0) red[(wave * EM + lg + 4 * r) * 5 + c] = y;
       }
     __syncthreads();
     double* jrow = xt;  // free after the units
-    for (int t = tid; t < EM * 5; t += NT) {
+    for (int u = tid; u < EM * NC; u += NT) {
+      const int t = VAL ? 5 * u : u;  // entry (il, c) of the reduction rows
       double y = 0.0;
 #pragma unroll
       for (int wv = 0; wv < EQ_WAVES; ++wv) y += red[wv * EM * 5 + t];
@@ -1867,15 +1878,16 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
   }
 }
 
-template <int C>
+template <int C, bool VAL>
 __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res(JasDevView v, JasOut o, double* __restrict__ Jt,
                                                               double* __restrict__ JtP, int ntiles) {
-  jas_een_res_body<C>(v, o, Jt, JtP, ntiles, blockIdx.x, gridDim.x);
+  jas_een_res_body<C, VAL>(v, o, Jt, JtP, ntiles, blockIdx.x, gridDim.x);
 }
 
+// value_only: the factors and e^J alone (no GL read or written)
 __device__ __forceinline__ void jas_total_body(const JasDevView& v, const JasOut& o, const double* __restrict__ Jt,
                                                const double* __restrict__ JtP, int ntiles, int have_een, int fused,
-                                               const int64_t w) {
+                                               int value_only, const int64_t w) {
   __shared__ double sval;
   const int Ne = (int)v.elec_num;
   if (threadIdx.x == 0) {
@@ -1897,6 +1909,7 @@ __device__ __forceinline__ void jas_total_body(const JasDevView& v, const JasOut
     o.value[w] = val;
     sval = val;
   }
+  if (value_only) return;
   __syncthreads();
   const double val = sval;
   const double* a = o.fee_gl + w * 4 * Ne;
@@ -1918,8 +1931,8 @@ __device__ __forceinline__ void jas_total_body(const JasDevView& v, const JasOut
 
 __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, const double* __restrict__ Jt,
                                                       const double* __restrict__ JtP, int ntiles, int have_een,
-                                                      int fused) {
-  jas_total_body(v, o, Jt, JtP, ntiles, have_een, fused, blockIdx.x);
+                                                      int fused, int value_only) {
+  jas_total_body(v, o, Jt, JtP, ntiles, have_een, fused, value_only, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2351,14 +2364,14 @@ __global__ void __launch_bounds__(64 * EQ_WAVES) k_jas_een_res_batch(const JasJo
                                                                     const int* __restrict__ starts, int njobs) {
   const int m = job_of(starts, njobs, blockIdx.x);
   const JasJob& J = jobs[m];
-  jas_een_res_body<C>(J.v, J.o, J.Jt, J.JtP, J.ntiles, (int)blockIdx.x - starts[m], starts[m + 1] - starts[m]);
+  jas_een_res_body<C, false>(J.v, J.o, J.Jt, J.JtP, J.ntiles, (int)blockIdx.x - starts[m], starts[m + 1] - starts[m]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_jas_total_batch(const JasJob* __restrict__ jobs,
                                                             const int* __restrict__ starts, int njobs) {
   const int m = job_of(starts, njobs, blockIdx.x);
   const JasJob& J = jobs[m];
-  jas_total_body(J.v, J.o, J.Jt, J.JtP, J.ntiles, J.have_een, J.fused, (int64_t)blockIdx.x - starts[m]);
+  jas_total_body(J.v, J.o, J.Jt, J.JtP, J.ntiles, J.have_een, J.fused, 0, (int64_t)blockIdx.x - starts[m]);
 }
 
 struct EenShape {
@@ -2719,7 +2732,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       hipError_t ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_mfma<CC>),                  \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);             \
       if (ea == hipSuccess)                                                                                    \
-        ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_res<CC>),                            \
+        ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_res<CC, false>),                     \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
       if (ea == hipSuccess && (CC == 4 || CC == 5))                                                            \
         ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_bal<CC == 4 ? 4 : 5>),                \
@@ -2728,7 +2741,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       attr_set = true;                                                                                         \
     }                                                                                                          \
     if (sh.res)                                                                                                \
-      hipLaunchKernelGGL(k_jas_een_res<CC>, dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES),     \
+      hipLaunchKernelGGL((k_jas_een_res<CC, false>), dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES), \
                          sh.lds, s, v, o, sv.Jt, sv.JtP, sh.ntiles);                                           \
     else if (sh.bal)                                                                                           \
       hipLaunchKernelGGL(k_jas_een_bal<CC == 4 ? 4 : 5>, grid, dim3(64 * EB_WAVES), sh.lds, s, v, o, sv.R0,      \
@@ -2776,7 +2789,44 @@ hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_
   const int have_een = (v.cord >= 2 && v.dim_c > 0) ? 1 : 0;
   const EenShape sh = een_shape(v);
   hipLaunchKernelGGL(k_jas_total, dim3((unsigned)v.walk_num), dim3(kBlock), 0, s, v, o, sv.Jt, sv.JtP, sh.ntiles,
-                     have_een, een_fused(v) ? 1 : 0);
+                     have_een, een_fused(v) ? 1 : 0, 0);
+  return hipGetLastError();
+}
+
+bool jastrow_values_only_ok(const JasDevView& v) { return een_fused(v); }
+
+hipError_t launch_jastrow_values(const JasDevView& v, const JasOut& o, hipStream_t s) {
+  if (!een_fused(v)) return hipErrorInvalidValue;
+  const ScratchView sv = scratch_view(v, o);
+  const EenShape sh = een_shape(v);
+  if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
+  switch (v.cord) {
+#define EEN_VAL_CASE(CC)                                                                                       \
+  case CC: {                                                                                                   \
+    static bool attr_set = false;                                                                              \
+    if (!attr_set) {                                                                                           \
+      const hipError_t ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_res<CC, true>),       \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);       \
+      if (ea != hipSuccess) return ea;                                                                         \
+      attr_set = true;                                                                                         \
+    }                                                                                                          \
+    hipLaunchKernelGGL((k_jas_een_res<CC, true>), dim3(een_res_grid(v.walk_num * sh.ntiles)),                  \
+                       dim3(64 * EQ_WAVES), sh.lds, s, v, o, sv.Jt, sv.JtP, sh.ntiles);                        \
+    break;                                                                                                     \
+  }
+    EEN_VAL_CASE(2)
+    EEN_VAL_CASE(3)
+    EEN_VAL_CASE(4)
+    EEN_VAL_CASE(5)
+    EEN_VAL_CASE(6)
+#undef EEN_VAL_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_jas_total, dim3((unsigned)v.walk_num), dim3(kBlock), 0, s, v, o, sv.Jt, sv.JtP, sh.ntiles, 1,
+                     1, 1);
   return hipGetLastError();
 }
 

@@ -285,6 +285,78 @@ def test_jastrow_orders_and_shapes_vs_oracle(nn, up, dn, aord, bord, cord):
     q.context_destroy(ctx)
 
 
+VALUE_GETTERS = ["qmckl_get_jastrow_champ_factor_ee", "qmckl_get_jastrow_champ_factor_en",
+                 "qmckl_get_jastrow_champ_factor_een", "qmckl_get_jastrow_champ_value"]
+
+
+@pytest.mark.parametrize("nn,up,dn,aord,bord,cord", [
+    (20, 19, 18, 5, 5, 2), (20, 19, 18, 3, 4, 3), (17, 24, 16, 5, 5, 4), (33, 40, 39, 2, 6, 6),
+    (25, 30, 29, 5, 5, 5), (3, 3, 2, 5, 5, 5), (18, 21, 20, 5, 5, 3),
+    (40, 120, 120, 5, 5, 5), (20, 19, 18, 5, 5, 7)])
+def test_jastrow_values_only(nn, up, dn, aord, bord, cord):
+    """The value and factor getters with no current GL run the values-only evaluation where
+    the LDS-resident e-e-n kernel applies (the value component of the products alone, the
+    fold of J alone): the three factors and e^J equal those of the full evaluation bit for
+    bit, through the host and the device getters, and the oracle's to 1e-12.  240 electrons
+    and cord 7 (no resident kernel) take them from the full evaluation."""
+    s, x, W = S.jastrow_workload(nn, up, dn, aord, bord, cord, walk_num=8 if up + dn < 200 else 2)
+    ctx = make_ctx(s, x, W, mo=False)
+    dev = q.DeviceBuffer(8 * W)
+    try:
+        vals = {n: q.get_array(ctx, n, W) for n in VALUE_GETTERS}  # values first: values-only path
+        S.set_coords(ctx, s, x, W)
+        q.check(ctx, q.lib().qmckl_amd_get_jastrow_champ_value_device(ctx, dev.ptr, W))
+        q.check(ctx, q.lib().qmckl_amd_synchronize(ctx))
+        dval = dev.to_array((W,))
+        S.set_coords(ctx, s, x, W)
+        q.get_array(ctx, "qmckl_get_jastrow_champ_gl", W * 4 * s.elec_num)  # the full evaluation
+        full = {n: q.get_array(ctx, n, W) for n in VALUE_GETTERS}
+        for n in VALUE_GETTERS:
+            assert np.array_equal(vals[n], full[n]), f"{s.name} {n}"
+        assert np.array_equal(dval, full["qmckl_get_jastrow_champ_value"])
+        ref = oracle.jastrow(s, x, W, want=("factor_ee", "factor_en", "factor_een"))
+        for k in ("factor_ee", "factor_en", "factor_een"):
+            err = np.abs(vals[f"qmckl_get_jastrow_champ_{k}"] - ref[k]) / np.maximum(1.0, np.abs(ref[k]))
+            assert err.max() <= 1e-12, f"{s.name} {k}: {err.max():.3e}"
+    finally:
+        dev.free()
+        q.context_destroy(ctx)
+
+
+def test_jastrow_value_then_gl_evaluates_once():
+    """A caller that reads the value and then the GL at every step gets the values-only
+    evaluation on its first step only: once it has asked for the GL after a value request,
+    the next value request runs the full evaluation (the GL request then finds it done), and
+    a step without a GL request returns to the values-only evaluation one step later.  The
+    Jastrow totals' profiling slot (4) counts the full evaluations (the values-only one
+    launches its total under the e-e-n slot)."""
+    s, x, W = S.workload("medium", walk_num=8)
+    ctx = make_ctx(s, x, W, mo=False)
+    L = q.lib()
+    ms = np.zeros(5)
+    cnt = np.zeros(5, np.int64)
+
+    def full_evals(steps, gl):
+        q.check(ctx, L.qmckl_amd_get_kernel_times(ctx, q.ptr(ms), q.ptr(cnt), 5))
+        for i in range(steps):
+            S.set_coords(ctx, s, x + 0.01 * i, W)
+            q.get_array(ctx, "qmckl_get_jastrow_champ_value", W)
+            if gl:
+                q.get_array(ctx, "qmckl_get_jastrow_champ_gl", W * 4 * s.elec_num)
+        q.check(ctx, L.qmckl_amd_get_kernel_times(ctx, q.ptr(ms), q.ptr(cnt), 5))
+        return int(cnt[4]), int(cnt[3])
+
+    try:
+        q.check(ctx, L.qmckl_amd_set_profiling(ctx, 1))
+        assert full_evals(1, gl=True) == (1, 2)  # values-only, then the full evaluation for the GL
+        assert full_evals(3, gl=True) == (3, 3)  # the full evaluation on each value request
+        assert full_evals(1, gl=False) == (1, 1)  # still full: the GL was asked after the last value
+        assert full_evals(2, gl=False) == (0, 2)  # values-only
+        q.check(ctx, L.qmckl_amd_set_profiling(ctx, 0))
+    finally:
+        q.context_destroy(ctx)
+
+
 def test_coordinate_update_recomputes():
     """A new set of coordinates invalidates every cached result (date semantics)."""
     s, x, W = S.workload("small", walk_num=16)
