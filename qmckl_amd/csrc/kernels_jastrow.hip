@@ -1940,32 +1940,37 @@ __global__ void __launch_bounds__(kBlock) k_jas_total(JasDevView v, JasOut o, co
 // the resident kernel): the products of k_jas_een_mfma, taken over two 16-nucleus tiles at
 // a time.  The (k, q) units of a tile are grouped as in the resident kernel -- pairs (q, q + 1)
 // of the same k, which share the A operand (ten matrix instructions per A), and single
-// units -- and the groups of the two tiles are dealt to 8 waves so that the two waves of
-// every SIMD (waves w and w + 4) carry the same matrix load: at C = 5 each wave one pair and
-// waves 0-3 one single (10 + 5 = 15 and 10 products per step, 25 per SIMD); at C = 4 waves
-// 0-3 one pair, waves 4-7 one single (15 per SIMD).  One wave per (k, q), as in
-// k_jas_een_mfma, puts 10 waves on 4 SIMDs, 3, 3, 2 and 2 of them.  The accumulators of a
-// wave's groups live across the passes over electrons j; the fold is that of k_jas_een_mfma.
+// units -- and every group of the two tiles gets a wave of its own, dealt so that the waves
+// of every SIMD (waves w, w + 4, w + 8) carry the same matrix load: at C = 5 12 waves, the 8
+// pairs on waves 0-7 and the 4 singles on waves 8-11 (10 + 10 + 5 = 25 products per step on
+// every SIMD, three waves each to hide one another's operand reads); at C = 4 8 waves, the 4
+// pairs on waves 0-3 and the 4 singles on waves 4-7 (15 per SIMD).  One wave per (k, q), as
+// in k_jas_een_mfma, puts 10 waves on 4 SIMDs, 3, 3, 2 and 2 of them.  The accumulators of a
+// wave's group live across the passes over electrons j; the fold is that of k_jas_een_mfma.
 // ------------------------------------------------------------------------------------
-constexpr int EB_WAVES = 8;
+constexpr int bal_waves(int C) { return C == 5 ? 12 : 8; }
 
-// the groups of wave w: pair (tile, index) and single (tile, index); tile -1: none
+// the group of wave w of C: a pair (tile, index) on the first 2 npair waves, else a single
+// (tile, index); tile -1: none
 template <int C>
 constexpr int bal_pair_tile(int w) {
-  return C == 5 ? w / 4 : (w < 4 ? w / 2 : -1);
+  return w < 2 * een_npair(C) ? w / een_npair(C) : -1;
 }
 template <int C>
 constexpr int bal_pair_idx(int w) {
-  return C == 5 ? w % 4 : w % 2;
+  return w % een_npair(C);
 }
 template <int C>
 constexpr int bal_single_tile(int w) {
-  return C == 5 ? (w < 4 ? w / 2 : -1) : (w >= 4 ? (w - 4) / 2 : -1);
+  return w >= 2 * een_npair(C) ? (w - 2 * een_npair(C)) / een_nsingle(C) : -1;
 }
 template <int C>
 constexpr int bal_single_idx(int w) {
-  return C == 5 ? w % 2 : (w - 4) % 2;
+  return (w - 2 * een_npair(C)) % een_nsingle(C);
 }
+static_assert(2 * een_npair(5) + 2 * een_nsingle(5) == bal_waves(5) &&
+                  2 * een_npair(4) + 2 * een_nsingle(4) == bal_waves(4),
+              "one group per wave");
 
 // Dynamic LDS of k_jas_een_bal, in doubles, for passes of JS electrons j.
 struct EenBalLds {
@@ -1981,7 +1986,7 @@ struct EenBalLds {
     r0t = o;  o += EM * (C - 1) * 5;        // q = 0 sums of the tile electrons
     st = o;   o += 2 * 16 * (C - 1);        // per tile: sum over all j of g_a(r_j)^q
     cfl = o;  o += 2 * dim * 16;            // per tile: c-coefficients, [n][a]
-    red = o;  o += EB_WAVES * EM * 5;
+    red = o;  o += bal_waves(C) * EM * 5;
     tot = o;  o += EM * 5;                  // running (J, gl) per row, summed over nuclei
     jrow = o; o += EM;
     total = o;
@@ -1996,40 +2001,26 @@ __device__ __forceinline__ double een_pow4(double y, int n) {
   return n == 1 ? y : n == 2 ? y2 : n == 3 ? y3 : y4;
 }
 
-// One pass of the groups of a wave over nsteps x 4 electrons j: a pair (K; Q, Q + 1) on
-// nucleus tile gp_tile (accumulators a, b) when has_pair, and a single unit (SK, SQ) on
-// tile gs_tile (accumulator s) when has_single.  (k, q) are wave-uniform run-time values:
-// one code path for all waves keeps the register count of a wave's own work (an 8-way
-// compile-time dispatch cost about 70 registers more).
-__device__ __forceinline__ void een_bal_pass(d4 (&a)[5], d4 (&b)[5], d4 (&s)[5], int has_pair, int K, int Q,
-                                             const double* gp_tile, int has_single, int SK, int SQ,
-                                             const double* gs_tile, const double* pdG, const double* pdR,
-                                             const double* xj, const double* xi, int JSP, int nsteps, int li,
-                                             int lg) {
+// One pass of a wave's group over nsteps x 4 electrons j: units (K, Q) into a and, for a pair
+// (two), (K, Q + 1) into b, on the nucleus tile g_tile.  (K, Q) are wave-uniform run-time
+// values: one code path for all waves keeps the registers of one group.
+__device__ __forceinline__ void een_bal_pass(d4 (&a)[5], d4 (&b)[5], int two, int K, int Q, const double* g_tile,
+                                             const double* pdG, const double* pdR, const double* xj,
+                                             const double* xi, int JSP, int nsteps, int li, int lg) {
 #pragma unroll 1
   for (int st = 0; st < nsteps; ++st) {
     const int jj = 4 * st + lg;
-    const double Ge = pdG[jj * EM + li], ri = pdR[jj * EM + li];
+    const double Ge = pdG[jj * EM + li], ri = pdR[jj * EM + li], g = g_tile[jj * 16 + li];
     const double dx = xi[0] - xj[jj], dy = xi[1] - xj[JSP + jj], dz = xi[2] - xj[2 * JSP + jj];
-    if (has_pair) {
-      const double g = gp_tile[jj * 16 + li];
-      const double Gk = een_pow4(Ge, K), gq = een_pow4(g, Q);
+    const double Gk = een_pow4(Ge, K), gq = een_pow4(g, Q);
+    const double t = Gk * ri;
+    const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
+#pragma unroll
+    for (int c = 0; c < 5; ++c) a[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, a[c], 0, 0, 0);
+    if (two) {
       const double gq1 = gq * g;
-      const double t = Gk * ri;
-      const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        a[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, a[c], 0, 0, 0);
-        b[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq1, b[c], 0, 0, 0);
-      }
-    }
-    if (has_single) {
-      const double g = gs_tile[jj * 16 + li];
-      const double Gk = een_pow4(Ge, SK), gq = een_pow4(g, SQ);
-      const double t = Gk * ri;
-      const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
-#pragma unroll
-      for (int c = 0; c < 5; ++c) s[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, s[c], 0, 0, 0);
+      for (int c = 0; c < 5; ++c) b[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq1, b[c], 0, 0, 0);
     }
   }
 }
@@ -2076,12 +2067,11 @@ struct EenBalWave {
 
 // The folds of wave W's groups at row register r (row il = lg + 4 r) into out[5].
 template <int C, int W = 0>
-__device__ __forceinline__ void een_bal_dispatch_fold(int wave, const d4 (&pa)[5], const d4 (&pb)[5],
-                                                      const d4 (&sa)[5], int r, int il, int li, int a0, int Nn,
-                                                      int ntile, const double* ep, const double* cfl,
-                                                      const double* r0t, const double* st, double kee,
-                                                      double out[5]) {
-  if constexpr (W < EB_WAVES) {
+__device__ __forceinline__ void een_bal_dispatch_fold(int wave, const d4 (&pa)[5], const d4 (&pb)[5], int r, int il,
+                                                      int li, int a0, int Nn, int ntile, const double* ep,
+                                                      const double* cfl, const double* r0t, const double* st,
+                                                      double kee, double out[5]) {
+  if constexpr (W < bal_waves(C)) {
     if (wave == W) {
       typedef EenBalWave<C, W> B;
       constexpr int K1 = C - 1, DIM = Lkm<C>::dim, EPS = EM * 16;
@@ -2103,11 +2093,11 @@ __device__ __forceinline__ void een_bal_dispatch_fold(int wave, const d4 (&pa)[5
         });
       if constexpr (B::ST >= 0)
         group(B::ST, [&](EenRow& R, const double* gp, const double* cf, const double* srow) {
-          een_bal_fold<C, B::SK, B::SQ>(R, sa, r, cf, gp, r0t + il * K1 * 5, srow, kee);
+          een_bal_fold<C, B::SK, B::SQ>(R, pa, r, cf, gp, r0t + il * K1 * 5, srow, kee);
         });
       return;
     }
-    een_bal_dispatch_fold<C, W + 1>(wave, pa, pb, sa, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
+    een_bal_dispatch_fold<C, W + 1>(wave, pa, pb, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
   }
 }
 
@@ -2117,7 +2107,8 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
                                                  double* __restrict__ Jt, int ntiles, int JS, const int64_t b) {
   static_assert(C == 4 || C == 5, "balanced pass-based kernel: cord 4 and 5");
   constexpr int K1 = C - 1;
-  constexpr int NT = 64 * EB_WAVES;
+  constexpr int NWV = bal_waves(C);
+  constexpr int NT = 64 * NWV;
   constexpr int DIM = Lkm<C>::dim;
   constexpr int EPS = EM * 16;
   extern __shared__ double lds[];
@@ -2156,19 +2147,19 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
   for (int t = tid; t < EM * 5; t += NT) tot[t] = 0.0;
   __syncthreads();
   const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
-  // this wave's groups: pair tile and (k, q), single tile and (k, q); tile -1: none
-  int g_pt = -1, g_pk = 1, g_pq = 1, g_st = -1, g_sk = 1, g_sq = 1;
-  for (int wv = 0; wv < EB_WAVES; ++wv)
+  // this wave's group: nucleus tile, (k, q), pair or single
+  int g_t = 0, g_k = 1, g_q = 1, g_two = 0;
+  for (int wv = 0; wv < NWV; ++wv)
     if (wv == wave) {
-      g_pt = bal_pair_tile<C>(wv);
-      g_st = bal_single_tile<C>(wv);
-      if (g_pt >= 0) {
-        g_pk = een_pair_k(C, bal_pair_idx<C>(wv));
-        g_pq = een_pair_q(C, bal_pair_idx<C>(wv));
-      }
-      if (g_st >= 0) {
-        g_sk = een_single_k(C, bal_single_idx<C>(wv));
-        g_sq = een_single_q(C, bal_single_idx<C>(wv));
+      g_two = bal_pair_tile<C>(wv) >= 0;
+      if (g_two) {
+        g_t = bal_pair_tile<C>(wv);
+        g_k = een_pair_k(C, bal_pair_idx<C>(wv));
+        g_q = een_pair_q(C, bal_pair_idx<C>(wv));
+      } else {
+        g_t = bal_single_tile<C>(wv);
+        g_k = een_single_k(C, bal_single_idx<C>(wv));
+        g_q = een_single_q(C, bal_single_idx<C>(wv));
       }
     }
 
@@ -2215,9 +2206,9 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
       e[(C + 4) * EPS] = ka2;
     }
 
-    d4 pa[5], pb[5], sa[5];
+    d4 pa[5], pb[5];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) pa[c] = pb[c] = sa[c] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < 5; ++c) pa[c] = pb[c] = d4{0.0, 0.0, 0.0, 0.0};
 
     for (int pass = 0; pass < npass; ++pass) {
       const int j0 = pass * JS;
@@ -2252,9 +2243,8 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
             (a < Nn && jj < jn) ? gN[gn_index(w, a, j0 + jj, gn_na(Nn), gn_jp(Ne))] : 0.0;
       }
       __syncthreads();
-      een_bal_pass(pa, pb, sa, g_pt >= 0 && g_pt < ntile, g_pk, g_pq, gch + max(g_pt, 0) * L.JSP * 16,
-                   g_st >= 0 && g_st < ntile, g_sk, g_sq, gch + max(g_st, 0) * L.JSP * 16, pdG, pdR, xj, xi,
-                   L.JSP, jn4 >> 2, li, lg);
+      if (g_t < ntile)
+        een_bal_pass(pa, pb, g_two, g_k, g_q, gch + g_t * L.JSP * 16, pdG, pdR, xj, xi, L.JSP, jn4 >> 2, li, lg);
       if (pass + 1 < npass) __syncthreads();  // before gch of the next pass is written
     }
 
@@ -2263,7 +2253,7 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
     for (int r = 0; r < 4; ++r) {
       const int il = lg + 4 * r, i = i0 + il;
       double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (i < Ne) een_bal_dispatch_fold<C>(wave, pa, pb, sa, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
+      if (i < Ne) een_bal_dispatch_fold<C>(wave, pa, pb, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
       // sum over the 16 nuclei of the tile(s): lanes li = 0..15 of this row group
 #pragma unroll
       for (int c = 0; c < 5; ++c) out[c] = row_sum16(out[c]);
@@ -2275,7 +2265,7 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
     for (int t = tid; t < EM * 5; t += NT) {
       double x = tot[t];
 #pragma unroll
-      for (int wv = 0; wv < EB_WAVES; ++wv) x += red[wv * EM * 5 + t];
+      for (int wv = 0; wv < NWV; ++wv) x += red[wv * EM * 5 + t];
       tot[t] = x;
     }
     // the next tile pair rewrites cfl, st, ep and gch only after this barrier
@@ -2297,7 +2287,7 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
 }
 
 template <int C>
-__global__ void __launch_bounds__(64 * EB_WAVES) k_jas_een_bal(JasDevView v, JasOut o, const double* __restrict__ R0,
+__global__ void __launch_bounds__(64 * bal_waves(C)) k_jas_een_bal(JasDevView v, JasOut o, const double* __restrict__ R0,
                                                               const double* __restrict__ gN,
                                                               const double* __restrict__ Sg, double* __restrict__ Jt,
                                                               int ntiles, int JS) {
@@ -2349,7 +2339,7 @@ __global__ void __launch_bounds__(64 * EenWaves<C>::n) k_jas_een_mfma_batch(cons
 }
 
 template <int C>
-__global__ void __launch_bounds__(64 * EB_WAVES) k_jas_een_bal_batch(const JasJob* __restrict__ jobs,
+__global__ void __launch_bounds__(64 * bal_waves(C)) k_jas_een_bal_batch(const JasJob* __restrict__ jobs,
                                                                     const int* __restrict__ starts, int njobs) {
   const int m = job_of(starts, njobs, blockIdx.x);
   const JasJob& J = jobs[m];
@@ -2744,7 +2734,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       hipLaunchKernelGGL((k_jas_een_res<CC, false>), dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES), \
                          sh.lds, s, v, o, sv.Jt, sv.JtP, sh.ntiles);                                           \
     else if (sh.bal)                                                                                           \
-      hipLaunchKernelGGL(k_jas_een_bal<CC == 4 ? 4 : 5>, grid, dim3(64 * EB_WAVES), sh.lds, s, v, o, sv.R0,      \
+      hipLaunchKernelGGL(k_jas_een_bal<CC == 4 ? 4 : 5>, grid, dim3(64 * bal_waves(CC)), sh.lds, s, v, o, sv.R0, \
                          sv.gN, sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                \
     else                                                                                                       \
       hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,    \
@@ -2960,7 +2950,7 @@ hipError_t jastrow_batch_build(const std::vector<std::pair<JasDevView, JasOut>>&
         add(JB_EEN_RES, 0, 64 * EQ_WAVES, sh.lds,
             (double)W * sh.ntiles * gn_jp(Ne) * 16.0 * gn_na(v.nucl_num));
       else if (sh.bal)
-        add(JB_EEN_BAL, W * sh.ntiles, 64 * EB_WAVES, sh.lds, 0.0);
+        add(JB_EEN_BAL, W * sh.ntiles, 64 * bal_waves(C), sh.lds, 0.0);
       else if (sh.mfma)
         add(JB_EEN_MFMA, W * sh.ntiles, 64 * (C * (C - 1) / 2), sh.lds, 0.0);
       else
