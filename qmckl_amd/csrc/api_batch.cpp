@@ -174,6 +174,13 @@ qmckl_exit_code step(qmckl_context ctx, const double* d_coord, double* d_mo, dou
   qmckl_exit_code rc = qmckl_amd_set_electron_coord_device(ctx, 'T', W, d_coord, 3 * P);
   if (rc == QMCKL_SUCCESS && d_mo != nullptr)
     rc = qmckl_amd_get_mo_basis_mo_vgl_device(ctx, d_mo, P * 5 * c->mo_num);
+  // the full Jastrow evaluation, as the recorded graph runs it (with a value alone asked for,
+  // the value getter could run the values-only one, which leaves the GL buffers the graph
+  // writes unallocated)
+  if (rc == QMCKL_SUCCESS && (d_value != nullptr || d_gl != nullptr)) {
+    std::lock_guard<std::recursive_mutex> g(c->mutex);
+    rc = provide_jastrow(ctx, c, "qmckl_amd_batch_record");
+  }
   if (rc == QMCKL_SUCCESS && d_value != nullptr) rc = qmckl_amd_get_jastrow_champ_value_device(ctx, d_value, W);
   if (rc == QMCKL_SUCCESS && d_gl != nullptr) rc = qmckl_amd_get_jastrow_champ_gl_device(ctx, d_gl, W * 4 * Ne);
   return rc;

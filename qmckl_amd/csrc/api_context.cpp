@@ -282,7 +282,7 @@ qmckl_context qmckl_context_copy(const qmckl_context context) {
   CP(shell_prim_num); CP(shell_prim_index); CP(shell_ang_mom); CP(r_power); CP(shell_factor);
   CP(exponent); CP(coefficient); CP(prim_factor); CP(ao_factor); CP(ao_cartesian); CP(ao_uninit);
   CP(ao_provided); CP(tables);
-  CP(mo_num); CP(mo_coef); CP(mo_uninit); CP(mo_provided); CP(mo_precision);
+  CP(mo_num); CP(mo_coef); CP(mo_uninit); CP(mo_provided); CP(mo_precision); CP(mo_tile_g); CP(mo_tile_nw);
   CP(aord); CP(bord); CP(cord); CP(dim_c); CP(type_nucl_num); CP(spin_independent);
   CP(type_nucl_vector); CP(a_vector); CP(b_vector); CP(c_vector); CP(rescale_en); CP(rescale_ee);
   CP(jas_uninit); CP(jas_provided); CP(asymp_jasa);

@@ -49,6 +49,36 @@ def _buffers(mols):
     return coord, mo, val, gl
 
 
+def test_batch_of_values_alone():
+    """A batch recorded for the Jastrow values alone (no GL buffers, no MO) on fresh contexts,
+    whose value getters would run the values-only evaluation: the recording evaluates the
+    whole Jastrow as its graph does, and the replayed values equal the eager values bit for
+    bit on two replays."""
+    mols = [S.molecule_workload(ne, walk_num=8, seed=200 + ne) for ne in (12, 45)]
+    ctxs = _contexts(mols)
+    ref_ctxs = _contexts(mols)
+    coord, _, val, _ = _buffers(mols)
+    b = Batch(ctxs, coord, None, val, None)
+    try:
+        for rep in range(2):
+            xs = [np.ascontiguousarray(x + 0.03 * rep) for _, x, _ in mols]
+            for buf, x in zip(coord, xs):
+                buf.upload(x)
+            q.device_synchronize()
+            b.replay()
+            b.synchronize()
+            for (s, _, W), rc, x, v in zip(mols, ref_ctxs, xs, val):
+                S.set_coords(rc, s, x, W)
+                ref_v = q.get_array(rc, "qmckl_get_jastrow_champ_value", W)
+                assert np.array_equal(v.to_array((W,)), ref_v), f"replay {rep} {s.name}"
+    finally:
+        b.destroy()
+        for ctx in ctxs + ref_ctxs:
+            q.context_destroy(ctx)
+        for buf in coord + val:
+            buf.free()
+
+
 def test_batch_replays_match_eager():
     mols = [S.molecule_workload(ne, walk_num=8, seed=100 + ne) for ne in (10, 37, 61)]
     ctxs = _contexts(mols)
