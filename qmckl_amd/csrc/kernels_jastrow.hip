@@ -2001,14 +2001,14 @@ __device__ __forceinline__ double een_pow4(double y, int n) {
   return n == 1 ? y : n == 2 ? y2 : n == 3 ? y3 : y4;
 }
 
-// One pass of a wave's group over nsteps x 4 electrons j: units (K, Q) into a and, for a pair
+// One pass of a wave's group over steps [st0, st1) of 4 electrons j: units (K, Q) into a and, for a pair
 // (two), (K, Q + 1) into b, on the nucleus tile g_tile.  (K, Q) are wave-uniform run-time
 // values: one code path for all waves keeps the registers of one group.
 __device__ __forceinline__ void een_bal_pass(d4 (&a)[5], d4 (&b)[5], int two, int K, int Q, const double* g_tile,
                                              const double* pdG, const double* pdR, const double* xj,
-                                             const double* xi, int JSP, int nsteps, int li, int lg) {
+                                             const double* xi, int JSP, int st0, int st1, int li, int lg) {
 #pragma unroll 1
-  for (int st = 0; st < nsteps; ++st) {
+  for (int st = st0; st < st1; ++st) {
     const int jj = 4 * st + lg;
     const double Ge = pdG[jj * EM + li], ri = pdR[jj * EM + li], g = g_tile[jj * 16 + li];
     const double dx = xi[0] - xj[jj], dy = xi[1] - xj[JSP + jj], dz = xi[2] - xj[2 * JSP + jj];
@@ -2148,20 +2148,28 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
   __syncthreads();
   const double xi[3] = {xt[li * 3], xt[li * 3 + 1], xt[li * 3 + 2]};  // electron i of row li
   // this wave's group: nucleus tile, (k, q), pair or single
-  int g_t = 0, g_k = 1, g_q = 1, g_two = 0;
+  int g_t = 0, g_k = 1, g_q = 1, g_two = 0, g_idx = 0;
   for (int wv = 0; wv < NWV; ++wv)
     if (wv == wave) {
       g_two = bal_pair_tile<C>(wv) >= 0;
       if (g_two) {
         g_t = bal_pair_tile<C>(wv);
-        g_k = een_pair_k(C, bal_pair_idx<C>(wv));
-        g_q = een_pair_q(C, bal_pair_idx<C>(wv));
+        g_idx = bal_pair_idx<C>(wv);
+        g_k = een_pair_k(C, g_idx);
+        g_q = een_pair_q(C, g_idx);
       } else {
         g_t = bal_single_tile<C>(wv);
-        g_k = een_single_k(C, bal_single_idx<C>(wv));
-        g_q = een_single_q(C, bal_single_idx<C>(wv));
+        g_idx = bal_single_idx<C>(wv);
+        g_k = een_single_k(C, g_idx);
+        g_q = een_single_q(C, g_idx);
       }
     }
+  // With an odd number of nucleus tiles the last iteration holds one tile: the waves of the
+  // second tile's groups then take the second half of the electron steps of the same group
+  // on the first tile (its partner wave the first half), and hand their partial sums to the
+  // partner through LDS (the pair records' space, free by then), when that space holds a
+  // pair group's sums for every pair (40 doubles per lane each).
+  const bool split_ok = L.JSP >= 40 * een_npair(C);
 
   for (int a0 = 0; a0 < Nn; a0 += 32) {
     const int ntile = a0 + 16 < Nn ? 2 : 1;
@@ -2243,9 +2251,43 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
             (a < Nn && jj < jn) ? gN[gn_index(w, a, j0 + jj, gn_na(Nn), gn_jp(Ne))] : 0.0;
       }
       __syncthreads();
-      if (g_t < ntile)
-        een_bal_pass(pa, pb, g_two, g_k, g_q, gch + g_t * L.JSP * 16, pdG, pdR, xj, xi, L.JSP, jn4 >> 2, li, lg);
+      const bool split = ntile == 1 && split_ok, helper = split && g_t == 1;
+      if (g_t < ntile || helper) {
+        const int ns = jn4 >> 2, h = split ? ns / 2 : ns;
+        een_bal_pass(pa, pb, g_two, g_k, g_q, gch + (helper ? 0 : g_t) * L.JSP * 16, pdG, pdR, xj, xi, L.JSP,
+                     helper ? h : 0, helper ? ns : h, li, lg);
+      }
       if (pass + 1 < npass) __syncthreads();  // before gch of the next pass is written
+    }
+    if (ntile == 1 && split_ok) {
+      // partial sums of the helpers (g_t == 1) into their partners (g_t == 0, same group):
+      // the pairs' sums first, then the singles', slot g_idx of 40 (20) doubles per lane
+      double* sbuf = lds + L.pdG;
+      for (int round = 0; round < 2; ++round) {
+        const bool mine = (round == 0) == (g_two != 0);
+        const int nc = g_two ? 10 : 5;
+        double* slot = sbuf + (size_t)g_idx * nc * 4 * 64 + lane;
+        __syncthreads();  // every product done; the previous round's slots read
+        if (mine && g_t == 1) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              slot[(c * 4 + r) * 64] = pa[c][r];
+              if (g_two) slot[((5 + c) * 4 + r) * 64] = pb[c][r];
+            }
+        }
+        __syncthreads();
+        if (mine && g_t == 0) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              pa[c][r] += slot[(c * 4 + r) * 64];
+              if (g_two) pb[c][r] += slot[((5 + c) * 4 + r) * 64];
+            }
+        }
+      }
     }
 
     // Epilogue: lane (li, lg) holds rows lg + 4 r (register r), column a = a0 + 16 tile + li.
