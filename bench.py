@@ -281,8 +281,8 @@ def main():
                                    "without derivatives, a fifth of the product's matrix work"},
         "jastrow_values_only": {"een_ms_per_launch": kjv["ms_per_step"] / max(1, kjv["launches_per_step"]),
                                 "note": "qmckl_get_jastrow_champ_value alone: the value component of the "
-                                        "e-e-n products only, no GL (LDS-resident kernel sizes; else the "
-                                        "full evaluation)"},
+                                        "e-e-n products only, no GL (LDS-resident and balanced pass-based "
+                                        "kernels; else the full evaluation)"},
         "mo_err_vs_fp64_max_normalised": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_err)),
         "mo_rel_err_vs_fp64_above_1e-3_of_max": dict(zip(["value", "dx", "dy", "dz", "lap"], mo_rel)),
         "jastrow_max_abs_dJ_vs_fp64": dJ,

@@ -554,18 +554,25 @@ qmckl_exit_code provide_jastrow_value(qmckl_context handle, Context* c, const ch
   if (rc != QMCKL_SUCCESS) return rc;
   const JasDevView v = jas_view(c);
   if (gl_too || !jastrow_values_only_ok(v)) return provide_jastrow_full(handle, c, fn);
-  const int64_t W = c->walk_num;
+  const int64_t W = c->walk_num, Ne = c->elec_num;
   hipError_t e = hipSuccess;
   DevBuf* w1[] = {&c->d_fee, &c->d_fen, &c->d_feen, &c->d_jvalue};
   for (DevBuf* b : w1)
     if (e == hipSuccess) e = b->reserve(sizeof(double) * W);
+  // the pair kernel, which runs where the resident e-e-n kernel does not, writes the e-e and
+  // e-n GL with the values
+  const bool pair_gl = !een_fused(v);
+  if (pair_gl && e == hipSuccess) e = c->d_fee_gl.reserve(sizeof(double) * W * 4 * Ne);
+  if (pair_gl && e == hipSuccess) e = c->d_fen_gl.reserve(sizeof(double) * W * 4 * Ne);
   if (e == hipSuccess) e = c->d_jscratch.reserve(sizeof(double) * std::max<int64_t>(jastrow_scratch_doubles(v), 1));
   if (e != hipSuccess) return hip_fail(handle, e, fn);
   JasOut o;
   o.fee = c->d_fee.as<double>();
   o.fen = c->d_fen.as<double>();
   o.feen = c->d_feen.as<double>();
-  o.fee_gl = o.fen_gl = o.feen_gl = o.gl = nullptr;  // not touched by the values-only kernels
+  o.fee_gl = pair_gl ? c->d_fee_gl.as<double>() : nullptr;
+  o.fen_gl = pair_gl ? c->d_fen_gl.as<double>() : nullptr;
+  o.feen_gl = o.gl = nullptr;  // not touched by the values-only kernels
   o.value = c->d_jvalue.as<double>();
   o.scratch = c->d_jscratch.as<double>();
   {

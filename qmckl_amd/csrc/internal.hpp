@@ -427,6 +427,8 @@ hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_
 // The factors and e^J alone (no GL) in one resident e-e-n launch and the total; only where
 // that kernel applies (jastrow_values_only_ok), else the full evaluation gives them.
 bool jastrow_values_only_ok(const JasDevView& v);
+// Whether the LDS-resident e-e-n kernel applies (it then also does the pair kernel's work).
+bool een_fused(const JasDevView& v);
 hipError_t launch_jastrow_values(const JasDevView& v, const JasOut& o, hipStream_t s);
 // Grouped Jastrow launches over a batch of contexts (one launch per kernel and cord): built
 // once (device job tables allocated), launched per step, freed with the batch.

@@ -2001,9 +2001,11 @@ __device__ __forceinline__ double een_pow4(double y, int n) {
   return n == 1 ? y : n == 2 ? y2 : n == 3 ? y3 : y4;
 }
 
-// One pass of a wave's group over steps [st0, st1) of 4 electrons j: units (K, Q) into a and, for a pair
+// One pass of a wave's group over steps [st0, st1) of 4 electrons j (the first NC components:
+// 5, or 1 for the values alone): units (K, Q) into a and, for a pair
 // (two), (K, Q + 1) into b, on the nucleus tile g_tile.  (K, Q) are wave-uniform run-time
 // values: one code path for all waves keeps the registers of one group.
+template <int NC>
 __device__ __forceinline__ void een_bal_pass(d4 (&a)[5], d4 (&b)[5], int two, int K, int Q, const double* g_tile,
                                              const double* pdG, const double* pdR, const double* xj,
                                              const double* xi, int JSP, int st0, int st1, int li, int lg) {
@@ -2016,11 +2018,11 @@ __device__ __forceinline__ void een_bal_pass(d4 (&a)[5], d4 (&b)[5], int two, in
     const double t = Gk * ri;
     const double A[5] = {Gk, t * dx, t * dy, t * dz, t};
 #pragma unroll
-    for (int c = 0; c < 5; ++c) a[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, a[c], 0, 0, 0);
+    for (int c = 0; c < NC; ++c) a[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq, a[c], 0, 0, 0);
     if (two) {
       const double gq1 = gq * g;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) b[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq1, b[c], 0, 0, 0);
+      for (int c = 0; c < NC; ++c) b[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[c], gq1, b[c], 0, 0, 0);
     }
   }
 }
@@ -2101,12 +2103,16 @@ __device__ __forceinline__ void een_bal_dispatch_fold(int wave, const d4 (&pa)[5
   }
 }
 
-template <int C>
+// VAL: the values alone (the value component of the products only, J folded alone, no GL
+// written); the value products stay zero in the other components, so J is the full
+// evaluation's bit for bit.
+template <int C, bool VAL>
 __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasOut& o, const double* __restrict__ R0,
                                                  const double* __restrict__ gN, const double* __restrict__ Sg,
                                                  double* __restrict__ Jt, int ntiles, int JS, const int64_t b) {
   static_assert(C == 4 || C == 5, "balanced pass-based kernel: cord 4 and 5");
   constexpr int K1 = C - 1;
+  constexpr int NC = VAL ? 1 : 5;
   constexpr int NWV = bal_waves(C);
   constexpr int NT = 64 * NWV;
   constexpr int DIM = Lkm<C>::dim;
@@ -2254,8 +2260,8 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
       const bool split = ntile == 1 && split_ok, helper = split && g_t == 1;
       if (g_t < ntile || helper) {
         const int ns = jn4 >> 2, h = split ? ns / 2 : ns;
-        een_bal_pass(pa, pb, g_two, g_k, g_q, gch + (helper ? 0 : g_t) * L.JSP * 16, pdG, pdR, xj, xi, L.JSP,
-                     helper ? h : 0, helper ? ns : h, li, lg);
+        een_bal_pass<NC>(pa, pb, g_two, g_k, g_q, gch + (helper ? 0 : g_t) * L.JSP * 16, pdG, pdR, xj, xi, L.JSP,
+                         helper ? h : 0, helper ? ns : h, li, lg);
       }
       if (pass + 1 < npass) __syncthreads();  // before gch of the next pass is written
     }
@@ -2270,7 +2276,7 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
         __syncthreads();  // every product done; the previous round's slots read
         if (mine && g_t == 1) {
 #pragma unroll
-          for (int c = 0; c < 5; ++c)
+          for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               slot[(c * 4 + r) * 64] = pa[c][r];
@@ -2280,7 +2286,7 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
         __syncthreads();
         if (mine && g_t == 0) {
 #pragma unroll
-          for (int c = 0; c < 5; ++c)
+          for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               pa[c][r] += slot[(c * 4 + r) * 64];
@@ -2298,13 +2304,14 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
       if (i < Ne) een_bal_dispatch_fold<C>(wave, pa, pb, r, il, li, a0, Nn, ntile, ep, cfl, r0t, st, kee, out);
       // sum over the 16 nuclei of the tile(s): lanes li = 0..15 of this row group
 #pragma unroll
-      for (int c = 0; c < 5; ++c) out[c] = row_sum16(out[c]);
+      for (int c = 0; c < NC; ++c) out[c] = row_sum16(out[c]);
       if (li == 0)
 #pragma unroll
-        for (int c = 0; c < 5; ++c) red[(wave * EM + il) * 5 + c] = out[c];
+        for (int c = 0; c < NC; ++c) red[(wave * EM + il) * 5 + c] = out[c];
     }
     __syncthreads();
-    for (int t = tid; t < EM * 5; t += NT) {
+    for (int u = tid; u < EM * NC; u += NT) {
+      const int t = VAL ? 5 * u : u;  // entry (il, c) of the row sums
       double x = tot[t];
 #pragma unroll
       for (int wv = 0; wv < NWV; ++wv) x += red[wv * EM * 5 + t];
@@ -2313,11 +2320,12 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
     // the next tile pair rewrites cfl, st, ep and gch only after this barrier
     __syncthreads();
   }
-  for (int t = tid; t < EM * 5; t += NT) {
+  for (int u = tid; u < EM * NC; u += NT) {
+    const int t = VAL ? 5 * u : u;
     const int il = t / 5, c = t - 5 * il, i = i0 + il;
     if (c == 0)
       jrow[il] = (i < Ne) ? tot[t] : 0.0;
-    else if (i < Ne)
+    else if (!VAL && i < Ne)
       o.feen_gl[w * 4 * Ne + (int64_t)(c - 1) * Ne + i] = tot[t];
   }
   __syncthreads();
@@ -2328,12 +2336,12 @@ __device__ __forceinline__ void jas_een_bal_body(const JasDevView& v, const JasO
   }
 }
 
-template <int C>
+template <int C, bool VAL>
 __global__ void __launch_bounds__(64 * bal_waves(C)) k_jas_een_bal(JasDevView v, JasOut o, const double* __restrict__ R0,
                                                               const double* __restrict__ gN,
                                                               const double* __restrict__ Sg, double* __restrict__ Jt,
                                                               int ntiles, int JS) {
-  jas_een_bal_body<C>(v, o, R0, gN, Sg, Jt, ntiles, JS, blockIdx.x);
+  jas_een_bal_body<C, VAL>(v, o, R0, gN, Sg, Jt, ntiles, JS, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2385,7 +2393,7 @@ __global__ void __launch_bounds__(64 * bal_waves(C)) k_jas_een_bal_batch(const J
                                                                     const int* __restrict__ starts, int njobs) {
   const int m = job_of(starts, njobs, blockIdx.x);
   const JasJob& J = jobs[m];
-  jas_een_bal_body<C>(J.v, J.o, J.R0, J.gN, J.Sg, J.Jt, J.ntiles, J.JC, (int64_t)blockIdx.x - starts[m]);
+  jas_een_bal_body<C, false>(J.v, J.o, J.R0, J.gN, J.Sg, J.Jt, J.ntiles, J.JC, (int64_t)blockIdx.x - starts[m]);
 }
 
 // The persistent resident kernel over several contexts: each context gets its own share of
@@ -2658,7 +2666,6 @@ __global__ void k_en_distance(const double* __restrict__ coord, int64_t P, int64
 
 // Scratch of the Jastrow kernels: gN, R0, Jt and Sg, each starting on a 256-byte boundary
 // (the e-e-n kernel copies gN into LDS 16 bytes per lane).
-bool een_fused(const JasDevView& v);
 
 // Regions of the Jastrow scratch, in doubles, each 256-byte aligned.  The pair kernel's
 // outputs (gN, R0, Sg) exist only when it runs; the resident e-e-n kernel, which does its
@@ -2767,7 +2774,7 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
         ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_res<CC, false>),                     \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
       if (ea == hipSuccess && (CC == 4 || CC == 5))                                                            \
-        ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_bal<CC == 4 ? 4 : 5>),                \
+        ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_bal<CC == 4 ? 4 : 5, false>),         \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
       if (ea != hipSuccess) return ea;                                                                         \
       attr_set = true;                                                                                         \
@@ -2776,8 +2783,8 @@ hipError_t launch_jastrow_een(const JasDevView& v, const JasOut& o, hipStream_t 
       hipLaunchKernelGGL((k_jas_een_res<CC, false>), dim3(een_res_grid(v.walk_num * sh.ntiles)), dim3(64 * EQ_WAVES), \
                          sh.lds, s, v, o, sv.Jt, sv.JtP, sh.ntiles);                                           \
     else if (sh.bal)                                                                                           \
-      hipLaunchKernelGGL(k_jas_een_bal<CC == 4 ? 4 : 5>, grid, dim3(64 * bal_waves(CC)), sh.lds, s, v, o, sv.R0, \
-                         sv.gN, sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                \
+      hipLaunchKernelGGL((k_jas_een_bal<CC == 4 ? 4 : 5, false>), grid, dim3(64 * bal_waves(CC)), sh.lds, s, v, o, \
+                         sv.R0, sv.gN, sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                         \
     else                                                                                                       \
       hipLaunchKernelGGL(k_jas_een_mfma<CC>, grid, dim3(64 * EenWaves<CC>::n), sh.lds, s, v, o, sv.R0, sv.gN,    \
                          sv.Sg, sv.Jt, sh.ntiles, sh.JC);                                                       \
@@ -2825,13 +2832,46 @@ hipError_t launch_jastrow_total(const JasDevView& v, const JasOut& o, hipStream_
   return hipGetLastError();
 }
 
-bool jastrow_values_only_ok(const JasDevView& v) { return een_fused(v); }
+bool jastrow_values_only_ok(const JasDevView& v) {
+  if (een_fused(v)) return true;
+  if (!(v.cord >= 2 && v.dim_c > 0)) return false;
+  const EenShape sh = een_shape(v);
+  return sh.mfma && sh.bal;
+}
 
 hipError_t launch_jastrow_values(const JasDevView& v, const JasOut& o, hipStream_t s) {
-  if (!een_fused(v)) return hipErrorInvalidValue;
+  if (!jastrow_values_only_ok(v)) return hipErrorInvalidValue;
   const ScratchView sv = scratch_view(v, o);
   const EenShape sh = een_shape(v);
   if (sh.lds > 160 * 1024) return hipErrorInvalidValue;
+  if (!een_fused(v)) {
+    // the pair kernel (e-e, e-n terms with their GL, the e-e-n inputs), the balanced e-e-n
+    // kernel on the values, the totals without GL
+    hipError_t e = launch_jastrow_pair(v, o, s);
+    if (e != hipSuccess) return e;
+#define EEN_BAL_VAL(CC)                                                                                        \
+  {                                                                                                            \
+    static bool attr_set = false;                                                                              \
+    if (!attr_set) {                                                                                           \
+      const hipError_t ea = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jas_een_bal<CC, true>),       \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);       \
+      if (ea != hipSuccess) return ea;                                                                         \
+      attr_set = true;                                                                                         \
+    }                                                                                                          \
+    hipLaunchKernelGGL((k_jas_een_bal<CC, true>), dim3((unsigned)(v.walk_num * sh.ntiles)),                    \
+                       dim3(64 * bal_waves(CC)), sh.lds, s, v, o, sv.R0, sv.gN, sv.Sg, sv.Jt, sh.ntiles, sh.JC); \
+  }
+    if (v.cord == 4)
+      EEN_BAL_VAL(4)
+    else
+      EEN_BAL_VAL(5)
+#undef EEN_BAL_VAL
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_jas_total, dim3((unsigned)v.walk_num), dim3(kBlock), 0, s, v, o, sv.Jt, sv.JtP, sh.ntiles, 1,
+                       0, 1);
+    return hipGetLastError();
+  }
   switch (v.cord) {
 #define EEN_VAL_CASE(CC)                                                                                       \
   case CC: {                                                                                                   \

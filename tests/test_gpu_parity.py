@@ -292,13 +292,15 @@ VALUE_GETTERS = ["qmckl_get_jastrow_champ_factor_ee", "qmckl_get_jastrow_champ_f
 @pytest.mark.parametrize("nn,up,dn,aord,bord,cord", [
     (20, 19, 18, 5, 5, 2), (20, 19, 18, 3, 4, 3), (17, 24, 16, 5, 5, 4), (33, 40, 39, 2, 6, 6),
     (25, 30, 29, 5, 5, 5), (3, 3, 2, 5, 5, 5), (18, 21, 20, 5, 5, 3),
-    (40, 120, 120, 5, 5, 5), (20, 19, 18, 5, 5, 7)])
+    (37, 180, 170, 5, 5, 5), (50, 200, 190, 4, 4, 4), (40, 120, 120, 5, 5, 5), (20, 19, 18, 5, 5, 7)])
 def test_jastrow_values_only(nn, up, dn, aord, bord, cord):
     """The value and factor getters with no current GL run the values-only evaluation where
-    the LDS-resident e-e-n kernel applies (the value component of the products alone, the
-    fold of J alone): the three factors and e^J equal those of the full evaluation bit for
-    bit, through the host and the device getters, and the oracle's to 1e-12.  240 electrons
-    and cord 7 (no resident kernel) take them from the full evaluation."""
+    the LDS-resident e-e-n kernel applies, and the balanced pass-based one (350 electrons at
+    cord 5, with an odd number of nucleus tiles, and 390 at cord 4): the value component of
+    the products alone, the fold of J alone.  The three factors and e^J equal those of the
+    full evaluation bit for bit, through the host and the device getters, and the oracle's
+    to 1e-12.  240 electrons (the one-wave-per-unit pass-based kernel) and cord 7 take them
+    from the full evaluation."""
     s, x, W = S.jastrow_workload(nn, up, dn, aord, bord, cord, walk_num=8 if up + dn < 200 else 2)
     ctx = make_ctx(s, x, W, mo=False)
     dev = q.DeviceBuffer(8 * W)
