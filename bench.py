@@ -201,7 +201,18 @@ def main():
         mo_rel.append(float((d[big] / np.abs(ref_c[big])).max()))
         log(f"MO error component {c}: {mo_err[-1]:.3e} of max, {mo_rel[-1]:.3e} per entry")
     del a, mo64
-    dJ = float(np.abs(jastrow_sum() - J64).max())
+    # e^J and its GL as the step left them: at the medium workload's recipe parameters J is
+    # several hundred, so e^J reaches ~1e300 and some GL entries overflow (the reference's
+    # arithmetic gives the same); parity is asserted on J and the factor GLs
+    ev, eg = val.to_array((W,)), gl.to_array((W, 4, Ne))
+    J_now = jastrow_sum()
+    expj = {"J_min": float(J_now.min()), "J_max": float(J_now.max()),
+            "value_nonfinite": int((~np.isfinite(ev)).sum()), "gl_nonfinite": int((~np.isfinite(eg)).sum()),
+            "gl_entries": int(eg.size),
+            "note": "e^J = exp(J) overflows FP64 range near J = 709; the workload recipe's random "
+                    "parameters put J at several hundred, so some GL entries of e^J are inf"}
+    del ev, eg
+    dJ = float(np.abs(J_now - J64).max())
     dEL = float(np.abs(kinetic_surrogate() - K64).max())
     ao_density = ao_nonzero_density(q, S, s, x, W)
 
@@ -225,6 +236,10 @@ def main():
     roof["kernel"] = dom
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = measured_traffic(args.workload, W, args.mode, dom)
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = ("profiles/hbm_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                  "passes of this kernel at this workload, each scaled by its reading for "
+                                  "a copy of known size (not measured in this run)")
 
     # The drop-in boundary with host buffers: the same step through the reference's
     # getters (coordinates from and results to host memory, over PCIe).  Reported beside
@@ -288,6 +303,7 @@ def main():
         "jastrow_max_abs_dJ_vs_fp64": dJ,
         "local_energy_surrogate_max_abs_dE_vs_fp64": dEL,
         "ao_nonzero_density": ao_density,
+        "exp_J": expj,
         "host_buffer_path": {"value": W / th, "ms_per_step": th * 1e3,
                              "note": "same step through the host-pointer getters (PCIe-inclusive)"},
         "roofline": roof,

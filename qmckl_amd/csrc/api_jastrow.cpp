@@ -121,6 +121,7 @@ JasDevView jas_view(Context* c) {
   v.bord = c->bord;
   v.cord = c->cord;
   v.dim_c = c->dim_c;
+  v.type_num = c->type_nucl_num;
   v.spin_independent = c->spin_independent;
   v.rescale_ee = c->rescale_ee;
   v.coord = c->d_coord.as<double>();

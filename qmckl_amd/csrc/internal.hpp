@@ -381,6 +381,7 @@ hipError_t launch_to_f32(const double* src, int64_t n, float* dst, hipStream_t s
 
 struct JasDevView {
   int64_t walk_num, elec_num, up_num, nucl_num, aord, bord, cord, dim_c;
+  int64_t type_num;  // nucleus types (rows of a_vector)
   int32_t spin_independent;
   double rescale_ee;
   const double* coord;  // [3][W*Ne]

@@ -206,11 +206,11 @@ __device__ __forceinline__ void poly2_horner(const double* c, int n, double x, d
     p = p * x;
   }
 }
+// b: the b_vector (in global memory or its LDS copy)
 template <int K1>
-__device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, double dx, double dy, double dz,
-                                        double kee, double kinv, EeAcc& s, double (&r0)[K1 > 0 ? K1 : 1][5],
-                                        double& ge, double& ri) {
-  const double* b = v.b_vector;
+__device__ __forceinline__ void ee_pair(const JasDevView& v, const double* b, int i, int j, double dx, double dy,
+                                        double dz, double kee, double kinv, EeAcc& s,
+                                        double (&r0)[K1 > 0 ? K1 : 1][5], double& ge, double& ri) {
   const int bord = (int)v.bord, up = (int)v.up_num;
   const bool use = j != i;
   double d = 1.0, rinv = 1.0;
@@ -276,17 +276,15 @@ __device__ __forceinline__ void ee_pair(const JasDevView& v, int i, int j, doubl
 struct EnAcc {
   double fen = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;
 };
-__device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
-                                        double rinv, double e, EnAcc& s) {
-  const int aord = (int)v.aord;
-  const double* av = v.a_vector + t * (aord + 1);
-  const double ka = v.kappa_en[t];
-  const double x = (1.0 - e) * (1.0 / ka);
+// Parameters of the type: av = a_vector row, ka = kappa_en, kinv = 1 / ka, asymp = asymp_jasa.
+__device__ __forceinline__ void en_pair_p(const double* av, int aord, double ka, double kinv, double asymp, double dx,
+                                          double dy, double dz, double d, double rinv, double e, EnAcc& s) {
+  const double x = (1.0 - e) * kinv;
   double pq, dq, hq;  // sum_{p>=2} a_p x^p, its derivative, half its second derivative
   poly2_horner(av, aord, x, pq, dq, hq);
   // one reciprocal of the Pade denominator for the value and the derivatives
   const double invd = 1.0 / (1.0 + av[1] * x), invd2 = invd * invd;
-  s.fen += av[0] * x * invd - v.asymp_jasa[t];
+  s.fen += av[0] * x * invd - asymp;
   s.fen += pq;
   const double dd = d > 1.e-20 ? d : 1.e-20;
   const double dinv = d > 1.e-20 ? rinv : 1.e20;  // 1 / dd
@@ -303,6 +301,12 @@ __device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, d
   s.h1 += dq * gy;
   s.h2 += dq * gz;
   s.h3 += dq * gl4 + 2.0 * hq * gc2;
+}
+__device__ __forceinline__ void en_pair(const JasDevView& v, int t, double dx, double dy, double dz, double d,
+                                        double rinv, double e, EnAcc& s) {
+  const int aord = (int)v.aord;
+  const double ka = v.kappa_en[t];
+  en_pair_p(v.a_vector + t * (aord + 1), aord, ka, 1.0 / ka, v.asymp_jasa[t], dx, dy, dz, d, rinv, e, s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -338,7 +342,7 @@ __device__ __forceinline__ void jas_pair_body(const JasDevView& v, const JasOut&
       for (int c = 0; c < 5; ++c) r0[k][c] = 0.0;
     for (int j = 0; j < Ne; ++j) {
       double ge, ri;
-      ee_pair<K1>(v, i, j, xi - xe[3 * j], yi - xe[3 * j + 1], zi - xe[3 * j + 2], kee, kinv, se, r0, ge, ri);
+      ee_pair<K1>(v, v.b_vector, i, j, xi - xe[3 * j], yi - xe[3 * j + 1], zi - xe[3 * j + 2], kee, kinv, se, r0, ge, ri);
     }
     fee += se.fee;
     double* og = o.fee_gl + w * 4 * Ne;
@@ -1040,9 +1044,17 @@ constexpr int JREC = 36;  // doubles per electron-j record: G[16], R[16], x, y, 
 
 // Dynamic LDS of k_jas_een_res, in doubles, every array on a 16-byte boundary.  JP = Ne
 // rounded up to 4, NA16 = 16 x number of nucleus tiles.
+// Doubles of the pair parameters the resident kernel keeps in LDS (EenResLds::prm): b_vector
+// [bord + 1], a_vector [type][aord + 1], per type kappa_en, 1 / kappa_en and asymp_jasa
+// [3][type], and the type of each nucleus of the padded table [NA16].
+__host__ __device__ inline int een_prm_doubles(const JasDevView& v, int NA16) {
+  const int nt = v.type_num > 0 ? (int)v.type_num : 0;
+  return (int)v.bord + 1 + nt * ((int)v.aord + 4) + NA16;
+}
+
 struct EenResLds {
-  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, pee, own, total;
-  __host__ __device__ EenResLds(int JP, int NA16, int C, int dim) {
+  int xt, jr, gch, nuc, enr, r0t, st, cfl, red, pee, own, prm, total;
+  __host__ __device__ EenResLds(int JP, int NA16, int C, int dim, int nprm) {
     int o = 0;
     auto take = [&o](int n) { const int at = o; o = (o + n + 1) & ~1; return at; };
     xt = take(EM * 3);                 // tile electrons i
@@ -1058,6 +1070,7 @@ struct EenResLds {
     int ngroup = 0;                    // groups of units per nucleus tile
     for (int k = 1; k < C; ++k) ngroup += (C - k + 1) / 2;
     own = take(ngroup * (NA16 / 16));  // wave of each group, as int: full tiles, then tiles of <= 8 rows
+    prm = take(nprm);                  // pair parameters (een_prm_doubles)
     total = o;
   }
 };
@@ -1576,9 +1589,24 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
     // 4.5 cycles per vector instruction, plus its fold.  A second table serves the electron
     // tiles of at most 8 rows, whose full nucleus tiles run on the 4x4x4 products by rows.
     const int Nn = (int)v.nucl_num, NA = gn_na(Nn), NA16 = 16 * NA;
-    const EenResLds L(gn_jp((int)v.elec_num), NA16, C, DIM);
+    const EenResLds L(gn_jp((int)v.elec_num), NA16, C, DIM, een_prm_doubles(v, NA16));
     lds_copy4(lds + L.nuc, v.nuc_pad, 4 * NA16, wave, threadIdx.x & 63);
     lds_copy4(lds + L.cfl, v.cfull_pad, DIM * NA16, wave, threadIdx.x & 63);
+    {
+      // the pair parameters, read by every pair of every tile: from LDS, the reads of the
+      // Horner sums issue without waiting on global memory
+      double* prm = lds + L.prm;
+      const int nb = (int)v.bord + 1, na1 = (int)v.aord + 1, nt = v.type_num > 0 ? (int)v.type_num : 0;
+      for (int t = threadIdx.x; t < nb; t += NT) prm[t] = v.b_vector[t];
+      for (int t = threadIdx.x; t < nt * na1; t += NT) prm[nb + t] = v.a_vector[t];
+      for (int t = threadIdx.x; t < nt; t += NT) {
+        const double ka = v.kappa_en[t];
+        prm[nb + nt * na1 + t] = ka;
+        prm[nb + nt * (na1 + 1) + t] = 1.0 / ka;
+        prm[nb + nt * (na1 + 2) + t] = v.asymp_jasa[t];
+      }
+      for (int a = threadIdx.x; a < NA16; a += NT) prm[nb + nt * (na1 + 3) + a] = a < Nn ? (double)v.type_nucl[a] : 0.0;
+    }
     if (threadIdx.x == 0) {
       static_assert(EQ_WAVES == 8, "two waves per SIMD: waves w and w + 4");
       const bool mini = Nn - 16 * (NA - 1) <= 4, mini2 = Nn - 16 * (NA - 1) <= 2;
@@ -1637,7 +1665,7 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
     asm volatile("" : "+s"(Ne), "+s"(Nn), "+v"(tidl));
     const int tid = tidl, lane = tid & 63, li = lane & 15, lg = lane >> 4;
     const int JP = gn_jp(Ne), NA = gn_na(Nn), NA16 = 16 * NA;
-    const EenResLds L(JP, NA16, C, DIM);
+    const EenResLds L(JP, NA16, C, DIM, een_prm_doubles(v, NA16));
     double* xt = lds + L.xt;
     double* jr = lds + L.jr;
     double* gch = lds + L.gch;
@@ -1716,6 +1744,9 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
       asm volatile("" : "+v"(tp), "+s"(kee));
       const int il = 2 * wave + ((tp & 63) >> 5), l32 = tp & 31, i = i0 + il;
       const double kinv = 1.0 / kee;
+      // the pair parameters from their LDS copy
+      const double* prm = lds + L.prm;
+      const int nb = (int)v.bord + 1, aord = (int)v.aord, nt = v.type_num > 0 ? (int)v.type_num : 0;
       const double xi = xt[il * 3], yi = xt[il * 3 + 1], zi = xt[il * 3 + 2];
       EeAcc se;
       EnAcc sn;
@@ -1730,11 +1761,11 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
         double ge0 = 0.0, ri0 = 0.0, ge1 = 0.0, ri1 = 0.0;
         if (i < Ne && j0 < Ne) {
           const double* pj = jr + j0 * JREC + 32;
-          ee_pair<K1>(v, i, j0, xi - pj[0], yi - pj[1], zi - pj[2], kee, kinv, se, r0, ge0, ri0);
+          ee_pair<K1>(v, prm, i, j0, xi - pj[0], yi - pj[1], zi - pj[2], kee, kinv, se, r0, ge0, ri0);
         }
         if (i < Ne && j1 < Ne) {
           const double* pj = jr + j1 * JREC + 32;
-          ee_pair<K1>(v, i, j1, xi - pj[0], yi - pj[1], zi - pj[2], kee, kinv, se, r0, ge1, ri1);
+          ee_pair<K1>(v, prm, i, j1, xi - pj[0], yi - pj[1], zi - pj[2], kee, kinv, se, r0, ge1, ri1);
         }
         jr[j0 * JREC + il] = ge0;
         jr[j0 * JREC + 16 + il] = ri0;
@@ -1749,7 +1780,9 @@ __device__ __forceinline__ void jas_een_res_body(const JasDevView& v, const JasO
           const double dx = xi - nuc[a], dy = yi - nuc[NA16 + a], dz = zi - nuc[2 * NA16 + a];
           double d;
           dist_and_inverse(dx * dx + dy * dy + dz * dz, d, rinv);
-          en_pair(v, (int)v.type_nucl[a], dx, dy, dz, d, rinv, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
+          const int t = (int)prm[nb + nt * (aord + 4) + a];
+          en_pair_p(prm + nb + t * (aord + 1), aord, prm[nb + nt * (aord + 1) + t], prm[nb + nt * (aord + 2) + t],
+                    prm[nb + nt * (aord + 3) + t], dx, dy, dz, d, rinv, gch[((a >> 4) * JP + i) * 16 + (a & 15)], sn);
         }
         enr[il * NA16 + a] = rinv;
       }
@@ -2433,7 +2466,8 @@ EenShape een_shape(const JasDevView& v) {
     const int C = (int)v.cord;
     s.TI = EM;
     const int ne4 = (Ne + 3) & ~3;
-    const size_t res_lds0 = sizeof(double) * (size_t)EenResLds(gn_jp(Ne), 16 * gn_na(Nn), C, (int)v.dim_c).total;
+    const size_t res_lds0 =
+        sizeof(double) * (size_t)EenResLds(gn_jp(Ne), 16 * gn_na(Nn), C, (int)v.dim_c, een_prm_doubles(v, 16 * gn_na(Nn))).total;
     // the balanced variant where k_jas_een_mfma needs more than one pass over j as well (it
     // holds fewer electrons per pass: with one pass, about 230-300 electrons at C = 5, the
     // one-wave-per-unit kernel builds the tile's pair records once and is as fast or faster)
