@@ -482,6 +482,39 @@ qmckl_exit_code qmckl_get_potential_energy(qmckl_context context, double* const 
 qmckl_exit_code qmckl_get_local_energy(qmckl_context context, double* const local_energy, const int64_t size_max);
 qmckl_exit_code qmckl_get_drift_vector(qmckl_context context, double* const drift_vector);
 
+/* ---- BLAS-like operations (org/qmckl_blas.org).  Column-major (Fortran-order) operands, as the
+ * reference's; host pointers; computed on the context's GPU.
+ * qmckl_dgemm (:984-999): C = alpha op(A) op(B) + beta C, op = 'N' or 'T', C m x n; lda, ldb,
+ *   ldc at least the rows of A, B, C as stored; beta = 0 does not read C.
+ * qmckl_dgemv (:3210-3225): Y = alpha op(A) X + beta Y, A m x n, strides incx, incy > 0.
+ * qmckl_dger (:3340-3352): A = alpha X Y^T + A, A m x n.
+ * qmckl_adjugate (:1710-1718): B = adj(A) of the n x n matrix A, *det_l = det(A); orders up to
+ *   5 by cofactors (singular matrices included), above through the inverse (a singular A
+ *   there returns QMCKL_FAILURE with *det_l = 0 and B unchanged). */
+qmckl_exit_code qmckl_dgemm(const qmckl_context context, const char TransA, const char TransB, const int64_t m,
+                            const int64_t n, const int64_t k, const double alpha, const double* A, const int64_t lda,
+                            const double* B, const int64_t ldb, const double beta, double* const C,
+                            const int64_t ldc);
+qmckl_exit_code qmckl_dgemv(const qmckl_context context, const char Trans, const int64_t m, const int64_t n,
+                            const double alpha, const double* A, const int64_t lda, const double* X,
+                            const int64_t incx, const double beta, double* const Y, const int64_t incy);
+qmckl_exit_code qmckl_dger(const qmckl_context context, const int64_t m, const int64_t n, const double alpha,
+                           const double* X, const int64_t incx, const double* Y, const int64_t incy, double* const A,
+                           const int64_t lda);
+qmckl_exit_code qmckl_adjugate(const qmckl_context context, const int64_t n, const double* A, const int64_t lda,
+                               double* const B, const int64_t ldb, double* det_l);
+/* The size-safe proxies (:1276-1294, :2595-2605): the array sizes (in elements) are checked
+ * against what the operation spans, then as qmckl_dgemm / qmckl_adjugate, the error codes
+ * numbered by these functions' own argument positions. */
+qmckl_exit_code qmckl_dgemm_safe(const qmckl_context context, const char TransA, const char TransB, const int64_t m,
+                                 const int64_t n, const int64_t k, const double alpha, const double* A,
+                                 const int64_t size_max_A, const int64_t lda, const double* B, const int64_t size_max_B,
+                                 const int64_t ldb, const double beta, double* const C, const int64_t size_max_C,
+                                 const int64_t ldc);
+qmckl_exit_code qmckl_adjugate_safe(const qmckl_context context, const int64_t n, const double* A,
+                                    const int64_t size_max_A, const int64_t lda, double* const B,
+                                    const int64_t size_max_B, const int64_t ldb, double* det_l);
+
 /* ---- Sherman-Morrison-Woodbury (org/under_development/qmckl_sherman_morrison_woodbury.org:
  * 299-347, 980-1036, 1623-1668, 2269-2314, 2896-2945).  Slater_inv [Dim][LDS]; Updates
  * [N_updates][LDS]; Updates_index 1-based: update l adds Updates[l] to column

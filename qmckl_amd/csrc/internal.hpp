@@ -208,6 +208,7 @@ struct Context {
   DevBuf d_pot_charge;
   uint64_t pot_charge_date = 0;
   DevBuf d_smw;  // Sherman-Morrison-Woodbury staging and scratch
+  DevBuf d_blas;  // operands of the BLAS-like entry points
 
   // MO cusp correction, org/qmckl_mo.org:479-700: r_cusp [Nn] (empty: none), cusp_param
   // [Nn][4][mo]; device: r_cusp, params, the s-AO table of each nucleus
@@ -534,6 +535,19 @@ struct SlaterArgs {
   double* kin;           // [W]: -1/2 sum_e lap_e D / D of determinant 0 (with drift), or null
 };
 hipError_t launch_slater(const SlaterArgs& a, int64_t n_mat, hipStream_t s);
+// adjugate B = det(A) A^-1 of one column-major n x n matrix; det(A) to *det (0: singular, B
+// untouched); work: n * n doubles when n > 128
+hipError_t launch_adjugate_gj(int n, const double* A, int64_t lda, double* B, int64_t ldb, double* det, double* work,
+                              hipStream_t s);
+// column-major BLAS-like operations (kernels_blas.hip)
+hipError_t launch_dgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
+hipError_t launch_dgemv(bool t, int64_t m, int64_t n, double alpha, const double* A, int64_t lda, const double* x,
+                        int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
+hipError_t launch_dger(int64_t m, int64_t n, double alpha, const double* x, int64_t incx, const double* y,
+                       int64_t incy, double* A, int64_t lda, hipStream_t s);
+hipError_t launch_adjugate_small(int n, const double* A, int64_t lda, double* B, int64_t ldb, double* det,
+                                 hipStream_t s);
 // out [det][W][5][n][n]: component c of MO idx[m] at electron e, [e][m] inner
 hipError_t launch_slater_vgl(const SlaterArgs& a, int64_t n_mat, double* out, hipStream_t s);
 // out[m] = value[m / nn] * inv[m] for m < count (adjugates of n x n matrices, nn = n * n)

@@ -437,6 +437,30 @@ __global__ void __launch_bounds__(kDetBlock) k_smw(SmwArgs a) {
   if (threadIdx.x == 0 && a.det) a.det[m] *= factor;
 }
 
+// The adjugate of one n x n matrix through its inverse, B = det(A) A^-1 (the general case
+// of qmckl_adjugate, org/qmckl_blas.org:1785-1790 with :2200-2260), column-major A (lda) and
+// B (ldb); det(A) to *det_out.  A singular A (a zero pivot) writes det 0 and leaves B.
+template <bool IN_LDS>
+__global__ void __launch_bounds__(kDetBlock) k_adjugate_gj(const double* A0, int n, int64_t lda, double* B,
+                                                           int64_t ldb, double* det_out, double* work) {
+  extern __shared__ double smem[];
+  const int ld = IN_LDS ? (n | 1) : n;
+  double* A = IN_LDS ? smem : work;
+  double* rowbuf = IN_LDS ? smem + n * ld : smem;
+  double* col = rowbuf + n;
+  double* sv = col + n;
+  int* si = reinterpret_cast<int*>(sv + kDetWaves);
+  int* perm = si + kDetWaves;
+  for (int j = threadIdx.x >> 6; j < n; j += kDetWaves)
+    for (int i = threadIdx.x & 63; i < n; i += 64) A[i * ld + j] = A0[i + j * lda];
+  __syncthreads();
+  const double det = gauss_jordan(A, n, ld, rowbuf, col, perm, sv, si);
+  if (threadIdx.x == 0) *det_out = det;
+  if (det == 0.0) return;
+  for (int j = threadIdx.x >> 6; j < n; j += kDetWaves)
+    for (int i = threadIdx.x & 63; i < n; i += 64) B[i + j * ldb] = det * A[i * ld + j];
+}
+
 }  // namespace
 
 static size_t slater_lds_bytes(int n, bool in_lds) {
@@ -495,6 +519,21 @@ hipError_t launch_smw(const SmwArgs& a, hipStream_t s) {
   } else {
     if (a.work == nullptr) return hipErrorInvalidValue;
     k_smw<false><<<dim3((unsigned)a.n_mat), dim3(kDetBlock), lds, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adjugate_gj(int n, const double* A, int64_t lda, double* B, int64_t ldb, double* det, double* work,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const bool in_lds = n <= kDetLdsMax;
+  const size_t lds = slater_lds_bytes(n, in_lds);
+  if (in_lds) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_adjugate_gj<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    k_adjugate_gj<true><<<dim3(1), dim3(kDetBlock), lds, s>>>(A, n, lda, B, ldb, det, work);
+  } else {
+    k_adjugate_gj<false><<<dim3(1), dim3(kDetBlock), lds, s>>>(A, n, lda, B, ldb, det, work);
   }
   return hipGetLastError();
 }
